@@ -1,0 +1,363 @@
+"""Python host mirror of libbcmhip (the C ABI in include/bcmhip.h).
+
+The classes follow Bslib's objects: :class:`Matrix` is a ``CSMatrixRS``
+(smatrix.h:1040-1048), :class:`Factor` the IC2 / ILU2 factors that
+``CSMatrixRS::Ich2`` (ich2rs.cpp:1075) and ``CSMatrixRS::Ilu2``
+(ilu2rs.cpp:1458) produce, :class:`Params` the solve-path fields of
+``CSlvParam`` (slvparam.h:19-214). Device vectors are passed as raw HIP
+pointers; torch tensors on ``cuda`` are accepted for convenience (torch is
+only used for device memory here).
+
+The library must be built (``make -C bcm-problems_amd``); nothing here falls
+back to a CPU path: device calls raise :class:`BcmError` when no GPU is
+present.
+"""
+
+from __future__ import annotations
+
+import ctypes as C
+import os
+from typing import Optional, Sequence
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(os.path.dirname(_HERE), "lib", "libbcmhip.so")
+
+OK = 0
+E_ARG, E_NOT_SPD, E_WORKSPACE, E_BREAKDOWN, E_HIP, E_NOCONV, E_NODEVICE, E_INTERNAL = range(1, 9)
+PCG, BICGSTAB, FGMRES, LANCZOS = 0, 1, 2, 3
+F64, F32, BF16 = 0, 1, 2
+
+
+class BcmError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"[{code}] {msg}")
+        self.code = code
+
+
+class Params(C.Structure):
+    _fields_ = [
+        ("msglev", C.c_int), ("ordtype", C.c_int), ("memory", C.c_double),
+        ("tau1", C.c_double), ("tau2", C.c_double), ("theta", C.c_double),
+        ("sclmin", C.c_double), ("pivmin", C.c_double),
+        ("ittype", C.c_int), ("sttype", C.c_int), ("niter", C.c_int), ("ichk", C.c_int),
+        ("eps", C.c_double),
+        ("nitperf", C.c_int), ("resfin", C.c_double), ("timefct", C.c_double),
+        ("timeitr", C.c_double), ("density", C.c_double), ("density2", C.c_double),
+    ]
+
+
+class SolveOpts(C.Structure):
+    _fields_ = [
+        ("kind", C.c_int), ("niter", C.c_int), ("eps", C.c_double), ("ittype", C.c_int),
+        ("check_every", C.c_int), ("restart", C.c_int), ("precision", C.c_int),
+        ("max_restarts", C.c_int),
+    ]
+
+
+class Stats(C.Structure):
+    _fields_ = [
+        ("iterations", C.c_int), ("restarts", C.c_int), ("breakdown", C.c_int), ("converged", C.c_int),
+        ("relres_true", C.c_double), ("relres_rec", C.c_double), ("seconds", C.c_double),
+    ]
+
+
+_P = C.c_void_p
+_IP = C.POINTER(C.c_int)
+_DP = C.POINTER(C.c_double)
+_CB_NSUP = C.CFUNCTYPE(None, C.c_void_p, _IP)
+_CB_SPRNDS = C.CFUNCTYPE(None, C.c_void_p, _IP)
+_CB_IAJA = C.CFUNCTYPE(None, C.c_void_p, C.POINTER(_IP), C.POINTER(_IP))
+_CB_BLOCK = C.CFUNCTYPE(None, C.c_void_p, C.c_int, _DP)
+
+# name -> (restype, argtypes); every symbol declared in include/bcmhip.h
+SIGNATURES = {
+    "bcm_version": (C.c_char_p, []),
+    "bcm_last_error": (C.c_char_p, []),
+    "bcm_params_default": (None, [C.POINTER(Params)]),
+    "bcm_solve_opts_default": (None, [C.POINTER(SolveOpts)]),
+    "bcm_matrix_create": (C.c_int, [C.c_int, _IP, _IP, _IP, _DP, C.c_int, C.POINTER(_P)]),
+    "bcm_matrix_destroy": (C.c_int, [_P]),
+    "bcm_matrix_info": (C.c_int, [_P, _IP, _IP, _IP, C.POINTER(C.c_longlong), _IP]),
+    "bcm_matrix_export": (C.c_int, [_P, _IP, _IP, _IP, _DP]),
+    "bcm_gen_heat2d": (C.c_int, [C.c_int, C.c_int, C.c_int, C.c_ulonglong, C.POINTER(_P), _DP]),
+    "bcm_gen_seepage3d": (C.c_int, [C.c_int, C.c_int, C.c_int, C.c_int, C.c_double, C.c_ulonglong,
+                                    C.POINTER(_P), _DP]),
+    "bcm_gen_elastic_cell": (C.c_int, [C.c_int, C.c_double, C.c_double, C.c_double, C.c_double,
+                                       C.c_ulonglong, C.POINTER(_P), _DP]),
+    "bcm_rcm_order": (C.c_int, [_P, _IP]),
+    "bcm_matrix_reorder": (C.c_int, [_P, _IP, C.POINTER(_P)]),
+    "bcm_vector_reorder": (C.c_int, [_P, _IP, C.c_char, _DP, C.c_int]),
+    "bcm_factor_ic2": (C.c_int, [_P, C.POINTER(Params), C.POINTER(_P)]),
+    "bcm_factor_ilu2": (C.c_int, [_P, C.POINTER(Params), C.POINTER(_P)]),
+    "bcm_factor_destroy": (C.c_int, [_P]),
+    "bcm_factor_info": (C.c_int, [_P, C.c_int, _IP, C.POINTER(C.c_longlong), _IP, _IP]),
+    "bcm_factor_export": (C.c_int, [_P, C.c_int, _IP, _IP, _DP]),
+    "bcm_device_count": (C.c_int, [_IP]),
+    "bcm_context_create": (C.c_int, [C.POINTER(_P)]),
+    "bcm_context_destroy": (C.c_int, [_P]),
+    "bcm_context_sync": (C.c_int, [_P]),
+    "bcm_mvm": (C.c_int, [_P, _P, C.c_void_p, C.c_void_p, C.c_int]),
+    "bcm_mvm_t": (C.c_int, [_P, _P, C.c_void_p, C.c_void_p, C.c_int]),
+    "bcm_residual": (C.c_int, [_P, _P, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int]),
+    "bcm_solve_l": (C.c_int, [_P, _P, C.c_int, C.c_void_p, C.c_void_p, C.c_int]),
+    "bcm_solve_u": (C.c_int, [_P, _P, C.c_int, C.c_void_p, C.c_void_p, C.c_int]),
+    "bcm_precond": (C.c_int, [_P, _P, C.c_void_p, C.c_void_p, C.c_int, C.c_int, C.c_int]),
+    "bcm_dot": (C.c_int, [_P, C.c_int, C.c_void_p, C.c_void_p, _DP]),
+    "bcm_solve": (C.c_int, [_P, _P, _P, C.POINTER(SolveOpts), C.c_void_p, C.c_void_p, C.POINTER(Stats)]),
+    "bcm_abstract_symm_solver": (C.c_int, [C.c_void_p, C.POINTER(Params), _CB_NSUP, _CB_SPRNDS, _CB_IAJA,
+                                           _CB_BLOCK, _CB_BLOCK, _CB_BLOCK, _CB_BLOCK]),
+    "bcm_abstract_solver": (C.c_int, [C.c_void_p, C.POINTER(Params), _CB_NSUP, _CB_SPRNDS, _CB_IAJA,
+                                      _CB_BLOCK, _CB_BLOCK, _CB_BLOCK, _CB_BLOCK]),
+    "bcm_alloc_ints": (_IP, [C.c_int]),
+}
+
+_lib: Optional[C.CDLL] = None
+
+
+def lib() -> C.CDLL:
+    """Loads libbcmhip.so (raises if it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise BcmError(E_INTERNAL, f"{LIB_PATH} is missing: build it with make -C bcm-problems_amd")
+        l = C.CDLL(LIB_PATH)
+        for name, (res, args) in SIGNATURES.items():
+            fn = getattr(l, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = l
+    return _lib
+
+
+def _check(code: int) -> None:
+    if code != OK:
+        raise BcmError(code, lib().bcm_last_error().decode(errors="replace"))
+
+
+def _ip(a: np.ndarray):
+    return a.ctypes.data_as(_IP)
+
+
+def _dp(a: np.ndarray):
+    return a.ctypes.data_as(_DP)
+
+
+def ptr(t) -> int:
+    """Raw device address of a torch tensor or an int."""
+    if isinstance(t, int):
+        return t
+    return t.data_ptr()
+
+
+def default_params(**kw) -> Params:
+    p = Params()
+    lib().bcm_params_default(C.byref(p))
+    for k, v in kw.items():
+        setattr(p, k, v)
+    return p
+
+
+def default_opts(**kw) -> SolveOpts:
+    o = SolveOpts()
+    lib().bcm_solve_opts_default(C.byref(o))
+    for k, v in kw.items():
+        setattr(o, k, v)
+    return o
+
+
+class Matrix:
+    """Host block-CSR matrix (CSMatrixRS) owned by the library."""
+
+    def __init__(self, handle: int):
+        self.h = C.c_void_p(handle)
+
+    @classmethod
+    def from_arrays(cls, sprnd, ia, ja, a, symmetric: bool) -> "Matrix":
+        sprnd = np.ascontiguousarray(sprnd, dtype=np.int32)
+        ia = np.ascontiguousarray(ia, dtype=np.int32)
+        ja = np.ascontiguousarray(ja, dtype=np.int32)
+        a = np.ascontiguousarray(a, dtype=np.float64)
+        out = C.c_void_p()
+        _check(lib().bcm_matrix_create(len(sprnd) - 1, _ip(sprnd), _ip(ia), _ip(ja), _dp(a), int(symmetric),
+                                       C.byref(out)))
+        return cls(out.value)
+
+    def __del__(self):
+        if getattr(self, "h", None) is not None and self.h.value and _lib is not None:
+            _lib.bcm_matrix_destroy(self.h)
+            self.h = None
+
+    def info(self):
+        nsup, n, nzja, sym = C.c_int(), C.c_int(), C.c_int(), C.c_int()
+        nza = C.c_longlong()
+        _check(lib().bcm_matrix_info(self.h, C.byref(nsup), C.byref(n), C.byref(nzja), C.byref(nza), C.byref(sym)))
+        return dict(nsup=nsup.value, n=n.value, nzja=nzja.value, nza=nza.value, symmetric=bool(sym.value))
+
+    @property
+    def n(self) -> int:
+        return self.info()["n"]
+
+    def arrays(self):
+        i = self.info()
+        sprnd = np.zeros(i["nsup"] + 1, np.int32)
+        ia = np.zeros(i["nsup"] + 1, np.int32)
+        ja = np.zeros(i["nzja"], np.int32)
+        a = np.zeros(i["nza"], np.float64)
+        _check(lib().bcm_matrix_export(self.h, _ip(sprnd), _ip(ia), _ip(ja), _dp(a)))
+        return sprnd, ia, ja, a
+
+    def rcm_order(self) -> np.ndarray:
+        o = np.zeros(self.info()["nsup"], np.int32)
+        _check(lib().bcm_rcm_order(self.h, _ip(o)))
+        return o
+
+    def reorder(self, order) -> "Matrix":
+        order = np.ascontiguousarray(order, dtype=np.int32)
+        out = C.c_void_p()
+        _check(lib().bcm_matrix_reorder(self.h, _ip(order), C.byref(out)))
+        return Matrix(out.value)
+
+    def reorder_vector(self, order, v: np.ndarray, inverse: bool = False) -> np.ndarray:
+        order = np.ascontiguousarray(order, dtype=np.int32)
+        v = np.array(v, dtype=np.float64, order="F", copy=True)
+        nrhs = 1 if v.ndim == 1 else v.shape[1]
+        _check(lib().bcm_vector_reorder(self.h, _ip(order), b"I" if inverse else b"D", _dp(v), nrhs))
+        return v
+
+    def dense(self) -> np.ndarray:
+        """Dense image (symmetric matrices are expanded); for small test systems."""
+        sprnd, ia, ja, a = self.arrays()
+        info = self.info()
+        n = info["n"]
+        d = np.zeros((n, n))
+        off = 0
+        for i in range(len(sprnd) - 1):
+            bi = sprnd[i + 1] - sprnd[i]
+            for j in range(ia[i], ia[i + 1]):
+                c = ja[j]
+                bj = sprnd[c + 1] - sprnd[c]
+                blk = a[off:off + bi * bj].reshape(bj, bi).T
+                d[sprnd[i]:sprnd[i + 1], sprnd[c]:sprnd[c + 1]] = blk
+                if info["symmetric"] and c != i:
+                    d[sprnd[c]:sprnd[c + 1], sprnd[i]:sprnd[i + 1]] = blk.T
+                off += bi * bj
+        return d
+
+
+def gen_heat2d(nx: int, ny: int, bla: int, seed: int):
+    rhs = np.zeros(nx * ny * bla)
+    out = C.c_void_p()
+    _check(lib().bcm_gen_heat2d(nx, ny, bla, seed, C.byref(out), _dp(rhs)))
+    return Matrix(out.value), rhs
+
+
+def gen_seepage3d(nx: int, ny: int, nz: int, bla: int, skew: float, seed: int):
+    rhs = np.zeros(nx * ny * nz * bla)
+    out = C.c_void_p()
+    _check(lib().bcm_gen_seepage3d(nx, ny, nz, bla, skew, seed, C.byref(out), _dp(rhs)))
+    return Matrix(out.value), rhs
+
+
+def gen_elastic_cell(m: int, radius: float, contrast: float, poisson: float, shift: float, seed: int):
+    rhs = np.zeros(6 * m ** 3 * 24)
+    out = C.c_void_p()
+    _check(lib().bcm_gen_elastic_cell(m, radius, contrast, poisson, shift, seed, C.byref(out), _dp(rhs)))
+    return Matrix(out.value), rhs.reshape(6, -1).T.copy(order="F")
+
+
+class Factor:
+    """IC2 (kind 0) or ILU2 (kind 1) host factor with device sweeps."""
+
+    def __init__(self, handle: int, kind: int, params: Params):
+        self.h = C.c_void_p(handle)
+        self.kind = kind
+        self.params = params
+
+    @classmethod
+    def ic2(cls, upper: Matrix, params: Optional[Params] = None) -> "Factor":
+        p = params or default_params()
+        out = C.c_void_p()
+        _check(lib().bcm_factor_ic2(upper.h, C.byref(p), C.byref(out)))
+        return cls(out.value, 0, p)
+
+    @classmethod
+    def ilu2(cls, a: Matrix, params: Optional[Params] = None) -> "Factor":
+        p = params or default_params()
+        out = C.c_void_p()
+        _check(lib().bcm_factor_ilu2(a.h, C.byref(p), C.byref(out)))
+        return cls(out.value, 1, p)
+
+    def __del__(self):
+        if getattr(self, "h", None) is not None and self.h.value and _lib is not None:
+            _lib.bcm_factor_destroy(self.h)
+            self.h = None
+
+    def info(self, which: int):
+        nzja, lf, lb = C.c_int(), C.c_int(), C.c_int()
+        nza = C.c_longlong()
+        _check(lib().bcm_factor_info(self.h, which, C.byref(nzja), C.byref(nza), C.byref(lf), C.byref(lb)))
+        return dict(nzja=nzja.value, nza=nza.value, levels_fwd=lf.value, levels_bwd=lb.value)
+
+    def arrays(self, which: int, nsup: int):
+        i = self.info(which)
+        ia = np.zeros(nsup + 1, np.int32)
+        ja = np.zeros(i["nzja"], np.int32)
+        a = np.zeros(i["nza"], np.float64)
+        _check(lib().bcm_factor_export(self.h, which, _ip(ia), _ip(ja), _dp(a)))
+        return ia, ja, a
+
+
+class Context:
+    """One HIP stream with its solver workspace."""
+
+    def __init__(self):
+        out = C.c_void_p()
+        _check(lib().bcm_context_create(C.byref(out)))
+        self.h = out
+
+    def __del__(self):
+        if getattr(self, "h", None) is not None and self.h.value and _lib is not None:
+            _lib.bcm_context_destroy(self.h)
+            self.h = None
+
+    def sync(self):
+        _check(lib().bcm_context_sync(self.h))
+
+    def mvm(self, a: Matrix, x, y, nrhs: int = 1):
+        _check(lib().bcm_mvm(self.h, a.h, ptr(x), ptr(y), nrhs))
+
+    def mvm_t(self, a: Matrix, x, y, nrhs: int = 1):
+        _check(lib().bcm_mvm_t(self.h, a.h, ptr(x), ptr(y), nrhs))
+
+    def residual(self, a: Matrix, b, x, r, nrhs: int = 1):
+        _check(lib().bcm_residual(self.h, a.h, ptr(b), ptr(x), ptr(r), nrhs))
+
+    def solve_l(self, f: Factor, which: int, x, y, nrhs: int = 1):
+        _check(lib().bcm_solve_l(self.h, f.h, which, ptr(x), ptr(y), nrhs))
+
+    def solve_u(self, f: Factor, which: int, x, y, nrhs: int = 1):
+        _check(lib().bcm_solve_u(self.h, f.h, which, ptr(x), ptr(y), nrhs))
+
+    def precond(self, f: Factor, r, z, nrhs: int = 1, precision: int = F64, transpose: bool = False):
+        _check(lib().bcm_precond(self.h, f.h, ptr(r), ptr(z), nrhs, precision, int(transpose)))
+
+    def dot(self, n: int, x, y) -> float:
+        out = C.c_double()
+        _check(lib().bcm_dot(self.h, n, ptr(x), ptr(y), C.byref(out)))
+        return out.value
+
+    def solve(self, a: Matrix, f: Factor, b, x, opts: Optional[SolveOpts] = None) -> Stats:
+        st = Stats()
+        o = opts or default_opts()
+        code = lib().bcm_solve(self.h, a.h, f.h, C.byref(o), ptr(b), ptr(x), C.byref(st))
+        if code not in (OK, E_BREAKDOWN):
+            _check(code)
+        return st
+
+
+def device_count() -> int:
+    n = C.c_int()
+    _check(lib().bcm_device_count(C.byref(n)))
+    return n.value

@@ -1,0 +1,523 @@
+// C ABI of libbcmhip (include/bcmhip.h). Every entry point converts
+// exceptions into status codes and keeps the message for bcm_last_error().
+#include "../../include/bcmhip.h"
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cstdlib>
+#include <cstring>
+#include <new>
+#include <string>
+
+#include "device/solver.hpp"
+#include "host/bcm_host.hpp"
+#include "internal.hpp"
+
+struct bcm_matrix_s {
+  bcm::Matrix m;
+};
+struct bcm_factor_s {
+  bcm::Factor f;
+};
+struct bcm_context_s {
+  bcm::Context c;
+};
+
+namespace {
+
+thread_local std::string g_err;
+
+template <class F>
+int guarded(F&& fn) {
+  try {
+    fn();
+    return BCM_OK;
+  } catch (const bcm::Error& e) {
+    g_err = e.what();
+    return e.code;
+  } catch (const std::bad_alloc&) {
+    g_err = "out of host memory";
+    return BCM_E_WORKSPACE;
+  } catch (const std::exception& e) {
+    g_err = e.what();
+    return BCM_E_INTERNAL;
+  } catch (const char* s) {
+    g_err = s;
+    return BCM_E_INTERNAL;
+  }
+}
+
+void need(bool ok, const char* what) {
+  if (!ok) throw bcm::Error(bcm::kErrArg, what);
+}
+
+void need_device() {
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess || n <= 0)
+    throw bcm::Error(bcm::kErrNoDevice, "no HIP device: the solve phase runs on the GPU only");
+}
+
+bcm::SlvParam to_slv(const bcm_params* p) {
+  bcm::SlvParam s;
+  if (!p) return s;
+  s.msglev = p->msglev;
+  s.ordtype = p->ordtype;
+  s.memory = p->memory;
+  s.tau1 = p->tau1;
+  s.tau2 = p->tau2;
+  s.theta = p->theta;
+  s.sclmin = p->sclmin;
+  s.pivmin = p->pivmin;
+  s.ittype = p->ittype;
+  s.sttype = p->sttype;
+  s.niter = p->niter;
+  s.ichk = p->ichk;
+  s.eps = p->eps;
+  return s;
+}
+
+bcm::SolveOptions to_opts(const bcm_solve_opts* o) {
+  bcm::SolveOptions s;
+  if (!o) return s;
+  s.kind = o->kind;
+  s.niter = o->niter;
+  s.eps = o->eps;
+  s.ittype = o->ittype;
+  s.check_every = std::max(1, o->check_every);
+  s.restart = std::max(1, o->restart);
+  s.prec = o->precision;
+  s.max_restarts = std::max(0, o->max_restarts);
+  return s;
+}
+
+void fill_stats(const bcm::SolveStats& s, bcm_stats* st) {
+  if (!st) return;
+  st->iterations = s.iterations;
+  st->restarts = s.restarts;
+  st->breakdown = s.breakdown;
+  st->converged = s.converged;
+  st->relres_true = s.relres_true;
+  st->relres_rec = s.relres_rec;
+  st->seconds = s.seconds;
+}
+
+bcm::BlockCsr make_csr(int nsup, const int* sprnd, const int* ia, const int* ja, const double* a) {
+  need(nsup > 0 && sprnd && ia && ja && a, "matrix: null argument or empty matrix");
+  bcm::BlockCsr m;
+  m.nsup = nsup;
+  m.sprnd.assign(sprnd, sprnd + nsup + 1);
+  m.ia.assign(ia, ia + nsup + 1);
+  need(m.ia[0] == 0, "matrix: ia[0] must be 0");
+  for (int i = 0; i < nsup; ++i) need(m.sprnd[i + 1] > m.sprnd[i], "matrix: empty block");
+  const int nzja = m.ia[nsup];
+  need(nzja >= 0, "matrix: bad ia");
+  m.ja.assign(ja, ja + nzja);
+  m.bsa.resize(nzja);
+  int64_t off = 0;
+  for (int i = 0; i < nsup; ++i) {
+    need(m.ia[i + 1] >= m.ia[i], "matrix: ia not monotone");
+    for (int j = m.ia[i]; j < m.ia[i + 1]; ++j) {
+      need(m.ja[j] >= 0 && m.ja[j] < nsup, "matrix: column index out of range");
+      m.bsa[j] = off;
+      off += (int64_t)m.bla(i) * m.bla(m.ja[j]);
+    }
+  }
+  m.a.assign(a, a + off);
+  m.set_blamx();
+  return m;
+}
+
+void build_factor_ic2(const bcm::BlockCsr& upper, const bcm::SlvParam& p, bcm::Factor& f) {
+  f.kind = 0;
+  f.u = bcm::ich2(upper, p, &f.stats);
+}
+
+void build_factor_ilu2(const bcm::BlockCsr& a, const bcm::SlvParam& p, bcm::Factor& f) {
+  bcm::BlockCsr al, au;
+  bcm::combine_lu(a, al, au);
+  f.kind = 1;
+  bcm::ilu2(al, au, p, f.l, f.u, &f.stats);
+}
+
+const bcm::BlockCsr& factor_part(const bcm::Factor& f, int which) { return which == 0 ? f.lower() : f.u; }
+
+}  // namespace
+
+void bcm::set_last_error(const std::string& msg) { g_err = msg; }
+
+extern "C" {
+
+const char* bcm_version(void) { return "libbcmhip 0.1 (gfx950)"; }
+
+const char* bcm_last_error(void) { return g_err.c_str(); }
+
+void bcm_params_default(bcm_params* p) {
+  if (!p) return;
+  std::memset(p, 0, sizeof(*p));
+  bcm::SlvParam s;
+  p->msglev = s.msglev;
+  p->ordtype = s.ordtype;
+  p->memory = s.memory;
+  p->tau1 = s.tau1;
+  p->tau2 = s.tau2;
+  p->theta = s.theta;
+  p->sclmin = s.sclmin;
+  p->pivmin = s.pivmin;
+  p->ittype = s.ittype;
+  p->sttype = s.sttype;
+  p->niter = s.niter;
+  p->ichk = s.ichk;
+  p->eps = s.eps;
+}
+
+void bcm_solve_opts_default(bcm_solve_opts* o) {
+  if (!o) return;
+  bcm::SolveOptions s;
+  o->kind = s.kind;
+  o->niter = s.niter;
+  o->eps = s.eps;
+  o->ittype = s.ittype;
+  o->check_every = s.check_every;
+  o->restart = s.restart;
+  o->precision = s.prec;
+  o->max_restarts = s.max_restarts;
+}
+
+int bcm_matrix_create(int nsup, const int* sprnd, const int* ia, const int* ja, const double* a, int symmetric,
+                      bcm_matrix* out) {
+  return guarded([&] {
+    need(out != nullptr, "matrix_create: out is null");
+    auto* h = new bcm_matrix_s;
+    try {
+      h->m.h = make_csr(nsup, sprnd, ia, ja, a);
+      h->m.symmetric = symmetric != 0;
+      h->m.h.validate(symmetric != 0);
+      if (symmetric)
+        for (int i = 0; i < nsup; ++i)
+          for (int j = ia[i]; j < ia[i + 1]; ++j) need(ja[j] >= i, "symmetric matrix must be upper-stored");
+    } catch (...) {
+      delete h;
+      throw;
+    }
+    *out = h;
+  });
+}
+
+int bcm_matrix_destroy(bcm_matrix m) {
+  return guarded([&] { delete m; });
+}
+
+int bcm_matrix_info(bcm_matrix m, int* nsup, int* n, int* nzja, long long* nza, int* symmetric) {
+  return guarded([&] {
+    need(m != nullptr, "matrix_info: null handle");
+    if (nsup) *nsup = m->m.h.nsup;
+    if (n) *n = m->m.h.n();
+    if (nzja) *nzja = m->m.h.nzja();
+    if (nza) *nza = (long long)m->m.h.nza();
+    if (symmetric) *symmetric = m->m.symmetric ? 1 : 0;
+  });
+}
+
+int bcm_matrix_export(bcm_matrix m, int* sprnd, int* ia, int* ja, double* a) {
+  return guarded([&] {
+    need(m != nullptr, "matrix_export: null handle");
+    const auto& h = m->m.h;
+    if (sprnd) std::copy(h.sprnd.begin(), h.sprnd.end(), sprnd);
+    if (ia) std::copy(h.ia.begin(), h.ia.end(), ia);
+    if (ja) std::copy(h.ja.begin(), h.ja.end(), ja);
+    if (a) std::copy(h.a.begin(), h.a.end(), a);
+  });
+}
+
+int bcm_gen_heat2d(int nx, int ny, int bla, unsigned long long seed, bcm_matrix* upper, double* rhs) {
+  return guarded([&] {
+    need(upper != nullptr, "gen_heat2d: out is null");
+    auto* h = new bcm_matrix_s;
+    std::vector<double> b;
+    try {
+      bcm::gen_heat2d(nx, ny, bla, seed, h->m.h, b);
+    } catch (...) {
+      delete h;
+      throw;
+    }
+    h->m.symmetric = true;
+    if (rhs) std::copy(b.begin(), b.end(), rhs);
+    *upper = h;
+  });
+}
+
+int bcm_gen_seepage3d(int nx, int ny, int nz, int bla, double skew, unsigned long long seed, bcm_matrix* a,
+                      double* rhs) {
+  return guarded([&] {
+    need(a != nullptr, "gen_seepage3d: out is null");
+    auto* h = new bcm_matrix_s;
+    std::vector<double> b;
+    try {
+      bcm::gen_seepage3d(nx, ny, nz, bla, skew, seed, h->m.h, b);
+    } catch (...) {
+      delete h;
+      throw;
+    }
+    h->m.symmetric = false;
+    if (rhs) std::copy(b.begin(), b.end(), rhs);
+    *a = h;
+  });
+}
+
+int bcm_gen_elastic_cell(int m, double radius, double contrast, double poisson, double shift,
+                         unsigned long long seed, bcm_matrix* upper, double* rhs6) {
+  return guarded([&] {
+    need(upper != nullptr, "gen_elastic_cell: out is null");
+    auto* h = new bcm_matrix_s;
+    std::vector<double> b;
+    bcm::CellParams cp;
+    cp.radius = radius;
+    cp.contrast = contrast;
+    cp.poisson = poisson;
+    cp.shift = shift;
+    try {
+      bcm::gen_elastic_cell(m, cp, seed, h->m.h, b);
+    } catch (...) {
+      delete h;
+      throw;
+    }
+    h->m.symmetric = true;
+    if (rhs6) std::copy(b.begin(), b.end(), rhs6);
+    *upper = h;
+  });
+}
+
+int bcm_rcm_order(bcm_matrix m, int* order) {
+  return guarded([&] {
+    need(m && order, "rcm_order: null argument");
+    auto o = bcm::rcm_order(m->m.h);
+    std::copy(o.begin(), o.end(), order);
+  });
+}
+
+int bcm_matrix_reorder(bcm_matrix m, const int* order, bcm_matrix* out) {
+  return guarded([&] {
+    need(m && order && out, "matrix_reorder: null argument");
+    std::vector<int> o(order, order + m->m.h.nsup);
+    std::vector<char> seen(o.size(), 0);
+    for (int v : o) {
+      need(v >= 0 && v < (int)o.size() && !seen[v], "matrix_reorder: order is not a permutation");
+      seen[v] = 1;
+    }
+    auto* h = new bcm_matrix_s;
+    try {
+      h->m.symmetric = m->m.symmetric;
+      h->m.h = m->m.symmetric ? bcm::ord_mtr_symm(m->m.h, o) : bcm::ord_mtr(m->m.h, o);
+    } catch (...) {
+      delete h;
+      throw;
+    }
+    *out = h;
+  });
+}
+
+int bcm_vector_reorder(bcm_matrix original, const int* order, char type, double* v, int nrhs) {
+  return guarded([&] {
+    need(original && order && v && nrhs > 0, "vector_reorder: null argument");
+    need(type == 'D' || type == 'I', "vector_reorder: type must be 'D' or 'I'");
+    const auto& h = original->m.h;
+    std::vector<int> o(order, order + h.nsup);
+    std::vector<int> spn(h.nsup + 1, 0);
+    for (int i = 0; i < h.nsup; ++i) spn[o[i] + 1] = h.bla(i);
+    for (int i = 0; i < h.nsup; ++i) spn[i + 1] += spn[i];
+    bcm::ord_vect(type, h.sprnd, spn, o, v, h.n(), nrhs);
+  });
+}
+
+int bcm_factor_ic2(bcm_matrix upper, bcm_params* p, bcm_factor* out) {
+  return guarded([&] {
+    need(upper && out, "factor_ic2: null argument");
+    need(upper->m.symmetric, "factor_ic2: matrix must be symmetric (upper-stored)");
+    auto* h = new bcm_factor_s;
+    try {
+      build_factor_ic2(upper->m.h, to_slv(p), h->f);
+    } catch (...) {
+      delete h;
+      throw;
+    }
+    if (p) {
+      p->timefct = h->f.stats.seconds;
+      p->density = h->f.stats.densu;
+      p->density2 = h->f.stats.densr;
+    }
+    *out = h;
+  });
+}
+
+int bcm_factor_ilu2(bcm_matrix a, bcm_params* p, bcm_factor* out) {
+  return guarded([&] {
+    need(a && out, "factor_ilu2: null argument");
+    need(!a->m.symmetric, "factor_ilu2: matrix must be stored in full");
+    auto* h = new bcm_factor_s;
+    try {
+      build_factor_ilu2(a->m.h, to_slv(p), h->f);
+    } catch (...) {
+      delete h;
+      throw;
+    }
+    if (p) {
+      p->timefct = h->f.stats.seconds;
+      p->density = h->f.stats.densu;
+      p->density2 = h->f.stats.densr;
+    }
+    *out = h;
+  });
+}
+
+int bcm_factor_destroy(bcm_factor f) {
+  return guarded([&] { delete f; });
+}
+
+int bcm_factor_info(bcm_factor f, int which, int* nzja, long long* nza, int* levels_fwd, int* levels_bwd) {
+  return guarded([&] {
+    need(f != nullptr && (which == 0 || which == 1), "factor_info: bad argument");
+    const auto& m = factor_part(f->f, which);
+    if (nzja) *nzja = m.nzja();
+    if (nza) *nza = (long long)m.nza();
+    auto levels = [&](bool fwd) {
+      std::vector<int> lev(m.nsup, 0);
+      int mx = 0;
+      if (fwd) {
+        for (int i = 0; i < m.nsup; ++i) {
+          for (int j = m.ia[i] + 1; j < m.ia[i + 1]; ++j) lev[m.ja[j]] = std::max(lev[m.ja[j]], lev[i] + 1);
+          mx = std::max(mx, lev[i]);
+        }
+      } else {
+        for (int i = m.nsup - 1; i >= 0; --i) {
+          int l = 0;
+          for (int j = m.ia[i] + 1; j < m.ia[i + 1]; ++j) l = std::max(l, lev[m.ja[j]] + 1);
+          lev[i] = l;
+          mx = std::max(mx, l);
+        }
+      }
+      return m.nsup ? mx + 1 : 0;
+    };
+    if (levels_fwd) *levels_fwd = levels(true);
+    if (levels_bwd) *levels_bwd = levels(false);
+  });
+}
+
+int bcm_factor_export(bcm_factor f, int which, int* ia, int* ja, double* a) {
+  return guarded([&] {
+    need(f != nullptr && (which == 0 || which == 1), "factor_export: bad argument");
+    const auto& m = factor_part(f->f, which);
+    if (ia) std::copy(m.ia.begin(), m.ia.end(), ia);
+    if (ja) std::copy(m.ja.begin(), m.ja.end(), ja);
+    if (a) std::copy(m.a.begin(), m.a.end(), a);
+  });
+}
+
+int bcm_device_count(int* count) {
+  return guarded([&] {
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess) n = 0;
+    if (count) *count = n;
+  });
+}
+
+int bcm_context_create(bcm_context* out) {
+  return guarded([&] {
+    need(out != nullptr, "context_create: out is null");
+    need_device();
+    *out = new bcm_context_s;
+  });
+}
+
+int bcm_context_destroy(bcm_context c) {
+  return guarded([&] { delete c; });
+}
+
+int bcm_context_sync(bcm_context c) {
+  return guarded([&] {
+    need(c != nullptr, "context_sync: null handle");
+    BCM_HIP(hipStreamSynchronize(c->c.stream));
+    c->c.check_sweep_error();
+  });
+}
+
+int bcm_mvm(bcm_context c, bcm_matrix a, const double* x, double* y, int nrhs) {
+  return guarded([&] {
+    need(c && a && x && y && nrhs > 0, "mvm: bad argument");
+    a->m.to_device(false);
+    bcm::launch_spmv(a->m.d, x, y, nullptr, nrhs, nullptr, nullptr, c->c.rw, c->c.stream);
+  });
+}
+
+int bcm_mvm_t(bcm_context c, bcm_matrix a, const double* x, double* y, int nrhs) {
+  return guarded([&] {
+    need(c && a && x && y && nrhs > 0, "mvm_t: bad argument");
+    a->m.to_device(true);
+    bcm::launch_spmv(a->m.dt, x, y, nullptr, nrhs, nullptr, nullptr, c->c.rw, c->c.stream);
+  });
+}
+
+int bcm_residual(bcm_context c, bcm_matrix a, const double* b, const double* x, double* r, int nrhs) {
+  return guarded([&] {
+    need(c && a && b && x && r && nrhs > 0, "residual: bad argument");
+    a->m.to_device(false);
+    bcm::launch_spmv(a->m.d, x, r, b, nrhs, nullptr, nullptr, c->c.rw, c->c.stream);
+  });
+}
+
+int bcm_solve_l(bcm_context c, bcm_factor f, int which, const double* x, double* y, int nrhs) {
+  return guarded([&] {
+    need(c && f && x && y && nrhs > 0 && (which == 0 || which == 1), "solve_l: bad argument");
+    need(x != y, "solve_l: x and y must not alias");
+    const bool t = which == 1 && f->f.kind == 1;
+    f->f.to_device(bcm::kF64, t);
+    bcm::launch_sweep(t ? f->f.fwdt : f->f.fwd, x, y, bcm::kF64, nrhs, c->c.sync, c->c.stream);
+  });
+}
+
+int bcm_solve_u(bcm_context c, bcm_factor f, int which, const double* x, double* y, int nrhs) {
+  return guarded([&] {
+    need(c && f && x && y && nrhs > 0 && (which == 0 || which == 1), "solve_u: bad argument");
+    need(x != y, "solve_u: x and y must not alias");
+    const bool t = which == 0 && f->f.kind == 1;
+    f->f.to_device(bcm::kF64, t);
+    bcm::launch_sweep(t ? f->f.bwdt : f->f.bwd, x, y, bcm::kF64, nrhs, c->c.sync, c->c.stream);
+  });
+}
+
+int bcm_precond(bcm_context c, bcm_factor f, const double* r, double* z, int nrhs, int precision, int transpose) {
+  return guarded([&] {
+    need(c && f && r && z && nrhs > 0, "precond: bad argument");
+    need(precision >= 0 && precision <= 2, "precond: bad precision");
+    bcm::precond_apply(f->f, c->c, r, z, precision, transpose != 0, nrhs);
+  });
+}
+
+int bcm_dot(bcm_context c, int n, const double* x, const double* y, double* result) {
+  return guarded([&] {
+    need(c && x && y && result && n >= 0, "dot: bad argument");
+    bcm::KState* ks = c->c.dstate();
+    const double* xs[1] = {x};
+    const double* ys[1] = {y};
+    bcm::launch_multidot(n, 1, xs, ys, &ks->sc[31], c->c.rw, c->c.stream);
+    c->c.pull_state();
+    *result = c->c.host_state->sc[31];
+  });
+}
+
+int bcm_solve(bcm_context c, bcm_matrix a, bcm_factor f, const bcm_solve_opts* o, const double* b, double* x,
+              bcm_stats* st) {
+  return guarded([&] {
+    need(c && a && f && b && x, "solve: bad argument");
+    bcm::SolveOptions opt = to_opts(o);
+    bcm::SolveStats s;
+    bcm::solve(a->m, f->f, c->c, opt, b, x, s);
+    fill_stats(s, st);
+    if (s.breakdown) throw bcm::Error(bcm::kErrBreakdown, "Krylov breakdown");
+  });
+}
+
+int* bcm_alloc_ints(int nint) { return new (std::nothrow) int[nint > 0 ? nint : 1]; }
+
+}  // extern "C"

@@ -1,0 +1,133 @@
+// Device-side objects of libbcmhip and the launch wrappers of its gfx950 kernels.
+//
+// Layout in HBM (one GPU, everything resident):
+//  * DevBsr: block-CSR with the reference's block order (row-major over block
+//    rows, ascending columns, column-major b x b blocks); uniform block size
+//    b <= 64 for the fast kernels. Values in fp64, fp32 or bf16.
+//  * DevTri: one triangular sweep. Row k lists the rows it depends on and
+//    the matching b x b blocks, already transposed where the reference applies
+//    the transpose (SolveL on a by-columns factor), plus the inverted diagonal
+//    block of every row (again pre-transposed for forward sweeps). Forward
+//    sweeps depend on smaller rows, backward sweeps on larger ones.
+//  * Scalars of the Krylov recurrences live in device memory, so a solver
+//    iteration is a fixed launch sequence with no host round trip.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "../host/bcm_host.hpp"
+
+#define BCM_HIP(call)                                                                       \
+  do {                                                                                      \
+    hipError_t e_ = (call);                                                                 \
+    if (e_ != hipSuccess)                                                                   \
+      throw ::bcm::Error(::bcm::kErrHip, std::string(#call) + ": " + hipGetErrorString(e_)); \
+  } while (0)
+
+namespace bcm {
+
+enum Prec : int { kF64 = 0, kF32 = 1, kBF16 = 2 };
+
+inline size_t prec_bytes(int p) { return p == kF64 ? 8 : (p == kF32 ? 4 : 2); }
+
+// Owning device buffer.
+struct DevBuf {
+  void* p = nullptr;
+  size_t bytes = 0;
+  DevBuf() = default;
+  DevBuf(const DevBuf&) = delete;
+  DevBuf& operator=(const DevBuf&) = delete;
+  DevBuf(DevBuf&& o) noexcept : p(o.p), bytes(o.bytes) { o.p = nullptr; o.bytes = 0; }
+  DevBuf& operator=(DevBuf&& o) noexcept {
+    if (this != &o) { release(); p = o.p; bytes = o.bytes; o.p = nullptr; o.bytes = 0; }
+    return *this;
+  }
+  ~DevBuf() { release(); }
+  void alloc(size_t nbytes);
+  void release();
+  void upload(const void* src, size_t nbytes, hipStream_t s = nullptr);
+  template <class T> T* as() const { return static_cast<T*>(p); }
+};
+
+struct DevBsr {
+  int nsup = 0, n = 0, b = 0, nzja = 0;
+  int prec = kF64;
+  DevBuf ia, ja, a;
+  int64_t value_bytes() const { return (int64_t)nzja * b * b * (int64_t)prec_bytes(prec); }
+  int64_t index_bytes() const { return 4LL * (nzja + nsup + 1); }
+};
+
+// Builds a device block-CSR from host storage (uniform block size required).
+void dev_bsr_upload(const BlockCsr& h, int prec, DevBsr& d);
+
+struct DevTri {
+  int nsup = 0, n = 0, b = 0, ndep = 0;
+  bool forward = true;
+  int prec = kF64;
+  int levels = 0;
+  DevBuf ia, ja;      // dependency lists per row (row index space of the sweep)
+  DevBuf off;         // ndep blocks, row after row
+  DevBuf diag;        // nsup inverted diagonal blocks
+  int64_t value_bytes() const { return ((int64_t)ndep + nsup) * b * b * (int64_t)prec_bytes(prec); }
+  int64_t index_bytes() const { return 4LL * (ndep + nsup + 1); }
+};
+
+// Forward sweep through a factor stored by columns (reference SolveL):
+// y_k = D_k^T (x_k - sum_{i<k} F_ik^T y_i).
+void dev_tri_forward(const BlockCsr& f, int prec, DevTri& t);
+// Backward sweep through a factor stored by rows (reference SolveU):
+// z_i = D_i (x_i - sum_{j>i} F_ij z_j).
+void dev_tri_backward(const BlockCsr& f, int prec, DevTri& t);
+
+// Synchronisation state shared by the sweeps on one stream.
+struct SweepSync {
+  DevBuf flags;   // per-row epoch of the last completed sweep
+  DevBuf ctl;     // [0] epoch, [1] ticket, [2] error (spin limit hit)
+  int nsup = 0;
+  void ensure(int nsup);
+};
+
+// Reduction workspace: partial sums and the completion counter of the
+// single-pass (last block finalises) reductions.
+struct RedWork {
+  DevBuf partials;  // kRedBlocks * kMaxDots doubles
+  DevBuf counter;   // unsigned
+  void ensure();
+};
+constexpr int kRedBlocks = 512;
+constexpr int kMaxDots = 8;
+
+// ---- launches (all asynchronous on `s`) ----
+
+// y = A x (mode 0) or y = bsub - A x (mode 1); nrhs columns with leading
+// dimension n. Optionally accumulates dot(wdot, y) over all rows into
+// *dot_out (first column only) when dot_out != nullptr.
+// Every launch takes an optional device flag `skip`: when *skip != 0 the
+// kernel returns at once (a finished solver keeps its queued iterations inert).
+void launch_spmv(const DevBsr& A, const double* x, double* y, const double* bsub, int nrhs,
+                 double* dot_out, const double* wdot, RedWork& rw, hipStream_t s, const int* skip = nullptr);
+
+// Triangular sweep y = T^-1 x for the given precision of vectors (TX).
+void launch_sweep(const DevTri& T, const void* x, void* y, int vec_prec, int nrhs, SweepSync& sync,
+                  hipStream_t s, const int* skip = nullptr);
+
+// out[k] = sum_i x_k[i] * y_k[i] for K pairs (K <= kMaxDots), one pass.
+void launch_multidot(int n, int k, const double* const* xs, const double* const* ys, double* out,
+                     RedWork& rw, hipStream_t s, const int* skip = nullptr);
+
+// Converts between fp64 and fp32 vectors.
+void launch_convert(int64_t n, const void* src, int src_prec, void* dst, int dst_prec, hipStream_t s);
+
+// Generic element-wise kernels used by the drivers (defined with the kernels).
+void launch_axpby(int64_t n, double alpha, const double* x, double beta, double* y, hipStream_t s);
+void launch_copy(int64_t n, const double* x, double* y, hipStream_t s);
+void launch_fill(int64_t n, double v, double* y, hipStream_t s);
+
+// Number of compute units of the current device.
+int device_cu_count();
+
+}  // namespace bcm

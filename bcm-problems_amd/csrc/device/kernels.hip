@@ -1,0 +1,570 @@
+// gfx950 kernels of the solve phase: block-CSR SpMV, sync-free triangular
+// sweeps through inverse-diagonal-block factors, and one-pass reductions.
+//
+// Reference operators: CSMatrixRS::MvmA (mvm.cpp:2530-2569), MvmAt
+// (mvm.cpp:2593-2632), MvmASymm (mvm.cpp:3008-3037; run here on the
+// symmetric matrix expanded to full storage), SolveL (mvm.cpp:3140-3217),
+// SolveU (mvm.cpp:3410-3484), ScProdSVect/DaxpySVect (svector.cpp:321-371).
+//
+// Mapping: one 64-lane wavefront per block row. For block size B the lanes
+// form G = 64/B column groups, lane = r + B*g: group g walks the block
+// columns c = g, g+G, ... of the row's blocks, so one load instruction reads
+// G*B consecutive doubles of a column-major block (coalesced), and the G
+// partial sums of each row element are folded with cross-lane shuffles.
+//
+// Triangular sweeps are sync-free: waves take rows in dependency order from
+// an atomic ticket, spin (bounded) on the per-row completion epochs of the
+// rows they depend on, and publish their own row with a release store. Every
+// wave only ever waits on rows claimed earlier by running waves, so the
+// sweep cannot deadlock, and every wait is bounded by a spin limit that sets
+// an error flag instead of hanging.
+#include "device.hpp"
+#include "dev_common.hpp"
+
+#include <algorithm>
+#include <cstring>
+
+namespace bcm {
+
+namespace {
+
+// Maps block index to a row tile so that consecutive tiles stay on one XCD
+// (the hardware deals workgroups round-robin over the 8 XCDs).
+__device__ __forceinline__ int xcd_tile(int bid, int nblk) {
+  if (nblk & 7) return bid;
+  return (bid & 7) * (nblk >> 3) + (bid >> 3);
+}
+
+// ---------------------------------------------------------------------------
+// SpMV: y = A x or y = bsub - A x, NR right-hand sides, optional dot(w, y).
+template <int B, int NR, class TA>
+__global__ __launch_bounds__(256) void k_spmv(int nsup, int n, const int* __restrict__ ia,
+                                              const int* __restrict__ ja, const TA* __restrict__ a,
+                                              const double* __restrict__ x, double* __restrict__ y,
+                                              const double* __restrict__ bsub, const double* __restrict__ wdot,
+                                              double* partials, unsigned* counter, double* dot_out,
+                                              int rows_per_wave, const int* skip) {
+  if (skip && *skip) return;
+  constexpr int G = 64 / B;
+  const int lane = threadIdx.x & 63;
+  const int r = lane % B, g = lane / B;
+  const int tile = xcd_tile(blockIdx.x, gridDim.x);
+  const int wave_global = tile * (blockDim.x >> 6) + (threadIdx.x >> 6);
+  double dotv = 0.0;
+  for (int q = 0; q < rows_per_wave; ++q) {
+    const int row = wave_global * rows_per_wave + q;
+    if (row >= nsup) break;
+    double acc[NR];
+#pragma unroll
+    for (int k = 0; k < NR; ++k) acc[k] = 0.0;
+    const int jb = ia[row], je = ia[row + 1];
+    if (g < G) {
+      const TA* arow = a + (size_t)jb * B * B;
+      const int ncol = (je - jb) * B;
+#pragma unroll 4
+      for (int t = g; t < ncol; t += G) {
+        const int bk = t / B, c = t - bk * B;
+        const int col = ja[jb + bk];
+        const double av = (double)ldnt(arow + (size_t)bk * B * B + c * B + r);
+#pragma unroll
+        for (int k = 0; k < NR; ++k) acc[k] += av * x[(size_t)k * n + (size_t)col * B + c];
+      }
+    }
+#pragma unroll
+    for (int k = 0; k < NR; ++k) {
+      double s = acc[k];
+#pragma unroll
+      for (int h = 1; h < G; ++h) s += __shfl(acc[k], lane + h * B);
+      acc[k] = s;
+    }
+    if (lane < B) {
+      const size_t e = (size_t)row * B + lane;
+#pragma unroll
+      for (int k = 0; k < NR; ++k) {
+        const double v = bsub ? bsub[(size_t)k * n + e] - acc[k] : acc[k];
+        y[(size_t)k * n + e] = v;
+        if (k == 0 && wdot) dotv += wdot[e] * v;
+      }
+    }
+  }
+  if (dot_out) {
+    double vv[1] = {dotv};
+    reduce_finalize<1>(vv, partials, counter, dot_out);
+  }
+}
+
+// Variable block sizes: one thread per scalar row (correctness path).
+__global__ void k_spmv_var(int nsup, int n, const int* __restrict__ sprnd, const int* __restrict__ ia,
+                           const int* __restrict__ ja, const int64_t* __restrict__ bsa,
+                           const double* __restrict__ a, const double* __restrict__ x, double* __restrict__ y,
+                           const double* __restrict__ bsub, int nrhs) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  int lo = 0, hi = nsup;
+  while (hi - lo > 1) { int m = (lo + hi) / 2; if (sprnd[m] <= i) lo = m; else hi = m; }
+  const int row = lo, r = i - sprnd[row], bi = sprnd[row + 1] - sprnd[row];
+  for (int k = 0; k < nrhs; ++k) {
+    double acc = 0.0;
+    for (int j = ia[row]; j < ia[row + 1]; ++j) {
+      const int col = ja[j], c0 = sprnd[col], bj = sprnd[col + 1] - c0;
+      const double* blk = a + bsa[j];
+      for (int c = 0; c < bj; ++c) acc += blk[(size_t)c * bi + r] * x[(size_t)k * n + c0 + c];
+    }
+    y[(size_t)k * n + i] = bsub ? bsub[(size_t)k * n + i] - acc : acc;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Sync-free triangular sweep.
+__global__ void k_sweep_arm(int* ctl) {
+  ctl[0] += 1;
+  ctl[1] = 0;
+}
+
+template <int B, int NR, class TA, class TX>
+__global__ __launch_bounds__(64) void k_sweep(int nsup, int n, int forward, const int* __restrict__ ia,
+                                              const int* __restrict__ ja, const TA* __restrict__ off,
+                                              const TA* __restrict__ diag, const TX* __restrict__ x, TX* y,
+                                              int* flags, int* ctl, const int* skip) {
+  if (skip && *skip) return;
+  using Acc = typename AccOf<TX>::type;
+  constexpr int G = 64 / B;
+  __shared__ Acc v[NR][B];
+  const int lane = threadIdx.x;
+  const int r = lane % B, g = lane / B;
+  const int epoch = ctl[0];
+  for (;;) {
+    int t = 0;
+    if (lane == 0) t = __hip_atomic_fetch_add(&ctl[1], 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    t = __shfl(t, 0);
+    if (t >= nsup) break;
+    const int row = forward ? t : nsup - 1 - t;
+    const int jb = ia[row], je = ia[row + 1];
+    // Wait until every row this one depends on has been published.
+    for (int base = jb; base < je; base += 64) {
+      const int j = base + lane;
+      if (j < je) {
+        const int k = ja[j];
+        int spins = 0;
+        while (__hip_atomic_load(&flags[k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != epoch) {
+          __builtin_amdgcn_s_sleep(1);
+          if (++spins > kSpinLimit) {
+            __hip_atomic_store(&ctl[2], 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            break;
+          }
+        }
+      }
+    }
+    Acc acc[NR];
+#pragma unroll
+    for (int k = 0; k < NR; ++k) acc[k] = Acc(0);
+    if (g < G) {
+      const TA* blk0 = off + (size_t)jb * B * B;
+      const int ncol = (je - jb) * B;
+      for (int tt = g; tt < ncol; tt += G) {
+        const int bk = tt / B, c = tt - bk * B;
+        const int k = ja[jb + bk];
+        const Acc av = (Acc)ldnt(blk0 + (size_t)bk * B * B + c * B + r);
+        for (int q = 0; q < NR; ++q) acc[q] += av * (Acc)ld_coherent(&y[(size_t)q * n + (size_t)k * B + c]);
+      }
+    }
+#pragma unroll
+    for (int q = 0; q < NR; ++q) {
+      Acc s = acc[q];
+#pragma unroll
+      for (int h = 1; h < G; ++h) s += __shfl(acc[q], lane + h * B);
+      if (lane < B) v[q][lane] = (Acc)x[(size_t)q * n + (size_t)row * B + lane] - s;
+    }
+    __syncthreads();
+    Acc z[NR];
+#pragma unroll
+    for (int q = 0; q < NR; ++q) z[q] = Acc(0);
+    if (g < G) {
+      const TA* d = diag + (size_t)row * B * B;
+#pragma unroll
+      for (int c = g; c < B; c += G) {
+        const Acc dv = (Acc)ldnt(d + c * B + r);
+#pragma unroll
+        for (int q = 0; q < NR; ++q) z[q] += dv * v[q][c];
+      }
+    }
+#pragma unroll
+    for (int q = 0; q < NR; ++q) {
+      Acc s = z[q];
+#pragma unroll
+      for (int h = 1; h < G; ++h) s += __shfl(z[q], lane + h * B);
+      if (lane < B) st_coherent(&y[(size_t)q * n + (size_t)row * B + lane], (TX)s);
+    }
+    __syncthreads();
+    if (lane == 0) __hip_atomic_store(&flags[row], epoch, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// BLAS-1
+
+struct DotArgs {
+  const double* x[kMaxDots];
+  const double* y[kMaxDots];
+};
+
+template <int K>
+__global__ __launch_bounds__(256) void k_multidot(int64_t n, DotArgs args, double* partials, unsigned* counter,
+                                                  double* out, const int* skip) {
+  if (skip && *skip) return;
+  double v[K];
+#pragma unroll
+  for (int k = 0; k < K; ++k) v[k] = 0.0;
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+#pragma unroll
+    for (int k = 0; k < K; ++k) v[k] += args.x[k][i] * args.y[k][i];
+  }
+  reduce_finalize<K>(v, partials, counter, out);
+}
+
+__global__ void k_axpby(int64_t n, double alpha, const double* __restrict__ x, double beta, double* __restrict__ y) {
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride)
+    y[i] = alpha * x[i] + beta * y[i];
+}
+
+__global__ void k_copy(int64_t n, const double* __restrict__ x, double* __restrict__ y) {
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) y[i] = x[i];
+}
+
+__global__ void k_fill(int64_t n, double v, double* __restrict__ y) {
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) y[i] = v;
+}
+
+template <class S, class D>
+__global__ void k_convert(int64_t n, const S* __restrict__ x, D* __restrict__ y) {
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) y[i] = (D)x[i];
+}
+
+int grid_for(int64_t n, int threads) {
+  int64_t g = (n + threads - 1) / threads;
+  return (int)std::max<int64_t>(1, std::min<int64_t>(g, 2048));
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// Buffers and host-side construction
+
+void DevBuf::alloc(size_t nbytes) {
+  release();
+  if (nbytes == 0) return;
+  BCM_HIP(hipMalloc(&p, nbytes));
+  bytes = nbytes;
+}
+
+void DevBuf::release() {
+  if (p) (void)hipFree(p);
+  p = nullptr;
+  bytes = 0;
+}
+
+void DevBuf::upload(const void* src, size_t nbytes, hipStream_t s) {
+  if (nbytes > bytes) alloc(nbytes);
+  if (nbytes) BCM_HIP(hipMemcpyAsync(p, src, nbytes, hipMemcpyHostToDevice, s));
+  if (nbytes) BCM_HIP(hipStreamSynchronize(s));
+}
+
+int device_cu_count() {
+  int dev = 0, cus = 0;
+  BCM_HIP(hipGetDevice(&dev));
+  BCM_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+  return cus;
+}
+
+static uint16_t to_bf16(float f) {
+  uint32_t u;
+  std::memcpy(&u, &f, 4);
+  if ((u & 0x7f800000u) == 0x7f800000u) return (uint16_t)(u >> 16);  // inf/nan
+  const uint32_t rounding = 0x7fffu + ((u >> 16) & 1u);  // round to nearest even
+  return (uint16_t)((u + rounding) >> 16);
+}
+
+static void upload_values(const std::vector<double>& v, int prec, DevBuf& d) {
+  if (prec == kF64) {
+    d.upload(v.data(), v.size() * 8);
+  } else if (prec == kF32) {
+    std::vector<float> f(v.begin(), v.end());
+    d.upload(f.data(), f.size() * 4);
+  } else {
+    std::vector<uint16_t> h(v.size());
+    for (size_t i = 0; i < v.size(); ++i) h[i] = to_bf16((float)v[i]);
+    d.upload(h.data(), h.size() * 2);
+  }
+}
+
+static int uniform_block(const BlockCsr& h) {
+  const int b = h.nsup ? h.bla(0) : 0;
+  for (int i = 0; i < h.nsup; ++i)
+    if (h.bla(i) != b) return 0;
+  return b;
+}
+
+void dev_bsr_upload(const BlockCsr& h, int prec, DevBsr& d) {
+  d.nsup = h.nsup;
+  d.n = h.n();
+  d.b = uniform_block(h);
+  if (d.b == 0 || d.b > 64) throw Error(kErrArg, "device matrices need a uniform block size of at most 64");
+  d.nzja = h.nzja();
+  d.prec = prec;
+  d.ia.upload(h.ia.data(), h.ia.size() * 4);
+  d.ja.upload(h.ja.data(), h.ja.size() * 4);
+  upload_values(h.a, prec, d.a);
+}
+
+static void build_tri(const BlockCsr& f, bool forward, int prec, DevTri& t) {
+  const int b = uniform_block(f);
+  if (b == 0 || b > 64) throw Error(kErrArg, "device factors need a uniform block size of at most 64");
+  const int nsup = f.nsup, bb = b * b;
+  t.nsup = nsup;
+  t.n = f.n();
+  t.b = b;
+  t.forward = forward;
+  t.prec = prec;
+  std::vector<int> ia(nsup + 1, 0), ja;
+  std::vector<double> off, diag((size_t)nsup * bb);
+  for (int i = 0; i < nsup; ++i) {
+    if (f.ia[i] == f.ia[i + 1] || f.ja[f.ia[i]] != i) throw Error(kErrArg, "factor rows must start with the diagonal block");
+    const double* d = &f.a[f.bsa[f.ia[i]]];
+    double* o = &diag[(size_t)i * bb];
+    for (int r = 0; r < b; ++r)
+      for (int c = 0; c < b; ++c) o[c * b + r] = forward ? d[r * b + c] : d[c * b + r];
+  }
+  if (!forward) {
+    for (int i = 0; i < nsup; ++i) {
+      for (int j = f.ia[i] + 1; j < f.ia[i + 1]; ++j) {
+        ja.push_back(f.ja[j]);
+        off.insert(off.end(), f.a.begin() + f.bsa[j], f.a.begin() + f.bsa[j] + bb);
+      }
+      ia[i + 1] = (int)ja.size();
+    }
+  } else {
+    // Row k gathers the transposed blocks F_ik of the column-stored factor.
+    std::vector<int> cnt(nsup + 1, 0);
+    for (int i = 0; i < nsup; ++i)
+      for (int j = f.ia[i] + 1; j < f.ia[i + 1]; ++j) cnt[f.ja[j] + 1]++;
+    for (int k = 0; k < nsup; ++k) ia[k + 1] = ia[k] + cnt[k + 1];
+    ja.resize(ia[nsup]);
+    off.resize((size_t)ia[nsup] * bb);
+    std::vector<int> pos(ia.begin(), ia.end() - 1);
+    for (int i = 0; i < nsup; ++i)
+      for (int j = f.ia[i] + 1; j < f.ia[i + 1]; ++j) {
+        const int k = f.ja[j];
+        const int p = pos[k]++;
+        ja[p] = i;
+        const double* s = &f.a[f.bsa[j]];
+        double* o = &off[(size_t)p * bb];
+        for (int r = 0; r < b; ++r)
+          for (int c = 0; c < b; ++c) o[c * b + r] = s[r * b + c];
+      }
+  }
+  // level count of the dependency graph (reported, not used by the kernel)
+  std::vector<int> lev(nsup, 0);
+  int mx = 0;
+  for (int s = 0; s < nsup; ++s) {
+    const int row = forward ? s : nsup - 1 - s;
+    int l = 0;
+    for (int j = ia[row]; j < ia[row + 1]; ++j) l = std::max(l, lev[ja[j]] + 1);
+    lev[row] = l;
+    mx = std::max(mx, l);
+  }
+  t.levels = nsup ? mx + 1 : 0;
+  t.ndep = ia[nsup];
+  t.ia.upload(ia.data(), ia.size() * 4);
+  t.ja.upload(ja.empty() ? ia.data() : ja.data(), std::max<size_t>(ja.size(), 1) * 4);
+  std::vector<double> offv = off.empty() ? std::vector<double>(1, 0.0) : off;
+  upload_values(offv, prec, t.off);
+  upload_values(diag, prec, t.diag);
+}
+
+void dev_tri_forward(const BlockCsr& f, int prec, DevTri& t) { build_tri(f, true, prec, t); }
+void dev_tri_backward(const BlockCsr& f, int prec, DevTri& t) { build_tri(f, false, prec, t); }
+
+void SweepSync::ensure(int n) {
+  if (n <= nsup && flags.p) return;
+  flags.alloc((size_t)std::max(n, 1) * 4);
+  BCM_HIP(hipMemset(flags.p, 0, flags.bytes));
+  if (!ctl.p) {
+    ctl.alloc(64);
+    BCM_HIP(hipMemset(ctl.p, 0, 64));
+  } else {
+    // restart the epochs so no stale flag can match
+    BCM_HIP(hipMemset(ctl.p, 0, 64));
+  }
+  nsup = n;
+}
+
+void RedWork::ensure() {
+  if (partials.p) return;
+  partials.alloc((size_t)2048 * kMaxDots * 8);
+  counter.alloc(64);
+  BCM_HIP(hipMemset(counter.p, 0, 64));
+}
+
+// ---------------------------------------------------------------------------
+// Launch wrappers
+
+template <int B, int NR, class TA>
+static void spmv_b(const DevBsr& A, const double* x, double* y, const double* bsub, double* dot_out,
+                   const double* wdot, RedWork& rw, hipStream_t s, const int* skip) {
+  constexpr int kWavesPerBlock = 4;
+  const int nwaves_max = 2048 * kWavesPerBlock;
+  const int rows_per_wave = (A.nsup + nwaves_max - 1) / nwaves_max;
+  const int nwaves = (A.nsup + rows_per_wave - 1) / rows_per_wave;
+  int grid = (nwaves + kWavesPerBlock - 1) / kWavesPerBlock;
+  if (grid > 8) grid = (grid + 7) / 8 * 8;  // whole XCD rounds, so the tile map is a permutation
+  if (dot_out) rw.ensure();
+  hipLaunchKernelGGL((k_spmv<B, NR, TA>), dim3(grid), dim3(64 * kWavesPerBlock), 0, s, A.nsup, A.n,
+                     A.ia.as<int>(), A.ja.as<int>(), A.a.as<TA>(), x, y, bsub, wdot,
+                     rw.partials.as<double>(), rw.counter.as<unsigned>(), dot_out, rows_per_wave, skip);
+  BCM_HIP(hipGetLastError());
+}
+
+template <int B, int NR>
+static void spmv_prec(const DevBsr& A, const double* x, double* y, const double* bsub, double* dot_out,
+                      const double* wdot, RedWork& rw, hipStream_t s, const int* skip) {
+  if (A.prec == kF64) spmv_b<B, NR, double>(A, x, y, bsub, dot_out, wdot, rw, s, skip);
+  else if (A.prec == kF32) spmv_b<B, NR, float>(A, x, y, bsub, dot_out, wdot, rw, s, skip);
+  else spmv_b<B, NR, uint16_t>(A, x, y, bsub, dot_out, wdot, rw, s, skip);
+}
+
+template <int B>
+static void spmv_nr(const DevBsr& A, const double* x, double* y, const double* bsub, int nrhs, double* dot_out,
+                    const double* wdot, RedWork& rw, hipStream_t s, const int* skip) {
+  int done = 0;
+  while (done < nrhs) {
+    const int left = nrhs - done;
+    const size_t o = (size_t)done * A.n;
+    const double* bs = bsub ? bsub + o : nullptr;
+    double* dd = done == 0 ? dot_out : nullptr;
+    if (left >= 6) { spmv_prec<B, 6>(A, x + o, y + o, bs, dd, wdot, rw, s, skip); done += 6; }
+    else if (left >= 3) { spmv_prec<B, 3>(A, x + o, y + o, bs, dd, wdot, rw, s, skip); done += 3; }
+    else { spmv_prec<B, 1>(A, x + o, y + o, bs, dd, wdot, rw, s, skip); done += 1; }
+  }
+}
+
+void launch_spmv(const DevBsr& A, const double* x, double* y, const double* bsub, int nrhs, double* dot_out,
+                 const double* wdot, RedWork& rw, hipStream_t s, const int* skip) {
+  switch (A.b) {
+    case 3: spmv_nr<3>(A, x, y, bsub, nrhs, dot_out, wdot, rw, s, skip); break;
+    case 6: spmv_nr<6>(A, x, y, bsub, nrhs, dot_out, wdot, rw, s, skip); break;
+    case 8: spmv_nr<8>(A, x, y, bsub, nrhs, dot_out, wdot, rw, s, skip); break;
+    case 9: spmv_nr<9>(A, x, y, bsub, nrhs, dot_out, wdot, rw, s, skip); break;
+    case 12: spmv_nr<12>(A, x, y, bsub, nrhs, dot_out, wdot, rw, s, skip); break;
+    case 16: spmv_nr<16>(A, x, y, bsub, nrhs, dot_out, wdot, rw, s, skip); break;
+    case 24: spmv_nr<24>(A, x, y, bsub, nrhs, dot_out, wdot, rw, s, skip); break;
+    case 32: spmv_nr<32>(A, x, y, bsub, nrhs, dot_out, wdot, rw, s, skip); break;
+    default: throw Error(kErrArg, "unsupported block size for the device SpMV (use 3,6,8,9,12,16,24,32)");
+  }
+}
+
+template <int B, int NR, class TA, class TX>
+static void sweep_b(const DevTri& T, const void* x, void* y, SweepSync& sync, hipStream_t s, const int* skip) {
+  const int waves = std::min(T.nsup, std::max(1, device_cu_count() * 8));
+  hipLaunchKernelGGL(k_sweep_arm, dim3(1), dim3(1), 0, s, sync.ctl.as<int>());
+  hipLaunchKernelGGL((k_sweep<B, NR, TA, TX>), dim3(waves), dim3(64), 0, s, T.nsup, T.n, T.forward ? 1 : 0,
+                     T.ia.as<int>(), T.ja.as<int>(), T.off.as<TA>(), T.diag.as<TA>(),
+                     static_cast<const TX*>(x), static_cast<TX*>(y), sync.flags.as<int>(), sync.ctl.as<int>(), skip);
+  BCM_HIP(hipGetLastError());
+}
+
+template <int B, int NR>
+static void sweep_prec(const DevTri& T, const void* x, void* y, int vec_prec, SweepSync& sync, hipStream_t s,
+                       const int* skip) {
+  if (T.prec == kF64 && vec_prec == kF64) sweep_b<B, NR, double, double>(T, x, y, sync, s, skip);
+  else if (T.prec == kF32 && vec_prec == kF32) sweep_b<B, NR, float, float>(T, x, y, sync, s, skip);
+  else if (T.prec == kBF16 && vec_prec == kF32) sweep_b<B, NR, uint16_t, float>(T, x, y, sync, s, skip);
+  else throw Error(kErrArg, "unsupported factor/vector precision pair (fp64/fp64, fp32/fp32, bf16/fp32)");
+}
+
+template <int B>
+static void sweep_nr(const DevTri& T, const void* x, void* y, int vec_prec, int nrhs, SweepSync& sync,
+                     hipStream_t s, const int* skip) {
+  const size_t es = prec_bytes(vec_prec);
+  int done = 0;
+  while (done < nrhs) {
+    const int left = nrhs - done;
+    const char* xs = static_cast<const char*>(x) + (size_t)done * T.n * es;
+    char* ys = static_cast<char*>(y) + (size_t)done * T.n * es;
+    if (left >= 6) { sweep_prec<B, 6>(T, xs, ys, vec_prec, sync, s, skip); done += 6; }
+    else if (left >= 3) { sweep_prec<B, 3>(T, xs, ys, vec_prec, sync, s, skip); done += 3; }
+    else { sweep_prec<B, 1>(T, xs, ys, vec_prec, sync, s, skip); done += 1; }
+  }
+}
+
+void launch_sweep(const DevTri& T, const void* x, void* y, int vec_prec, int nrhs, SweepSync& sync,
+                  hipStream_t s, const int* skip) {
+  sync.ensure(T.nsup);
+  switch (T.b) {
+    case 3: sweep_nr<3>(T, x, y, vec_prec, nrhs, sync, s, skip); break;
+    case 6: sweep_nr<6>(T, x, y, vec_prec, nrhs, sync, s, skip); break;
+    case 8: sweep_nr<8>(T, x, y, vec_prec, nrhs, sync, s, skip); break;
+    case 9: sweep_nr<9>(T, x, y, vec_prec, nrhs, sync, s, skip); break;
+    case 12: sweep_nr<12>(T, x, y, vec_prec, nrhs, sync, s, skip); break;
+    case 16: sweep_nr<16>(T, x, y, vec_prec, nrhs, sync, s, skip); break;
+    case 24: sweep_nr<24>(T, x, y, vec_prec, nrhs, sync, s, skip); break;
+    case 32: sweep_nr<32>(T, x, y, vec_prec, nrhs, sync, s, skip); break;
+    default: throw Error(kErrArg, "unsupported block size for the device sweep (use 3,6,8,9,12,16,24,32)");
+  }
+}
+
+void launch_multidot(int n, int k, const double* const* xs, const double* const* ys, double* out, RedWork& rw,
+                     hipStream_t s, const int* skip) {
+  if (k < 1 || k > kMaxDots) throw Error(kErrArg, "multidot: 1..8 products");
+  rw.ensure();
+  DotArgs args{};
+  for (int i = 0; i < k; ++i) { args.x[i] = xs[i]; args.y[i] = ys[i]; }
+  const int grid = std::min(grid_for(n, 256), 1024);
+  double* p = rw.partials.as<double>();
+  unsigned* c = rw.counter.as<unsigned>();
+  switch (k) {
+    case 1: hipLaunchKernelGGL(k_multidot<1>, dim3(grid), dim3(256), 0, s, (int64_t)n, args, p, c, out, skip); break;
+    case 2: hipLaunchKernelGGL(k_multidot<2>, dim3(grid), dim3(256), 0, s, (int64_t)n, args, p, c, out, skip); break;
+    case 3: hipLaunchKernelGGL(k_multidot<3>, dim3(grid), dim3(256), 0, s, (int64_t)n, args, p, c, out, skip); break;
+    case 4: hipLaunchKernelGGL(k_multidot<4>, dim3(grid), dim3(256), 0, s, (int64_t)n, args, p, c, out, skip); break;
+    case 5: hipLaunchKernelGGL(k_multidot<5>, dim3(grid), dim3(256), 0, s, (int64_t)n, args, p, c, out, skip); break;
+    case 6: hipLaunchKernelGGL(k_multidot<6>, dim3(grid), dim3(256), 0, s, (int64_t)n, args, p, c, out, skip); break;
+    case 7: hipLaunchKernelGGL(k_multidot<7>, dim3(grid), dim3(256), 0, s, (int64_t)n, args, p, c, out, skip); break;
+    default: hipLaunchKernelGGL(k_multidot<8>, dim3(grid), dim3(256), 0, s, (int64_t)n, args, p, c, out, skip); break;
+  }
+  BCM_HIP(hipGetLastError());
+}
+
+void launch_convert(int64_t n, const void* src, int sp, void* dst, int dp, hipStream_t s) {
+  const int grid = grid_for(n, 256);
+  if (sp == kF64 && dp == kF32)
+    hipLaunchKernelGGL((k_convert<double, float>), dim3(grid), dim3(256), 0, s, n, (const double*)src, (float*)dst);
+  else if (sp == kF32 && dp == kF64)
+    hipLaunchKernelGGL((k_convert<float, double>), dim3(grid), dim3(256), 0, s, n, (const float*)src, (double*)dst);
+  else if (sp == dp)
+    BCM_HIP(hipMemcpyAsync(dst, src, (size_t)n * prec_bytes(sp), hipMemcpyDeviceToDevice, s));
+  else
+    throw Error(kErrArg, "convert: unsupported precision pair");
+  BCM_HIP(hipGetLastError());
+}
+
+void launch_axpby(int64_t n, double alpha, const double* x, double beta, double* y, hipStream_t s) {
+  hipLaunchKernelGGL(k_axpby, dim3(grid_for(n, 256)), dim3(256), 0, s, n, alpha, x, beta, y);
+  BCM_HIP(hipGetLastError());
+}
+
+void launch_copy(int64_t n, const double* x, double* y, hipStream_t s) {
+  hipLaunchKernelGGL(k_copy, dim3(grid_for(n, 256)), dim3(256), 0, s, n, x, y);
+  BCM_HIP(hipGetLastError());
+}
+
+void launch_fill(int64_t n, double v, double* y, hipStream_t s) {
+  hipLaunchKernelGGL(k_fill, dim3(grid_for(n, 256)), dim3(256), 0, s, n, v, y);
+  BCM_HIP(hipGetLastError());
+}
+
+}  // namespace bcm

@@ -1,0 +1,467 @@
+// GPU-resident Krylov drivers: preconditioned CG (the reference's serial Cg,
+// cg.cpp:31-349, and its parallel form cg.cpp:355-771) and right-preconditioned
+// BiCGStab; plus the device images of matrices and factors and the
+// preconditioner application.
+//
+// Every recurrence scalar lives in device memory (KState). A solver iteration
+// is a fixed sequence of launches; scalar updates run in one-thread kernels
+// between the vector kernels, so no vector kernel ever races with a scalar
+// write. When the device-side test marks the solve finished, every kernel
+// still queued returns at entry, so the host only polls the state every few
+// iterations and the reported iteration count is exact.
+#include <chrono>
+#include <cmath>
+#include <cstring>
+
+#include "dev_common.hpp"
+#include "solver.hpp"
+
+namespace bcm {
+
+namespace {
+
+enum ScalarOp : int {
+  kOpPcgCheck = 1,
+  kOpPcgAlpha = 2,
+  kOpBicgInit = 10,
+  kOpBicgAlpha = 11,
+  kOpBicgSCheck = 12,
+  kOpBicgOmega = 13,
+  kOpBicgNext = 14,
+  kOpCgParInit = 20,
+  kOpCgParStep = 21,
+};
+
+// Scalar slots (KState::sc)
+enum : int {
+  // PCG
+  kGamold = 1, kPq = 3, kEnrsq0 = 4, kBeta = 5, kAlpha = 6, kEnrsq = 8, kGamma = 9,
+  // BiCGStab
+  kRho = 0, kRhoOld = 1, kBAlpha = 2, kOmega = 3, kRv = 4, kSs = 5, kTs = 6, kTt = 7, kRr = 8,
+  kRhoNew = 10,
+  // shared
+  kBnorm2 = 20, kR0norm2 = 21, kTrue0 = 22, kTrue1 = 23,
+};
+
+__device__ double threshold(const KState* ks, int ittype, double eps2) {
+  if (ittype == 0) return eps2 * ks->sc[kR0norm2];
+  if (ittype == 1) return eps2;
+  return eps2 * ks->sc[kBnorm2];
+}
+
+__global__ void k_state_reset(KState* ks, int keep_scalars) {
+  const int t = threadIdx.x;
+  if (!keep_scalars && t < 32) ks->sc[t] = 0.0;
+  if (t < 8) ks->st[t] = 0;
+}
+
+__global__ void k_scalar(int op, KState* ks, double eps2, int ittype, int niter) {
+  double* sc = ks->sc;
+  int* st = ks->st;
+  if (st[0]) return;
+  switch (op) {
+    case kOpPcgCheck: {
+      const int k = ++st[1];
+      if (k == 1) sc[kEnrsq0] = sc[kEnrsq];
+      const double thr = ittype == 0 ? eps2 * sc[kEnrsq0] : threshold(ks, ittype, eps2);
+      if (!(sc[kEnrsq] > thr)) { st[0] = 1; st[4] = 1; return; }
+      if (k >= niter) { st[0] = 1; return; }
+      if (!(sc[kGamma] > 0.0)) { st[0] = 1; st[2] = 1; return; }
+      sc[kBeta] = k > 1 ? sc[kGamma] / sc[kGamold] : 0.0;
+      break;
+    }
+    case kOpPcgAlpha: {
+      const double alpha = sc[kGamma] / sc[kPq];
+      if (!(alpha >= 0.0)) { st[0] = 1; st[2] = 1; return; }  // "Matrix is not positive definite"
+      sc[kAlpha] = alpha;
+      sc[kGamold] = sc[kGamma];
+      break;
+    }
+    case kOpBicgInit: {
+      sc[kRho] = sc[kRhoNew];
+      sc[kRhoOld] = 1.0;
+      sc[kBAlpha] = 1.0;
+      sc[kOmega] = 1.0;
+      sc[kR0norm2] = sc[kRr];
+      if (!(sc[kRr] > threshold(ks, ittype, eps2))) { st[0] = 1; st[4] = 1; }
+      break;
+    }
+    case kOpBicgAlpha: {
+      const double a = sc[kRho] / sc[kRv];
+      if (!(std::isfinite(a)) || sc[kRv] == 0.0) { st[0] = 1; st[2] = 1; return; }
+      sc[kBAlpha] = a;
+      break;
+    }
+    case kOpBicgSCheck: {
+      if (!(sc[kSs] > threshold(ks, ittype, eps2))) { st[0] = 1; st[3] = 1; st[4] = 1; ++st[1]; }
+      break;
+    }
+    case kOpBicgOmega: {
+      const double w = sc[kTs] / sc[kTt];
+      if (!(std::isfinite(w)) || w == 0.0) { st[0] = 1; st[2] = 1; return; }
+      sc[kOmega] = w;
+      break;
+    }
+    case kOpBicgNext: {
+      const int k = ++st[1];
+      sc[kRhoOld] = sc[kRho];
+      sc[kRho] = sc[kRhoNew];
+      if (!(sc[kRr] > threshold(ks, ittype, eps2))) { st[0] = 1; st[4] = 1; return; }
+      if (k >= niter) { st[0] = 1; return; }
+      if (sc[kRho] == 0.0 || !std::isfinite(sc[kRho])) { st[0] = 1; st[2] = 1; return; }
+      break;
+    }
+    default: break;
+  }
+}
+
+// ---- PCG vector kernels ----
+__global__ void k_pcg_p(int64_t n, const KState* ks, const double* __restrict__ z, double* __restrict__ p) {
+  if (ks->st[0]) return;
+  const double beta = ks->sc[kBeta];
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride)
+    p[i] = p[i] * beta + 1.0 * z[i];
+}
+
+__global__ void k_pcg_xr(int64_t n, const KState* ks, const double* __restrict__ p, const double* __restrict__ q,
+                         double* __restrict__ x, double* __restrict__ r) {
+  if (ks->st[0]) return;
+  const double alpha = ks->sc[kAlpha];
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+    r[i] = r[i] + (-alpha) * q[i];
+    x[i] = x[i] + alpha * p[i];
+  }
+}
+
+// ---- BiCGStab vector kernels ----
+__global__ void k_bicg_p(int64_t n, const KState* ks, const double* __restrict__ r, const double* __restrict__ v,
+                         double* __restrict__ p) {
+  if (ks->st[0]) return;
+  const double beta = (ks->sc[kRho] / ks->sc[kRhoOld]) * (ks->sc[kBAlpha] / ks->sc[kOmega]);
+  const double w = ks->sc[kOmega];
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride)
+    p[i] = r[i] + beta * (p[i] - w * v[i]);
+}
+
+__global__ __launch_bounds__(256) void k_bicg_s(int64_t n, KState* ks, const double* __restrict__ r,
+                                                const double* __restrict__ v, double* __restrict__ s,
+                                                double* partials, unsigned* counter) {
+  if (ks->st[0]) return;
+  const double a = ks->sc[kBAlpha];
+  double acc[1] = {0.0};
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+    const double si = r[i] - a * v[i];
+    s[i] = si;
+    acc[0] += si * si;
+  }
+  reduce_finalize<1>(acc, partials, counter, &ks->sc[kSs]);
+}
+
+__global__ void k_bicg_xs(int64_t n, const KState* ks, double thr_eps2, int ittype, const double* __restrict__ phat,
+                          double* __restrict__ x) {
+  if (ks->st[0]) return;
+  if (ks->sc[kSs] > threshold(ks, ittype, thr_eps2)) return;
+  const double a = ks->sc[kBAlpha];
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) x[i] += a * phat[i];
+}
+
+__global__ __launch_bounds__(256) void k_bicg_xr(int64_t n, KState* ks, const double* __restrict__ phat,
+                                                 const double* __restrict__ shat, const double* __restrict__ s,
+                                                 const double* __restrict__ t, const double* __restrict__ rhat,
+                                                 double* __restrict__ x, double* __restrict__ r, double* partials,
+                                                 unsigned* counter) {
+  if (ks->st[0]) return;
+  const double a = ks->sc[kBAlpha], w = ks->sc[kOmega];
+  double acc[2] = {0.0, 0.0};
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+    x[i] = x[i] + a * phat[i] + w * shat[i];
+    const double ri = s[i] - w * t[i];
+    r[i] = ri;
+    acc[0] += ri * ri;
+    acc[1] += rhat[i] * ri;
+  }
+  // finalises into sc[kRr] and sc[kRr + 1]; kRhoNew is copied by the scalar step
+  reduce_finalize<2>(acc, partials, counter, &ks->sc[kRr]);
+}
+
+__global__ void k_move(KState* ks, int from, int to) { ks->sc[to] = ks->sc[from]; }
+
+int ew_grid(int64_t n) {
+  int64_t g = (n + 255) / 256;
+  return (int)std::max<int64_t>(1, std::min<int64_t>(g, 1024));
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// Device images
+
+void Matrix::to_device(bool with_transpose) {
+  if (!has_d) {
+    if (symmetric) {
+      BlockCsr full = symm_mtr(h);
+      dev_bsr_upload(full, kF64, d);
+      if (with_transpose) { dev_bsr_upload(full, kF64, dt); has_dt = true; }
+    } else {
+      dev_bsr_upload(h, kF64, d);
+    }
+    has_d = true;
+  }
+  if (with_transpose && !has_dt) {
+    dev_bsr_upload(symmetric ? symm_mtr(h) : transp_mtr(h), kF64, dt);
+    has_dt = true;
+  }
+}
+
+void Factor::to_device(int prec, bool with_transpose) {
+  if (dev_prec != prec) {
+    dev_tri_forward(lower(), prec, fwd);
+    dev_tri_backward(u, prec, bwd);
+    dev_prec = prec;
+    has_t = false;
+  }
+  if (with_transpose && !has_t) {
+    if (kind == 0) {
+      dev_tri_forward(u, prec, fwdt);
+      dev_tri_backward(u, prec, bwdt);
+    } else {
+      dev_tri_forward(u, prec, fwdt);
+      dev_tri_backward(l, prec, bwdt);
+    }
+    has_t = true;
+  }
+}
+
+Context::Context() {
+  BCM_HIP(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+  own_stream = true;
+  kstate.alloc(sizeof(KState));
+  BCM_HIP(hipMemset(kstate.p, 0, sizeof(KState)));
+  BCM_HIP(hipHostMalloc((void**)&host_state, sizeof(KState), hipHostMallocDefault));
+  std::memset(host_state, 0, sizeof(KState));
+  rw.ensure();
+}
+
+Context::~Context() {
+  if (stream) (void)hipStreamSynchronize(stream);
+  if (host_state) (void)hipHostFree(host_state);
+  if (own_stream && stream) (void)hipStreamDestroy(stream);
+}
+
+double* Context::vec(int i, size_t nbytes) {
+  if ((int)vecs.size() <= i) vecs.resize(i + 1);
+  if (vecs[i].bytes < nbytes) vecs[i].alloc(nbytes);
+  return vecs[i].as<double>();
+}
+
+void Context::pull_state() {
+  BCM_HIP(hipMemcpyAsync(host_state, kstate.p, sizeof(KState), hipMemcpyDeviceToHost, stream));
+  BCM_HIP(hipStreamSynchronize(stream));
+}
+
+void Context::check_sweep_error() {
+  if (!sync.ctl.p) return;
+  int ctl[4] = {0, 0, 0, 0};
+  BCM_HIP(hipMemcpyAsync(ctl, sync.ctl.p, sizeof(ctl), hipMemcpyDeviceToHost, stream));
+  BCM_HIP(hipStreamSynchronize(stream));
+  if (ctl[2]) throw Error(kErrInternal, "triangular sweep: dependency wait exceeded its spin limit");
+}
+
+void precond_apply(Factor& F, Context& ctx, const double* r, double* z, int prec, bool transpose, int nrhs,
+                   const int* skip) {
+  F.to_device(prec, transpose);
+  const DevTri& t1 = transpose ? F.fwdt : F.fwd;
+  const DevTri& t2 = transpose ? F.bwdt : F.bwd;
+  const int64_t n = t1.n;
+  if (prec == kF64) {
+    double* tmp = ctx.vec(12, (size_t)n * nrhs * 8);
+    launch_sweep(t1, r, tmp, kF64, nrhs, ctx.sync, ctx.stream, skip);
+    launch_sweep(t2, tmp, z, kF64, nrhs, ctx.sync, ctx.stream, skip);
+  } else {
+    float* r32 = reinterpret_cast<float*>(ctx.vec(13, (size_t)n * nrhs * 4));
+    float* t32 = reinterpret_cast<float*>(ctx.vec(14, (size_t)n * nrhs * 4));
+    launch_convert(n * nrhs, r, kF64, r32, kF32, ctx.stream);
+    launch_sweep(t1, r32, t32, kF32, nrhs, ctx.sync, ctx.stream, skip);
+    launch_sweep(t2, t32, r32, kF32, nrhs, ctx.sync, ctx.stream, skip);
+    launch_convert(n * nrhs, r32, kF32, z, kF64, ctx.stream);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Drivers
+
+namespace {
+
+struct TrueResidual {
+  double relres = 0.0, bnorm = 0.0;
+};
+
+// relres = ||b - A x|| / ||b|| in fp64 on the device; leaves b - A x in r.
+TrueResidual true_residual(Matrix& A, Context& ctx, const double* b, const double* x, double* r) {
+  const int n = A.d.n;
+  KState* ks = ctx.dstate();
+  launch_spmv(A.d, x, r, b, 1, nullptr, nullptr, ctx.rw, ctx.stream);
+  const double* xs[2] = {r, b};
+  const double* ys[2] = {r, b};
+  launch_multidot(n, 2, xs, ys, &ks->sc[kTrue0], ctx.rw, ctx.stream);
+  ctx.pull_state();
+  TrueResidual t;
+  t.bnorm = std::sqrt(ctx.host_state->sc[kTrue1]);
+  t.relres = t.bnorm > 0 ? std::sqrt(ctx.host_state->sc[kTrue0]) / t.bnorm : std::sqrt(ctx.host_state->sc[kTrue0]);
+  return t;
+}
+
+void pcg(Matrix& A, Factor& F, Context& ctx, const SolveOptions& opt, const double* b, double* x, SolveStats& st) {
+  const int n = A.d.n;
+  const size_t vb = (size_t)n * 8;
+  double* r = ctx.vec(0, vb);
+  double* z = ctx.vec(1, vb);
+  double* p = ctx.vec(2, vb);
+  double* q = ctx.vec(3, vb);
+  KState* ks = ctx.dstate();
+  const int* done = &ks->st[0];
+  const double eps2 = opt.eps * opt.eps;
+  hipStream_t s = ctx.stream;
+  int total = 0;
+  for (int attempt = 0; attempt <= opt.max_restarts; ++attempt) {
+    hipLaunchKernelGGL(k_state_reset, dim3(1), dim3(64), 0, s, ks, attempt > 0 ? 1 : 0);
+    launch_spmv(A.d, x, r, b, 1, nullptr, nullptr, ctx.rw, s);  // r = b - A x
+    launch_fill(n, 0.0, p, s);
+    {
+      const double* xs[2] = {b, r};
+      const double* ys[2] = {b, r};
+      launch_multidot(n, 2, xs, ys, &ks->sc[kBnorm2], ctx.rw, s);  // sc[20] = b.b, sc[21] = r0.r0
+    }
+    int it = 0;
+    while (true) {
+      for (int c = 0; c < opt.check_every; ++c) {
+        precond_apply(F, ctx, r, z, opt.prec, false, 1, done);
+        const double* xs[2] = {r, r};
+        const double* ys[2] = {r, z};
+        launch_multidot(n, 2, xs, ys, &ks->sc[kEnrsq], ctx.rw, s, done);  // (r,r) -> 8, (r,z) -> 9
+        hipLaunchKernelGGL(k_scalar, dim3(1), dim3(1), 0, s, kOpPcgCheck, ks, eps2, opt.ittype, opt.niter - total);
+        hipLaunchKernelGGL(k_pcg_p, dim3(ew_grid(n)), dim3(256), 0, s, (int64_t)n, ks, z, p);
+        launch_spmv(A.d, p, q, nullptr, 1, &ks->sc[kPq], p, ctx.rw, s, done);
+        hipLaunchKernelGGL(k_scalar, dim3(1), dim3(1), 0, s, kOpPcgAlpha, ks, eps2, opt.ittype, opt.niter - total);
+        hipLaunchKernelGGL(k_pcg_xr, dim3(ew_grid(n)), dim3(256), 0, s, (int64_t)n, ks, p, q, x, r);
+        ++it;
+      }
+      ctx.pull_state();
+      if (ctx.host_state->st[0]) break;
+    }
+    ctx.check_sweep_error();
+    const KState& hs = *ctx.host_state;
+    const int k = hs.st[1];
+    total += std::max(0, k - 1);
+    st.breakdown = hs.st[2];
+    st.relres_rec = hs.sc[kBnorm2] > 0 ? std::sqrt(hs.sc[kEnrsq] / hs.sc[kBnorm2]) : std::sqrt(hs.sc[kEnrsq]);
+    TrueResidual tr = true_residual(A, ctx, b, x, r);
+    st.relres_true = tr.relres;
+    const bool ok = opt.ittype == 3 ? tr.relres <= opt.eps : hs.st[4] != 0;
+    st.converged = ok ? 1 : 0;
+    if (ok || st.breakdown || total >= opt.niter) break;
+    st.restarts++;
+  }
+  st.iterations = total;
+}
+
+void bicgstab(Matrix& A, Factor& F, Context& ctx, const SolveOptions& opt, const double* b, double* x,
+              SolveStats& st) {
+  const int n = A.d.n;
+  const size_t vb = (size_t)n * 8;
+  double* r = ctx.vec(0, vb);
+  double* rhat = ctx.vec(1, vb);
+  double* p = ctx.vec(2, vb);
+  double* v = ctx.vec(3, vb);
+  double* phat = ctx.vec(4, vb);
+  double* sv = ctx.vec(5, vb);
+  double* shat = ctx.vec(6, vb);
+  double* t = ctx.vec(7, vb);
+  KState* ks = ctx.dstate();
+  const int* done = &ks->st[0];
+  const double eps2 = opt.eps * opt.eps;
+  hipStream_t s = ctx.stream;
+  const int ge = ew_grid(n);
+  int total = 0;
+  for (int attempt = 0; attempt <= opt.max_restarts; ++attempt) {
+    hipLaunchKernelGGL(k_state_reset, dim3(1), dim3(64), 0, s, ks, 0);
+    launch_spmv(A.d, x, r, b, 1, nullptr, nullptr, ctx.rw, s);
+    launch_copy(n, r, rhat, s);
+    launch_fill(n, 0.0, p, s);
+    launch_fill(n, 0.0, v, s);
+    {
+      const double* xs[3] = {b, r, rhat};
+      const double* ys[3] = {b, r, r};
+      launch_multidot(n, 3, xs, ys, &ks->sc[kBnorm2], ctx.rw, s);  // 20: b.b 21: r.r 22: rhat.r
+      hipLaunchKernelGGL(k_move, dim3(1), dim3(1), 0, s, ks, kR0norm2, kRr);
+      hipLaunchKernelGGL(k_move, dim3(1), dim3(1), 0, s, ks, 22, kRhoNew);
+      hipLaunchKernelGGL(k_scalar, dim3(1), dim3(1), 0, s, kOpBicgInit, ks, eps2, opt.ittype, opt.niter);
+    }
+    while (true) {
+      for (int c = 0; c < opt.check_every; ++c) {
+        hipLaunchKernelGGL(k_bicg_p, dim3(ge), dim3(256), 0, s, (int64_t)n, ks, r, v, p);
+        precond_apply(F, ctx, p, phat, opt.prec, false, 1, done);
+        launch_spmv(A.d, phat, v, nullptr, 1, &ks->sc[kRv], rhat, ctx.rw, s, done);
+        hipLaunchKernelGGL(k_scalar, dim3(1), dim3(1), 0, s, kOpBicgAlpha, ks, eps2, opt.ittype, opt.niter);
+        hipLaunchKernelGGL(k_bicg_s, dim3(ge), dim3(256), 0, s, (int64_t)n, ks, r, v, sv,
+                           ctx.rw.partials.as<double>(), ctx.rw.counter.as<unsigned>());
+        hipLaunchKernelGGL(k_bicg_xs, dim3(ge), dim3(256), 0, s, (int64_t)n, ks, eps2, opt.ittype, phat, x);
+        hipLaunchKernelGGL(k_scalar, dim3(1), dim3(1), 0, s, kOpBicgSCheck, ks, eps2, opt.ittype, opt.niter);
+        precond_apply(F, ctx, sv, shat, opt.prec, false, 1, done);
+        launch_spmv(A.d, shat, t, nullptr, 1, nullptr, nullptr, ctx.rw, s, done);
+        {
+          const double* xs[2] = {t, t};
+          const double* ys[2] = {sv, t};
+          launch_multidot(n, 2, xs, ys, &ks->sc[kTs], ctx.rw, s, done);
+        }
+        hipLaunchKernelGGL(k_scalar, dim3(1), dim3(1), 0, s, kOpBicgOmega, ks, eps2, opt.ittype, opt.niter);
+        hipLaunchKernelGGL(k_bicg_xr, dim3(ge), dim3(256), 0, s, (int64_t)n, ks, phat, shat, sv, t, rhat, x, r,
+                           ctx.rw.partials.as<double>(), ctx.rw.counter.as<unsigned>());
+        hipLaunchKernelGGL(k_move, dim3(1), dim3(1), 0, s, ks, kRr + 1, kRhoNew);
+        hipLaunchKernelGGL(k_scalar, dim3(1), dim3(1), 0, s, kOpBicgNext, ks, eps2, opt.ittype,
+                           opt.niter - total);
+      }
+      ctx.pull_state();
+      if (ctx.host_state->st[0]) break;
+    }
+    ctx.check_sweep_error();
+    const KState& hs = *ctx.host_state;
+    total += hs.st[1];
+    st.breakdown = hs.st[2];
+    st.relres_rec = hs.sc[kBnorm2] > 0 ? std::sqrt((hs.st[3] ? hs.sc[kSs] : hs.sc[kRr]) / hs.sc[kBnorm2]) : 0.0;
+    TrueResidual tr = true_residual(A, ctx, b, x, r);
+    st.relres_true = tr.relres;
+    const bool ok = opt.ittype == 3 ? tr.relres <= opt.eps : hs.st[4] != 0;
+    st.converged = ok ? 1 : 0;
+    if (ok || total >= opt.niter) break;
+    st.restarts++;
+  }
+  st.iterations = total;
+}
+
+}  // namespace
+
+void solve(Matrix& A, Factor& F, Context& ctx, const SolveOptions& opt, const double* b, double* x,
+           SolveStats& st) {
+  auto t0 = std::chrono::steady_clock::now();
+  st = SolveStats();
+  A.to_device(opt.kind == kLanczos);
+  if (A.d.n != F.u.n()) throw Error(kErrArg, "matrix and factor sizes differ");
+  switch (opt.kind) {
+    case kPcg: pcg(A, F, ctx, opt, b, x, st); break;
+    case kBicgstab: bicgstab(A, F, ctx, opt, b, x, st); break;
+    case kFgmres: solve_fgmres(A, F, ctx, opt, b, x, st); break;
+    case kLanczos: solve_lanczos(A, F, ctx, opt, b, x, st); break;
+    default: throw Error(kErrArg, "solver kind not available");
+  }
+  BCM_HIP(hipStreamSynchronize(ctx.stream));
+  st.seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+}
+
+}  // namespace bcm

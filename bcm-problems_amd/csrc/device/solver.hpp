@@ -1,0 +1,107 @@
+// Handles behind the C ABI: matrices, factors and the per-stream solver
+// context, plus the GPU-resident Krylov drivers.
+#pragma once
+
+#include <memory>
+
+#include "device.hpp"
+
+namespace bcm {
+
+// A block matrix as the caller sees it (host copy, reference layout) plus
+// its device images. A symmetric matrix is kept upper-stored on the host
+// (as the reference's AbstractSymmSolver assembles it) and expanded to full
+// storage on the device.
+struct Matrix {
+  BlockCsr h;
+  bool symmetric = false;
+  DevBsr d;       // full matrix on device
+  DevBsr dt;      // transpose (two-sided Lanczos only)
+  bool has_d = false, has_dt = false;
+  void to_device(bool with_transpose);
+};
+
+// IC2 / ILU2 factor: host storage in the reference layout and the device
+// sweeps of M^-1 (fwd then bwd) and, on request, of M^-T (fwdt then bwdt).
+struct Factor {
+  int kind = 0;            // 0 IC2: A ~ U^T U (U only); 1 ILU2: A ~ L U
+  BlockCsr l, u;           // L by columns, U by rows, inverted diagonal blocks
+  FactorStats stats;
+  std::vector<int> order;  // ordering the factor was computed in (order[old] = new), may be empty
+  DevTri fwd, bwd, fwdt, bwdt;
+  int dev_prec = -1;
+  bool has_t = false;
+  void to_device(int prec, bool with_transpose);
+  const BlockCsr& lower() const { return kind == 0 ? u : l; }
+};
+
+// Scalars and flags of one running solver, in device memory.
+struct KState {
+  double sc[32];
+  int st[8];  // [0] done, [1] iteration, [2] breakdown, [3] converged-on-s (BiCGStab)
+};
+
+struct Context {
+  hipStream_t stream = nullptr;
+  bool own_stream = false;
+  SweepSync sync;
+  RedWork rw;
+  DevBuf kstate;        // KState
+  KState* host_state = nullptr;  // pinned mirror
+  std::vector<DevBuf> vecs;      // solver work vectors
+  Context();
+  ~Context();
+  KState* dstate() const { return kstate.as<KState>(); }
+  double* vec(int i, size_t nbytes);  // returns work vector i of at least nbytes
+  void pull_state();                  // copy state to the pinned mirror and wait
+  void check_sweep_error();
+};
+
+enum SolverKind : int {
+  kPcg = 0,          // reference serial Cg (cg.cpp:31-349)
+  kBicgstab = 1,     // right-preconditioned BiCGStab (no reference counterpart)
+  kFgmres = 2,       // flexible GMRES(m), mixed-precision preconditioner
+  kLanczos = 3,      // reference two-sided Lanczos (lanczos.cpp:34-371)
+  kCgPar = 4,        // reference parallel-form Cg (cg.cpp:355-771)
+  kBlockPcg = 5,     // block PCG over several right-hand sides
+};
+
+struct SolveOptions {
+  int kind = kPcg;
+  int niter = 1000;
+  double eps = 1e-10;
+  int ittype = 3;        // 0: ||r|| <= eps ||r0||, 1: ||r|| <= eps, 3: ||r|| <= eps ||b||
+  int check_every = 8;   // host polls the device state every k iterations
+  int restart = 30;      // FGMRES cycle length
+  int prec = kF64;       // precision of the preconditioner application
+  int max_restarts = 3;  // true-residual restarts when the recursion drifted
+};
+
+struct SolveStats {
+  int iterations = 0;
+  int restarts = 0;
+  int breakdown = 0;
+  int converged = 0;
+  double relres_true = 0.0;   // ||b - A x|| / ||b|| recomputed in fp64 at the end
+  double relres_rec = 0.0;    // from the recursion
+  double seconds = 0.0;
+  int launches = 0;
+};
+
+// Solves A x = b for one right-hand side (b, x device pointers, x holds x0).
+void solve(Matrix& A, Factor& F, Context& ctx, const SolveOptions& opt, const double* b, double* x,
+           SolveStats& st);
+
+// Applies M^-1 (or M^-T) in the given precision: z = M^-1 r.
+void precond_apply(Factor& F, Context& ctx, const double* r, double* z, int prec, bool transpose, int nrhs,
+                   const int* skip = nullptr);
+
+// FGMRES(m) driver (fgmres.cpp).
+void solve_fgmres(Matrix& A, Factor& F, Context& ctx, const SolveOptions& opt, const double* b, double* x,
+                  SolveStats& st);
+
+// Two-sided Lanczos driver (lanczos.hip).
+void solve_lanczos(Matrix& A, Factor& F, Context& ctx, const SolveOptions& opt, const double* b, double* x,
+                   SolveStats& st);
+
+}  // namespace bcm

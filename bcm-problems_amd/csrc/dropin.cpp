@@ -1,0 +1,226 @@
+// Drop-in entry points: the reference's AbstractSymmSolver / AbstractSolver
+// (solvers.cpp:5383-5586, :5591-5794) and the dispatchers they call
+// (CSVector::SolveSymm solvers.cpp:4452-4610, CSVector::Solver :1568-1860),
+// with the Krylov phase on the GPU.
+//
+// Steps kept from the reference: callback assembly (rows sorted by column,
+// blocks copied in sorted order), RCM profile ordering (ordtype > 0),
+// symmetric / general reordering, IC2 (symmetric) or CombineLU + ILU2
+// (general), ordering of the right-hand side and initial guess, the Krylov
+// solve, inverse ordering and CSTORESOL per block.
+#include "../../include/bcmhip.h"
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cstring>
+#include <numeric>
+#include <string>
+
+#include "device/solver.hpp"
+#include "host/bcm_host.hpp"
+#include "internal.hpp"
+
+namespace {
+
+struct Callbacks {
+  void* obj;
+  bcm_cb_nsup fnsup;
+  bcm_cb_sprnds fsprnds;
+  bcm_cb_iaja fiaja;
+  bcm_cb_blockrow fblockrow;
+  bcm_cb_rhs frhs;
+  bcm_cb_iniguess finiguess;
+  bcm_cb_storesol fstoresol;
+};
+
+// Assembles the block matrix exactly as AbstractSolver does: the callback
+// returns each row's blocks in its own ja order; rows are re-sorted by column
+// and blocks copied in the sorted order (solvers.cpp:5683-5736).
+bcm::BlockCsr assemble(const Callbacks& cb) {
+  int nsup = 0;
+  cb.fnsup(cb.obj, &nsup);
+  if (nsup <= 0) throw bcm::Error(bcm::kErrArg, "callback reported no block rows");
+  std::vector<int> sprnd(nsup + 1);
+  cb.fsprnds(cb.obj, sprnd.data());
+  int* ia = nullptr;
+  int* ja = nullptr;
+  cb.fiaja(cb.obj, &ia, &ja);
+  if (!ia || !ja) throw bcm::Error(bcm::kErrArg, "callback returned no sparsity");
+  bcm::BlockCsr m;
+  try {
+    m.nsup = nsup;
+    m.sprnd = sprnd;
+    m.ia.assign(ia, ia + nsup + 1);
+    m.ja.resize(m.ia[nsup]);
+    m.bsa.resize(m.ia[nsup]);
+    size_t rowmax = 0, total = 0;
+    for (int i = 0; i < nsup; ++i) {
+      size_t row = 0;
+      for (int j = ia[i]; j < ia[i + 1]; ++j) {
+        if (ja[j] < 0 || ja[j] >= nsup) throw bcm::Error(bcm::kErrArg, "callback column index out of range");
+        row += (size_t)m.bla(i) * m.bla(ja[j]);
+      }
+      rowmax = std::max(rowmax, row);
+      total += row;
+    }
+    m.a.resize(total);
+    std::vector<double> arow(std::max<size_t>(rowmax, 1));
+    std::vector<size_t> off;
+    std::vector<int> idx;
+    size_t nz = 0;
+    for (int i = 0; i < nsup; ++i) {
+      const int nzj = ia[i + 1] - ia[i];
+      const int bi = m.bla(i);
+      off.assign(nzj, 0);
+      size_t pos = 0;
+      for (int k = 0; k < nzj; ++k) {
+        off[k] = pos;
+        pos += (size_t)bi * m.bla(ja[ia[i] + k]);
+      }
+      cb.fblockrow(cb.obj, i, arow.data());
+      idx.resize(nzj);
+      std::iota(idx.begin(), idx.end(), 0);
+      std::sort(idx.begin(), idx.end(), [&](int x, int y) { return ja[ia[i] + x] < ja[ia[i] + y]; });
+      for (int k = 0; k < nzj; ++k) {
+        const int src = idx[k];
+        const int col = ja[ia[i] + src];
+        const size_t sz = (size_t)bi * m.bla(col);
+        m.ja[ia[i] + k] = col;
+        m.bsa[ia[i] + k] = (int64_t)nz;
+        std::memcpy(&m.a[nz], &arow[off[src]], sz * sizeof(double));
+        nz += sz;
+      }
+    }
+  } catch (...) {
+    delete[] ia;
+    delete[] ja;
+    throw;
+  }
+  delete[] ia;
+  delete[] ja;
+  m.set_blamx();
+  return m;
+}
+
+void gather_blocks(const Callbacks& cb, const bcm::BlockCsr& m, bcm_cb_rhs f, std::vector<double>& v) {
+  v.assign(m.n(), 0.0);
+  for (int i = 0; i < m.nsup; ++i) f(cb.obj, i, &v[m.sprnd[i]]);
+}
+
+struct DevVec {
+  double* p = nullptr;
+  explicit DevVec(size_t n) {
+    if (hipMalloc(&p, std::max<size_t>(n, 1) * sizeof(double)) != hipSuccess)
+      throw bcm::Error(bcm::kErrHip, "hipMalloc failed");
+  }
+  ~DevVec() {
+    if (p) (void)hipFree(p);
+  }
+};
+
+int run(bool symmetric, bcm_params* p, const Callbacks& cb) {
+  try {
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0)
+      throw bcm::Error(bcm::kErrNoDevice, "no HIP device: the solve phase runs on the GPU only");
+    bcm_params defaults;
+    bcm_params_default(&defaults);
+    bcm_params& prm = p ? *p : defaults;
+    bcm::SlvParam slv;
+    slv.ordtype = prm.ordtype;
+    slv.memory = prm.memory;
+    slv.tau1 = prm.tau1;
+    slv.tau2 = prm.tau2;
+    slv.theta = prm.theta;
+    slv.sclmin = prm.sclmin;
+    slv.pivmin = prm.pivmin;
+
+    bcm::BlockCsr a = assemble(cb);
+    if (symmetric)
+      for (int i = 0; i < a.nsup; ++i)
+        for (int j = a.ia[i]; j < a.ia[i + 1]; ++j)
+          if (a.ja[j] < i) throw bcm::Error(bcm::kErrArg, "symmetric solver expects upper-triangular sparsity");
+    a.validate(symmetric);
+    std::vector<double> b, x;
+    gather_blocks(cb, a, cb.frhs, b);
+    gather_blocks(cb, a, cb.finiguess, x);
+
+    // ordering: ordtype > 0 -> RCM profile ordering, 0 -> identity
+    std::vector<int> order(a.nsup);
+    if (prm.ordtype != 0) order = bcm::rcm_order(a);
+    else std::iota(order.begin(), order.end(), 0);
+    bcm::Matrix mat;
+    mat.symmetric = symmetric;
+    mat.h = prm.ordtype != 0 ? (symmetric ? bcm::ord_mtr_symm(a, order) : bcm::ord_mtr(a, order)) : a;
+    std::vector<int> sp_new = mat.h.sprnd;
+    if (prm.ordtype != 0) {
+      bcm::ord_vect('D', a.sprnd, sp_new, order, b.data(), a.n(), 1);
+      bcm::ord_vect('D', a.sprnd, sp_new, order, x.data(), a.n(), 1);
+    }
+    bcm::Factor f;
+    if (symmetric) {
+      f.kind = 0;
+      f.u = bcm::ich2(mat.h, slv, &f.stats);
+    } else {
+      bcm::BlockCsr al, au;
+      bcm::combine_lu(mat.h, al, au);
+      f.kind = 1;
+      bcm::ilu2(al, au, slv, f.l, f.u, &f.stats);
+    }
+    prm.timefct = f.stats.seconds;
+    prm.density = f.stats.densu;
+    prm.density2 = f.stats.densr;
+
+    bcm::Context ctx;
+    const size_t n = a.n();
+    DevVec db(n), dx(n);
+    BCM_HIP(hipMemcpy(db.p, b.data(), n * 8, hipMemcpyHostToDevice));
+    BCM_HIP(hipMemcpy(dx.p, x.data(), n * 8, hipMemcpyHostToDevice));
+    bcm::SolveOptions opt;
+    opt.kind = symmetric ? bcm::kPcg : (prm.ittype == 1 ? bcm::kLanczos : bcm::kBicgstab);
+    opt.niter = prm.niter;
+    opt.eps = prm.eps;
+    // Cg's stopping rules (cg.cpp:238-241): 0 relative, 1 absolute, 2 max-norm
+    // (absolute, norm vector of ones); the GPU drivers test the 2-norm for 2 as well.
+    opt.ittype = symmetric ? (prm.ittype == 0 ? 0 : prm.ittype == 3 ? 3 : 1) : (prm.sttype == 1 ? 0 : 1);
+    opt.max_restarts = 0;
+    bcm::SolveStats st;
+    bcm::solve(mat, f, ctx, opt, db.p, dx.p, st);
+    BCM_HIP(hipMemcpy(x.data(), dx.p, n * 8, hipMemcpyDeviceToHost));
+    prm.nitperf = st.iterations;
+    prm.resfin = st.relres_true;
+    prm.timeitr = st.seconds;
+    if (prm.ordtype != 0) bcm::ord_vect('I', a.sprnd, sp_new, order, x.data(), a.n(), 1);
+    for (int i = 0; i < a.nsup; ++i) cb.fstoresol(cb.obj, i, &x[a.sprnd[i]]);
+    if (st.breakdown) throw bcm::Error(bcm::kErrBreakdown, "Krylov breakdown (best iterate stored)");
+    return BCM_OK;
+  } catch (const bcm::Error& e) {
+    bcm::set_last_error(e.what());
+    return e.code;
+  } catch (const std::exception& e) {
+    bcm::set_last_error(e.what());
+    return BCM_E_INTERNAL;
+  }
+}
+
+}  // namespace
+
+extern "C" {
+
+int bcm_abstract_symm_solver(void* pgener, bcm_params* p, bcm_cb_nsup fnsup, bcm_cb_sprnds fsprnds,
+                             bcm_cb_iaja fiaja, bcm_cb_blockrow fblockrow, bcm_cb_rhs frhs,
+                             bcm_cb_iniguess finiguess, bcm_cb_storesol fstoresol) {
+  if (!fnsup || !fsprnds || !fiaja || !fblockrow || !frhs || !finiguess || !fstoresol) return BCM_E_ARG;
+  return run(true, p, Callbacks{pgener, fnsup, fsprnds, fiaja, fblockrow, frhs, finiguess, fstoresol});
+}
+
+int bcm_abstract_solver(void* pgener, bcm_params* p, bcm_cb_nsup fnsup, bcm_cb_sprnds fsprnds, bcm_cb_iaja fiaja,
+                        bcm_cb_blockrow fblockrow, bcm_cb_rhs frhs, bcm_cb_iniguess finiguess,
+                        bcm_cb_storesol fstoresol) {
+  if (!fnsup || !fsprnds || !fiaja || !fblockrow || !frhs || !finiguess || !fstoresol) return BCM_E_ARG;
+  return run(false, p, Callbacks{pgener, fnsup, fsprnds, fiaja, fblockrow, frhs, finiguess, fstoresol});
+}
+
+}  // extern "C"

@@ -1,0 +1,172 @@
+/*
+ * libbcmhip: MI355X (gfx950) solve phase of Bslib's block preconditioned
+ * Krylov library. Plain C ABI: opaque handles, plain pointers and sizes, int
+ * status codes (BCM_OK == 0). Device pointers are HIP device allocations
+ * holding column-major n x nrhs fp64 vectors.
+ *
+ * Each declaration names the reference interface it replaces
+ * (paths relative to the Bslib / Bcmlib sources).
+ */
+#ifndef BCMHIP_H
+#define BCMHIP_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes (the reference throws C strings instead) ---- */
+#define BCM_OK 0
+#define BCM_E_ARG 1
+#define BCM_E_NOT_SPD 2       /* ich2rs.cpp:118-123, :575 negative eigenvalue / pivot */
+#define BCM_E_WORKSPACE 3     /* ich2rs.cpp:1246-1250, ilu2rs.cpp:1655-1663 */
+#define BCM_E_BREAKDOWN 4     /* cg.cpp:270-278, lanczos.cpp:797-799 */
+#define BCM_E_HIP 5
+#define BCM_E_NOCONV 6
+#define BCM_E_NODEVICE 7
+#define BCM_E_INTERNAL 8
+
+/* ---- solver kinds and precisions ---- */
+#define BCM_PCG 0        /* CSVector::Cg serial form, cg.cpp:31-349 */
+#define BCM_BICGSTAB 1   /* no reference counterpart (nonsymmetric drop-in) */
+#define BCM_FGMRES 2     /* flexible GMRES(m); reference block GMRES is gmres.cpp:337 */
+#define BCM_LANCZOS 3    /* CSVector::Lanczos, lanczos.cpp:34-371 */
+
+#define BCM_F64 0
+#define BCM_F32 1
+#define BCM_BF16 2
+
+typedef struct bcm_matrix_s* bcm_matrix;   /* CSMatrixRS, smatrix.h:1040-1048 */
+typedef struct bcm_factor_s* bcm_factor;   /* IC2 U / ILU2 L,U factors (CSMatrixRS) */
+typedef struct bcm_context_s* bcm_context; /* one HIP stream + solver workspace */
+
+/* Solver parameters: the solve-path subset of CSlvParam (slvparam.h:19-113,
+ * defaults slvparam.h:121-214). Fields after `eps` are outputs. */
+typedef struct bcm_params {
+  int msglev;
+  int ordtype;    /* 1: RCM profile ordering (OrderPrfMtr), 0: none */
+  double memory;  /* factor workspace in units of nza(A); <= 0: unbounded */
+  double tau1, tau2, theta, sclmin, pivmin;
+  int ittype;     /* Cg stopping rule: 0 rel. to ||r0||, 1 absolute, 2 max-norm, 3 rel. to ||b|| */
+  int sttype;
+  int niter;
+  int ichk;
+  double eps;
+  int nitperf;    /* out */
+  double resfin;  /* out: final true relative residual */
+  double timefct; /* out: host factorization seconds */
+  double timeitr; /* out: solve seconds */
+  double density, density2; /* out: factor fill (first / second order) per nza(A) */
+} bcm_params;
+
+typedef struct bcm_solve_opts {
+  int kind;          /* BCM_PCG, BCM_BICGSTAB, BCM_FGMRES, BCM_LANCZOS */
+  int niter;
+  double eps;
+  int ittype;        /* as bcm_params.ittype; 3 = ||b - A x|| <= eps ||b|| */
+  int check_every;   /* device state polled every k iterations */
+  int restart;       /* FGMRES cycle length */
+  int precision;     /* preconditioner storage/application: BCM_F64, BCM_F32, BCM_BF16 */
+  int max_restarts;  /* true-residual restarts */
+} bcm_solve_opts;
+
+typedef struct bcm_stats {
+  int iterations, restarts, breakdown, converged;
+  double relres_true, relres_rec, seconds;
+} bcm_stats;
+
+/* ---- library ---- */
+const char* bcm_version(void);
+const char* bcm_last_error(void);            /* message of the last failed call on this thread */
+void bcm_params_default(bcm_params* p);      /* CSlvParam::CSlvParam, slvparam.h:121-214 */
+void bcm_solve_opts_default(bcm_solve_opts* o);
+
+/* ---- host block matrices (CSMatrixRS) ---- */
+/* Copies nsup block rows: sprnd[nsup+1] block-size prefix sums, ia[nsup+1],
+ * ja[nzja] (ascending per row, diagonal first when symmetric), a: blocks in ja
+ * order, each column-major bla_i x bla_j (mvm.cpp:2560). symmetric != 0 means
+ * upper-triangular storage as AbstractSymmSolver assembles it (solvers.cpp:5383). */
+int bcm_matrix_create(int nsup, const int* sprnd, const int* ia, const int* ja, const double* a, int symmetric,
+                      bcm_matrix* out);
+int bcm_matrix_destroy(bcm_matrix m);
+int bcm_matrix_info(bcm_matrix m, int* nsup, int* n, int* nzja, long long* nza, int* symmetric);
+int bcm_matrix_export(bcm_matrix m, int* sprnd, int* ia, int* ja, double* a);
+
+/* ---- seeded synthetic systems (stand-ins for the Bcmlib producers) ---- */
+int bcm_gen_heat2d(int nx, int ny, int bla, unsigned long long seed, bcm_matrix* upper, double* rhs);
+int bcm_gen_seepage3d(int nx, int ny, int nz, int bla, double skew, unsigned long long seed, bcm_matrix* a,
+                      double* rhs);
+int bcm_gen_elastic_cell(int m, double radius, double contrast, double poisson, double shift,
+                         unsigned long long seed, bcm_matrix* upper, double* rhs6);
+
+/* ---- ordering (ordmatr.cpp:28-227 OrderPrfMtr(1) -> rcm.cpp genrcm) ---- */
+int bcm_rcm_order(bcm_matrix m, int* order);                       /* order[old] = new */
+/* OrdMtrSymm (smatrixrs.cpp:456) for symmetric, OrdMtr (smatrixrs.cpp:345) otherwise. */
+int bcm_matrix_reorder(bcm_matrix m, const int* order, bcm_matrix* out);
+/* CSVector::OrdVect (svector.cpp:418): type 'D' applies order, 'I' undoes it. */
+int bcm_vector_reorder(bcm_matrix original, const int* order, char type, double* v, int nrhs);
+
+/* ---- host factorizations ---- */
+/* CSMatrixRS::Ich2 (ich2rs.cpp:1075) of an upper-stored SPD matrix. */
+int bcm_factor_ic2(bcm_matrix upper, bcm_params* p, bcm_factor* out);
+/* CombineLU (smatrixrs.cpp:1645) + CSMatrixRS::Ilu2 (ilu2rs.cpp:1458). */
+int bcm_factor_ilu2(bcm_matrix a, bcm_params* p, bcm_factor* out);
+int bcm_factor_destroy(bcm_factor f);
+/* which: 0 = L (by columns; for IC2 the U factor), 1 = U (by rows). */
+int bcm_factor_info(bcm_factor f, int which, int* nzja, long long* nza, int* levels_fwd, int* levels_bwd);
+int bcm_factor_export(bcm_factor f, int which, int* ia, int* ja, double* a);
+
+/* ---- device context ---- */
+int bcm_device_count(int* count);
+int bcm_context_create(bcm_context* out);
+int bcm_context_destroy(bcm_context c);
+int bcm_context_sync(bcm_context c);
+
+/* ---- device operators: the CMVM / CSOLVEL / CSOLVEU / CSOLVELU plug-ins (svector.h:34-41) ---- */
+/* y = A x: CSMatrixRS::MvmA (mvm.cpp:2530); symmetric matrices: MvmASymm (mvm.cpp:3008). */
+int bcm_mvm(bcm_context c, bcm_matrix a, const double* x, double* y, int nrhs);
+/* y = A^T x: CSMatrixRS::MvmAt (mvm.cpp:2593). */
+int bcm_mvm_t(bcm_context c, bcm_matrix a, const double* x, double* y, int nrhs);
+/* r = b - A x (operator- after MvmA, svector.cpp:285). */
+int bcm_residual(bcm_context c, bcm_matrix a, const double* b, const double* x, double* r, int nrhs);
+/* Forward sweep of CSMatrixRS::SolveL (mvm.cpp:3140) on factor part `which`. */
+int bcm_solve_l(bcm_context c, bcm_factor f, int which, const double* x, double* y, int nrhs);
+/* Backward sweep of CSMatrixRS::SolveU (mvm.cpp:3410) on factor part `which`. */
+int bcm_solve_u(bcm_context c, bcm_factor f, int which, const double* x, double* y, int nrhs);
+/* z = M^-1 r (CParSolveLU::SolveLU, gsmatrix.h:1008-1071) in the given precision; transpose: M^-T. */
+int bcm_precond(bcm_context c, bcm_factor f, const double* r, double* z, int nrhs, int precision, int transpose);
+/* ScProdSVect (svector.cpp:364): result written to host memory. */
+int bcm_dot(bcm_context c, int n, const double* x, const double* y, double* result);
+
+/* ---- GPU solve: x (device) holds the initial guess on entry ---- */
+int bcm_solve(bcm_context c, bcm_matrix a, bcm_factor f, const bcm_solve_opts* o, const double* b, double* x,
+              bcm_stats* st);
+
+/* ---- drop-in entry points (callback ABI of smatrix.h:15-21; C++ references
+ *      are passed as pointers, which is the same machine-level convention) ---- */
+typedef void (*bcm_cb_nsup)(void* obj, int* nsup);
+typedef void (*bcm_cb_sprnds)(void* obj, int* sprnds);
+typedef void (*bcm_cb_iaja)(void* obj, int** ia, int** ja);  /* callee allocates with new[], library frees with delete[] */
+typedef void (*bcm_cb_blockrow)(void* obj, int isup, double* arow);
+typedef void (*bcm_cb_rhs)(void* obj, int isup, double* rhs);
+typedef void (*bcm_cb_iniguess)(void* obj, int isup, double* xini);
+typedef void (*bcm_cb_storesol)(void* obj, int isup, double* sol);
+
+/* AbstractSymmSolver (solvers.cpp:5383): assemble upper storage, RCM, IC2,
+ * GPU PCG, return the solution through storesol. */
+int bcm_abstract_symm_solver(void* pgener, bcm_params* p, bcm_cb_nsup fnsup, bcm_cb_sprnds fsprnds,
+                             bcm_cb_iaja fiaja, bcm_cb_blockrow fblockrow, bcm_cb_rhs frhs,
+                             bcm_cb_iniguess finiguess, bcm_cb_storesol fstoresol);
+/* AbstractSolver (solvers.cpp:5591): assemble, RCM, ILU2, GPU Krylov
+ * (ittype 1: two-sided Lanczos as the reference; otherwise BiCGStab). */
+int bcm_abstract_solver(void* pgener, bcm_params* p, bcm_cb_nsup fnsup, bcm_cb_sprnds fsprnds, bcm_cb_iaja fiaja,
+                        bcm_cb_blockrow fblockrow, bcm_cb_rhs frhs, bcm_cb_iniguess finiguess,
+                        bcm_cb_storesol fstoresol);
+
+/* Callback memory for bcm_cb_iaja (operator new[] of nint ints, as new_struct in Bcmlib/utils.h:29-35). */
+int* bcm_alloc_ints(int nint);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* BCMHIP_H */
