@@ -422,6 +422,42 @@ int bcm_device_count(int* count) {
   });
 }
 
+int bcm_dev_alloc(unsigned long long bytes, void** out) {
+  return guarded([&] {
+    need(out != nullptr, "dev_alloc: out is null");
+    need_device();
+    *out = nullptr;
+    BCM_HIP(hipMalloc(out, bytes ? bytes : 1));
+  });
+}
+
+int bcm_dev_free(void* p) {
+  return guarded([&] {
+    if (p) BCM_HIP(hipFree(p));
+  });
+}
+
+int bcm_copy_to_device(void* dst, const void* src, unsigned long long bytes) {
+  return guarded([&] {
+    need(dst && src, "copy_to_device: null pointer");
+    if (bytes) BCM_HIP(hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice));
+  });
+}
+
+int bcm_copy_to_host(void* dst, const void* src, unsigned long long bytes) {
+  return guarded([&] {
+    need(dst && src, "copy_to_host: null pointer");
+    if (bytes) BCM_HIP(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost));
+  });
+}
+
+int bcm_dev_zero(void* p, unsigned long long bytes) {
+  return guarded([&] {
+    need(p != nullptr, "dev_zero: null pointer");
+    if (bytes) BCM_HIP(hipMemset(p, 0, bytes));
+  });
+}
+
 int bcm_context_create(bcm_context* out) {
   return guarded([&] {
     need(out != nullptr, "context_create: out is null");

@@ -9,7 +9,8 @@
 namespace bcm {
 namespace {
 
-constexpr int kSpinLimit = 1 << 25;
+// Upper bound on the polls of one dependency wait (~0.1 s with s_sleep 1).
+constexpr int kSpinLimit = 1 << 22;
 
 // ---- value loaders ----
 __device__ __forceinline__ double ldv(const double* p) { return *p; }

@@ -148,7 +148,11 @@ __global__ __launch_bounds__(64) void k_sweep(int nsup, int n, int forward, cons
         int spins = 0;
         while (__hip_atomic_load(&flags[k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != epoch) {
           __builtin_amdgcn_s_sleep(1);
-          if (++spins > kSpinLimit) {
+          ++spins;
+          // A wave that gave up raises ctl[2]; the others then stop waiting too,
+          // so a broken schedule costs one spin limit, not one per row.
+          if (spins > kSpinLimit ||
+              ((spins & 1023) == 0 && __hip_atomic_load(&ctl[2], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))) {
             __hip_atomic_store(&ctl[2], 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             break;
           }

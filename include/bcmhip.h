@@ -115,8 +115,15 @@ int bcm_factor_destroy(bcm_factor f);
 int bcm_factor_info(bcm_factor f, int which, int* nzja, long long* nza, int* levels_fwd, int* levels_bwd);
 int bcm_factor_export(bcm_factor f, int which, int* ia, int* ja, double* a);
 
-/* ---- device context ---- */
+/* ---- device memory (HIP allocations owned by the caller) ---- */
 int bcm_device_count(int* count);
+int bcm_dev_alloc(unsigned long long bytes, void** out);
+int bcm_dev_free(void* p);
+int bcm_copy_to_device(void* dst, const void* src, unsigned long long bytes);
+int bcm_copy_to_host(void* dst, const void* src, unsigned long long bytes);
+int bcm_dev_zero(void* p, unsigned long long bytes);
+
+/* ---- device context ---- */
 int bcm_context_create(bcm_context* out);
 int bcm_context_destroy(bcm_context c);
 int bcm_context_sync(bcm_context c);

@@ -1,0 +1,265 @@
+"""ORACLE - test infrastructure only.
+
+ctypes access to the CPU oracle (liboracle.so, a plain-C restatement of the
+reference's solve phase) and to the reference's own compiled solve-phase code
+(_ref/libbslib_ref.so, built from the Bslib sources by Makefile.ref). Only the
+tests, ``__graft_entry__.smoke()`` and bench.py's ``cpu_baseline`` leg use this
+module, as the checker and as the timed CPU baseline.
+
+Block matrices are passed as ``(sprnd, ia, ja, a)`` numpy arrays with blocks
+contiguous in ja order (column-major), as the C ABI of libbcmhip exports them.
+"""
+
+from __future__ import annotations
+
+import ctypes as C
+import os
+from typing import Optional, Tuple
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ORACLE_SO = os.path.join(HERE, "liboracle.so")
+REF_SO = os.path.join(HERE, "_ref", "libbslib_ref.so")
+
+_IP = C.POINTER(C.c_int)
+_DP = C.POINTER(C.c_double)
+_LP = C.POINTER(C.c_longlong)
+
+
+class Bsr(C.Structure):
+    _fields_ = [("nsup", C.c_int), ("sprnd", _IP), ("ia", _IP), ("ja", _IP), ("bsa", _LP), ("a", _DP)]
+
+
+class Result(C.Structure):
+    _fields_ = [("nitperf", C.c_int), ("updates", C.c_int), ("resfin", C.c_double), ("breakdown", C.c_int)]
+
+
+_orc: Optional[C.CDLL] = None
+_ref: Optional[C.CDLL] = None
+
+
+def build() -> None:
+    """Compiles the oracle (and the reference library when its sources exist)."""
+    import subprocess
+
+    subprocess.run(["make", "-s", "-C", HERE], check=True)
+
+
+def orc() -> C.CDLL:
+    global _orc
+    if _orc is None:
+        if not os.path.exists(ORACLE_SO):
+            build()
+        l = C.CDLL(ORACLE_SO)
+        for name in ("orc_mvma", "orc_mvmat", "orc_solvel", "orc_solveu"):
+            getattr(l, name).argtypes = [C.POINTER(Bsr), _DP, _DP, C.c_int]
+        l.orc_mvmasymm.argtypes = [C.POINTER(Bsr), _DP, _DP]
+        l.orc_scprod.argtypes = [C.c_int, _DP, _DP]
+        l.orc_scprod.restype = C.c_double
+        l.orc_cg.argtypes = [C.POINTER(Bsr), C.POINTER(Bsr), _DP, _DP, C.c_int, C.c_double, C.c_int, _DP]
+        l.orc_cg.restype = Result
+        l.orc_lanczos.argtypes = [C.POINTER(Bsr)] * 3 + [_DP, _DP, C.c_int, C.c_double, C.c_int, _DP]
+        l.orc_lanczos.restype = Result
+        l.orc_bicgstab.argtypes = [C.POINTER(Bsr)] * 3 + [_DP, _DP, C.c_int, C.c_double, _DP]
+        l.orc_bicgstab.restype = Result
+        l.orc_fgmres.argtypes = [C.POINTER(Bsr)] * 3 + [_DP, _DP, C.c_int, C.c_int, C.c_double, _DP]
+        l.orc_fgmres.restype = Result
+        _orc = l
+    return _orc
+
+
+def ref_available() -> bool:
+    return os.path.exists(REF_SO)
+
+
+def ref() -> C.CDLL:
+    """The reference's compiled solve-phase code (lazy binding: MPI/LAPACK paths are never called)."""
+    global _ref
+    if _ref is None:
+        l = C.CDLL(REF_SO, mode=os.RTLD_LAZY)
+        mat = [C.c_int, _IP, _IP, _IP, _DP]
+        for name in ("ref_mvma", "ref_mvmat", "ref_solvel", "ref_solveu"):
+            getattr(l, name).argtypes = mat + [_DP, _DP, C.c_int]
+        l.ref_mvmasymm.argtypes = mat + [_DP, _DP]
+        l.ref_scprod.argtypes = [C.c_int, _DP, _DP]
+        l.ref_scprod.restype = C.c_double
+        l.ref_cg.argtypes = mat + [_IP, _IP, _DP, _DP, _DP, C.c_int, C.c_double, C.c_int, _DP, _IP, _DP]
+        l.ref_lanczos.argtypes = mat + [_IP, _IP, _DP, _IP, _IP, _DP, _DP, _DP, C.c_int, C.c_double, C.c_int,
+                                        _DP, _IP, _DP]
+        l.ref_genrcm.argtypes = [C.c_int, _IP, _IP, _IP]
+        l.ref_order_prf.argtypes = mat + [_IP]
+        l.ref_ordmtr.argtypes = [C.c_int] + mat + [_IP, _IP, _IP, _IP, _IP, _DP]
+        l.ref_combinelu.argtypes = [C.c_int] + mat + [_IP, _IP, _IP, _IP, _DP]
+        _ref = l
+    return _ref
+
+
+def _i(a):
+    return np.ascontiguousarray(a, dtype=np.int32)
+
+
+def _d(a):
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+class Block:
+    """A block-CSR matrix as numpy arrays plus its C view for the oracle."""
+
+    def __init__(self, sprnd, ia, ja, a):
+        self.sprnd, self.ia, self.ja, self.a = _i(sprnd), _i(ia), _i(ja), _d(a)
+        self.nsup = len(self.sprnd) - 1
+        self.n = int(self.sprnd[-1])
+        bla = np.diff(self.sprnd)
+        rows = np.repeat(np.arange(self.nsup), np.diff(self.ia))
+        sizes = bla[rows].astype(np.int64) * bla[self.ja].astype(np.int64)
+        self.bsa = np.concatenate([[0], np.cumsum(sizes)[:-1]]).astype(np.int64) if len(sizes) else np.zeros(0, np.int64)
+        self.c = Bsr(self.nsup, self.sprnd.ctypes.data_as(_IP), self.ia.ctypes.data_as(_IP),
+                     self.ja.ctypes.data_as(_IP), self.bsa.ctypes.data_as(_LP), self.a.ctypes.data_as(_DP))
+
+    def args(self):
+        return (self.nsup, self.sprnd.ctypes.data_as(_IP), self.ia.ctypes.data_as(_IP),
+                self.ja.ctypes.data_as(_IP), self.a.ctypes.data_as(_DP))
+
+
+def _dp(x):
+    return x.ctypes.data_as(_DP)
+
+
+def _vec(x, n, nrhs):
+    return np.asfortranarray(np.asarray(x, dtype=np.float64).reshape((n, nrhs), order="F"))
+
+
+# ---- oracle (restatement) ----
+
+def mvma(m: Block, x, nrhs=1):
+    x = _vec(x, m.n, nrhs); y = np.zeros_like(x)
+    orc().orc_mvma(C.byref(m.c), _dp(x), _dp(y), nrhs)
+    return y.reshape(-1, order="F") if nrhs == 1 else y
+
+
+def mvmat(m: Block, x, nrhs=1):
+    x = _vec(x, m.n, nrhs); y = np.zeros_like(x)
+    orc().orc_mvmat(C.byref(m.c), _dp(x), _dp(y), nrhs)
+    return y.reshape(-1, order="F") if nrhs == 1 else y
+
+
+def mvmasymm(m: Block, x):
+    x = _d(x); y = np.zeros_like(x)
+    orc().orc_mvmasymm(C.byref(m.c), _dp(x), _dp(y))
+    return y
+
+
+def solvel(f: Block, x, nrhs=1):
+    x = _vec(x, f.n, nrhs); y = np.zeros_like(x)
+    orc().orc_solvel(C.byref(f.c), _dp(x), _dp(y), nrhs)
+    return y.reshape(-1, order="F") if nrhs == 1 else y
+
+
+def solveu(f: Block, x, nrhs=1):
+    x = _vec(x, f.n, nrhs); y = np.zeros_like(x)
+    orc().orc_solveu(C.byref(f.c), _dp(x), _dp(y), nrhs)
+    return y.reshape(-1, order="F") if nrhs == 1 else y
+
+
+def cg(a: Block, u: Block, b, x0=None, niter=1000, eps=1e-10, ittype=3):
+    b = _d(b); x0 = np.zeros_like(b) if x0 is None else _d(x0); x = np.zeros_like(b)
+    r = orc().orc_cg(C.byref(a.c), C.byref(u.c), _dp(x0), _dp(b), niter, eps, ittype, _dp(x))
+    return x, r
+
+
+def lanczos(a: Block, l: Block, u: Block, b, x0=None, niter=1000, eps=1e-10, sttype=1):
+    b = _d(b); x0 = np.zeros_like(b) if x0 is None else _d(x0); x = np.zeros_like(b)
+    r = orc().orc_lanczos(C.byref(a.c), C.byref(l.c), C.byref(u.c), _dp(x0), _dp(b), niter, eps, sttype, _dp(x))
+    return x, r
+
+
+def bicgstab(a: Block, l: Block, u: Block, b, x0=None, niter=1000, eps=1e-10):
+    b = _d(b); x0 = np.zeros_like(b) if x0 is None else _d(x0); x = np.zeros_like(b)
+    r = orc().orc_bicgstab(C.byref(a.c), C.byref(l.c), C.byref(u.c), _dp(x0), _dp(b), niter, eps, _dp(x))
+    return x, r
+
+
+def fgmres(a: Block, l: Block, u: Block, b, x0=None, m=30, niter=1000, eps=1e-10):
+    b = _d(b); x0 = np.zeros_like(b) if x0 is None else _d(x0); x = np.zeros_like(b)
+    r = orc().orc_fgmres(C.byref(a.c), C.byref(l.c), C.byref(u.c), _dp(x0), _dp(b), m, niter, eps, _dp(x))
+    return x, r
+
+
+# ---- the reference's compiled code ----
+
+def ref_mvma(m: Block, x, nrhs=1):
+    x = _vec(x, m.n, nrhs); y = np.zeros_like(x)
+    ref().ref_mvma(*m.args(), _dp(x), _dp(y), nrhs)
+    return y.reshape(-1, order="F") if nrhs == 1 else y
+
+
+def ref_mvmat(m: Block, x, nrhs=1):
+    x = _vec(x, m.n, nrhs); y = np.zeros_like(x)
+    ref().ref_mvmat(*m.args(), _dp(x), _dp(y), nrhs)
+    return y.reshape(-1, order="F") if nrhs == 1 else y
+
+
+def ref_mvmasymm(m: Block, x):
+    x = _d(x); y = np.zeros_like(x)
+    ref().ref_mvmasymm(*m.args(), _dp(x), _dp(y))
+    return y
+
+
+def ref_solvel(f: Block, x, nrhs=1):
+    x = _vec(x, f.n, nrhs); y = np.zeros_like(x)
+    ref().ref_solvel(*f.args(), _dp(x), _dp(y), nrhs)
+    return y.reshape(-1, order="F") if nrhs == 1 else y
+
+
+def ref_solveu(f: Block, x, nrhs=1):
+    x = _vec(x, f.n, nrhs); y = np.zeros_like(x)
+    ref().ref_solveu(*f.args(), _dp(x), _dp(y), nrhs)
+    return y.reshape(-1, order="F") if nrhs == 1 else y
+
+
+def ref_cg(a: Block, u: Block, b, x0=None, niter=1000, eps=1e-10, ittype=0) -> Tuple[np.ndarray, int, float]:
+    b = _d(b); x0 = np.zeros_like(b) if x0 is None else _d(x0); x = np.zeros_like(b)
+    nit = C.c_int(); res = C.c_double()
+    ref().ref_cg(*a.args(), u.ia.ctypes.data_as(_IP), u.ja.ctypes.data_as(_IP), u.a.ctypes.data_as(_DP),
+                 _dp(x0), _dp(b), niter, eps, ittype, _dp(x), C.byref(nit), C.byref(res))
+    return x, nit.value, res.value
+
+
+def ref_lanczos(a: Block, l: Block, u: Block, b, x0=None, niter=1000, eps=1e-10, sttype=1):
+    b = _d(b); x0 = np.zeros_like(b) if x0 is None else _d(x0); x = np.zeros_like(b)
+    nit = C.c_int(); res = C.c_double()
+    ref().ref_lanczos(*a.args(), l.ia.ctypes.data_as(_IP), l.ja.ctypes.data_as(_IP), l.a.ctypes.data_as(_DP),
+                      u.ia.ctypes.data_as(_IP), u.ja.ctypes.data_as(_IP), u.a.ctypes.data_as(_DP),
+                      _dp(x0), _dp(b), niter, eps, sttype, _dp(x), C.byref(nit), C.byref(res))
+    return x, nit.value, res.value
+
+
+def ref_genrcm(n, xadj0, adj0):
+    """genrcm on a 0-based adjacency; returns the 0-based permutation perm[new] = old."""
+    xadj = _i(np.asarray(xadj0) + 1); adj = _i(np.asarray(adj0) + 1); perm = np.zeros(n, np.int32)
+    ref().ref_genrcm(n, xadj.ctypes.data_as(_IP), adj.ctypes.data_as(_IP), perm.ctypes.data_as(_IP))
+    return perm - 1
+
+
+def ref_order_prf(m: Block):
+    order = np.zeros(m.nsup, np.int32)
+    ref().ref_order_prf(*m.args(), order.ctypes.data_as(_IP))
+    return order
+
+
+def _take(fn, *args, nsup):
+    nzja = C.c_int(); nza = C.c_int()
+    fn(*args, C.byref(nzja), C.byref(nza), None, None, None)
+    ia = np.zeros(nsup + 1, np.int32); ja = np.zeros(nzja.value, np.int32); a = np.zeros(nza.value)
+    fn(*args, C.byref(nzja), C.byref(nza), ia.ctypes.data_as(_IP), ja.ctypes.data_as(_IP), a.ctypes.data_as(_DP))
+    return ia, ja, a
+
+
+def ref_ordmtr(m: Block, order, symmetric: bool):
+    order = _i(order)
+    return _take(ref().ref_ordmtr, int(symmetric), *m.args(), order.ctypes.data_as(_IP), nsup=m.nsup)
+
+
+def ref_combinelu(m: Block, which: int):
+    return _take(ref().ref_combinelu, which, *m.args(), nsup=m.nsup)
