@@ -56,6 +56,14 @@ class SolveOpts(C.Structure):
     ]
 
 
+class Profile(C.Structure):
+    _fields_ = [
+        ("spmv_ms", C.c_double), ("sweep_ms", C.c_double),
+        ("spmv_launches", C.c_longlong), ("sweep_launches", C.c_longlong),
+        ("spmv_bytes", C.c_double), ("sweep_bytes", C.c_double),
+    ]
+
+
 class Stats(C.Structure):
     _fields_ = [
         ("iterations", C.c_int), ("restarts", C.c_int), ("breakdown", C.c_int), ("converged", C.c_int),
@@ -103,6 +111,8 @@ SIGNATURES = {
     "bcm_context_create": (C.c_int, [C.POINTER(_P)]),
     "bcm_context_destroy": (C.c_int, [_P]),
     "bcm_context_sync": (C.c_int, [_P]),
+    "bcm_context_profile": (C.c_int, [_P, C.c_int]),
+    "bcm_context_profile_read": (C.c_int, [_P, C.POINTER(Profile), C.c_int]),
     "bcm_mvm": (C.c_int, [_P, _P, C.c_void_p, C.c_void_p, C.c_int]),
     "bcm_mvm_t": (C.c_int, [_P, _P, C.c_void_p, C.c_void_p, C.c_int]),
     "bcm_residual": (C.c_int, [_P, _P, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int]),
@@ -366,6 +376,14 @@ class Context:
 
     def sync(self):
         _check(lib().bcm_context_sync(self.h))
+
+    def profile(self, enable: bool = True):
+        _check(lib().bcm_context_profile(self.h, int(enable)))
+
+    def profile_read(self, reset: bool = False) -> Profile:
+        p = Profile()
+        _check(lib().bcm_context_profile_read(self.h, C.byref(p), int(reset)))
+        return p
 
     def mvm(self, a: Matrix, x, y, nrhs: int = 1):
         _check(lib().bcm_mvm(self.h, a.h, ptr(x), ptr(y), nrhs))

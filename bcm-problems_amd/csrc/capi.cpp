@@ -454,7 +454,11 @@ int bcm_copy_to_host(void* dst, const void* src, unsigned long long bytes) {
 int bcm_dev_zero(void* p, unsigned long long bytes) {
   return guarded([&] {
     need(p != nullptr, "dev_zero: null pointer");
-    if (bytes) BCM_HIP(hipMemset(p, 0, bytes));
+    if (bytes) {
+      // the null stream does not order with the library's non-blocking streams: complete here
+      BCM_HIP(hipMemset(p, 0, bytes));
+      BCM_HIP(hipDeviceSynchronize());
+    }
   });
 }
 
@@ -474,7 +478,31 @@ int bcm_context_sync(bcm_context c) {
   return guarded([&] {
     need(c != nullptr, "context_sync: null handle");
     BCM_HIP(hipStreamSynchronize(c->c.stream));
-    c->c.check_sweep_error();
+  });
+}
+
+int bcm_context_profile(bcm_context c, int enable) {
+  return guarded([&] {
+    need(c != nullptr, "context_profile: null handle");
+    if (!enable) c->c.prof_flush();
+    c->c.prof = enable != 0;
+  });
+}
+
+int bcm_context_profile_read(bcm_context c, bcm_profile* out, int reset) {
+  return guarded([&] {
+    need(c && out, "context_profile_read: null argument");
+    c->c.prof_flush();
+    const auto& sp = c->c.prof_acc[bcm::kProfSpmv];
+    const auto& sw = c->c.prof_acc[bcm::kProfSweep];
+    out->spmv_ms = sp.ms;
+    out->spmv_launches = sp.launches;
+    out->spmv_bytes = sp.bytes;
+    out->sweep_ms = sw.ms;
+    out->sweep_launches = sw.launches;
+    out->sweep_bytes = sw.bytes;
+    if (reset)
+      for (auto& a : c->c.prof_acc) a = bcm::ProfAcc();
   });
 }
 
@@ -482,6 +510,7 @@ int bcm_mvm(bcm_context c, bcm_matrix a, const double* x, double* y, int nrhs) {
   return guarded([&] {
     need(c && a && x && y && nrhs > 0, "mvm: bad argument");
     a->m.to_device(false);
+    bcm::ProfScope ps(c->c, bcm::kProfSpmv, bcm::spmv_bytes(a->m.d, nrhs));
     bcm::launch_spmv(a->m.d, x, y, nullptr, nrhs, nullptr, nullptr, c->c.rw, c->c.stream);
   });
 }
@@ -508,7 +537,7 @@ int bcm_solve_l(bcm_context c, bcm_factor f, int which, const double* x, double*
     need(x != y, "solve_l: x and y must not alias");
     const bool t = which == 1 && f->f.kind == 1;
     f->f.to_device(bcm::kF64, t);
-    bcm::launch_sweep(t ? f->f.fwdt : f->f.fwd, x, y, bcm::kF64, nrhs, c->c.sync, c->c.stream);
+    bcm::launch_sweep(t ? f->f.fwdt : f->f.fwd, x, y, bcm::kF64, nrhs, c->c.stream);
   });
 }
 
@@ -518,7 +547,7 @@ int bcm_solve_u(bcm_context c, bcm_factor f, int which, const double* x, double*
     need(x != y, "solve_u: x and y must not alias");
     const bool t = which == 0 && f->f.kind == 1;
     f->f.to_device(bcm::kF64, t);
-    bcm::launch_sweep(t ? f->f.bwdt : f->f.bwd, x, y, bcm::kF64, nrhs, c->c.sync, c->c.stream);
+    bcm::launch_sweep(t ? f->f.bwdt : f->f.bwd, x, y, bcm::kF64, nrhs, c->c.stream);
   });
 }
 

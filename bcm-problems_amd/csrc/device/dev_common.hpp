@@ -9,9 +9,6 @@
 namespace bcm {
 namespace {
 
-// Upper bound on the polls of one dependency wait (~0.1 s with s_sleep 1).
-constexpr int kSpinLimit = 1 << 22;
-
 // ---- value loaders ----
 __device__ __forceinline__ double ldv(const double* p) { return *p; }
 __device__ __forceinline__ float ldv(const float* p) { return *p; }

@@ -69,6 +69,8 @@ struct DevTri {
   bool forward = true;
   int prec = kF64;
   int levels = 0;
+  std::vector<int> level_ptr;  // host: rows of level l are level_rows[level_ptr[l] .. level_ptr[l+1])
+  DevBuf level_rows;
   DevBuf ia, ja;      // dependency lists per row (row index space of the sweep)
   DevBuf off;         // ndep blocks, row after row
   DevBuf diag;        // nsup inverted diagonal blocks
@@ -82,14 +84,6 @@ void dev_tri_forward(const BlockCsr& f, int prec, DevTri& t);
 // Backward sweep through a factor stored by rows (reference SolveU):
 // z_i = D_i (x_i - sum_{j>i} F_ij z_j).
 void dev_tri_backward(const BlockCsr& f, int prec, DevTri& t);
-
-// Synchronisation state shared by the sweeps on one stream.
-struct SweepSync {
-  DevBuf flags;   // per-row epoch of the last completed sweep
-  DevBuf ctl;     // [0] epoch, [1] ticket, [2] error (spin limit hit)
-  int nsup = 0;
-  void ensure(int nsup);
-};
 
 // Reduction workspace: partial sums and the completion counter of the
 // single-pass (last block finalises) reductions.
@@ -111,9 +105,10 @@ constexpr int kMaxDots = 8;
 void launch_spmv(const DevBsr& A, const double* x, double* y, const double* bsub, int nrhs,
                  double* dot_out, const double* wdot, RedWork& rw, hipStream_t s, const int* skip = nullptr);
 
-// Triangular sweep y = T^-1 x for the given precision of vectors (TX).
-void launch_sweep(const DevTri& T, const void* x, void* y, int vec_prec, int nrhs, SweepSync& sync,
-                  hipStream_t s, const int* skip = nullptr);
+// Triangular sweep y = T^-1 x for the given precision of vectors (TX): one
+// launch per dependency level. x and y must not alias.
+void launch_sweep(const DevTri& T, const void* x, void* y, int vec_prec, int nrhs, hipStream_t s,
+                  const int* skip = nullptr);
 
 // out[k] = sum_i x_k[i] * y_k[i] for K pairs (K <= kMaxDots), one pass.
 void launch_multidot(int n, int k, const double* const* xs, const double* const* ys, double* out,

@@ -1,5 +1,6 @@
-// gfx950 kernels of the solve phase: block-CSR SpMV, sync-free triangular
-// sweeps through inverse-diagonal-block factors, and one-pass reductions.
+// gfx950 kernels of the solve phase: block-CSR SpMV, level-scheduled
+// triangular sweeps through inverse-diagonal-block factors, and one-pass
+// reductions.
 //
 // Reference operators: CSMatrixRS::MvmA (mvm.cpp:2530-2569), MvmAt
 // (mvm.cpp:2593-2632), MvmASymm (mvm.cpp:3008-3037; run here on the
@@ -12,12 +13,10 @@
 // G*B consecutive doubles of a column-major block (coalesced), and the G
 // partial sums of each row element are folded with cross-lane shuffles.
 //
-// Triangular sweeps are sync-free: waves take rows in dependency order from
-// an atomic ticket, spin (bounded) on the per-row completion epochs of the
-// rows they depend on, and publish their own row with a release store. Every
-// wave only ever waits on rows claimed earlier by running waves, so the
-// sweep cannot deadlock, and every wait is bounded by a spin limit that sets
-// an error flag instead of hanging.
+// Triangular sweeps are level-scheduled: the host groups the block rows by
+// dependency level once, and each level is one launch whose rows depend only
+// on rows finished by earlier launches of the same stream. No wave ever waits
+// on another wave.
 #include "device.hpp"
 #include "dev_common.hpp"
 
@@ -115,92 +114,66 @@ __global__ void k_spmv_var(int nsup, int n, const int* __restrict__ sprnd, const
 }
 
 // ---------------------------------------------------------------------------
-// Sync-free triangular sweep.
-__global__ void k_sweep_arm(int* ctl) {
-  ctl[0] += 1;
-  ctl[1] = 0;
-}
-
+// Level-scheduled triangular sweep: one launch per dependency level; every
+// row of a level depends only on rows of earlier levels, which earlier
+// launches on the same stream have completed, so no wave ever waits on
+// another. One wavefront per block row.
 template <int B, int NR, class TA, class TX>
-__global__ __launch_bounds__(64) void k_sweep(int nsup, int n, int forward, const int* __restrict__ ia,
-                                              const int* __restrict__ ja, const TA* __restrict__ off,
-                                              const TA* __restrict__ diag, const TX* __restrict__ x, TX* y,
-                                              int* flags, int* ctl, const int* skip) {
+__global__ __launch_bounds__(256) void k_sweep_level(int nrows, const int* __restrict__ rows, int n,
+                                                     const int* __restrict__ ia, const int* __restrict__ ja,
+                                                     const TA* __restrict__ off, const TA* __restrict__ diag,
+                                                     const TX* __restrict__ x, TX* __restrict__ y,
+                                                     const int* skip) {
   if (skip && *skip) return;
   using Acc = typename AccOf<TX>::type;
   constexpr int G = 64 / B;
-  __shared__ Acc v[NR][B];
-  const int lane = threadIdx.x;
+  __shared__ Acc vs[4][NR][B];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int idx = blockIdx.x * 4 + wave;
   const int r = lane % B, g = lane / B;
-  const int epoch = ctl[0];
-  for (;;) {
-    int t = 0;
-    if (lane == 0) t = __hip_atomic_fetch_add(&ctl[1], 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    t = __shfl(t, 0);
-    if (t >= nsup) break;
-    const int row = forward ? t : nsup - 1 - t;
+  const int row = idx < nrows ? rows[idx] : -1;
+  Acc acc[NR];
+#pragma unroll
+  for (int q = 0; q < NR; ++q) acc[q] = Acc(0);
+  if (row >= 0 && g < G) {
     const int jb = ia[row], je = ia[row + 1];
-    // Wait until every row this one depends on has been published.
-    for (int base = jb; base < je; base += 64) {
-      const int j = base + lane;
-      if (j < je) {
-        const int k = ja[j];
-        int spins = 0;
-        while (__hip_atomic_load(&flags[k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != epoch) {
-          __builtin_amdgcn_s_sleep(1);
-          ++spins;
-          // A wave that gave up raises ctl[2]; the others then stop waiting too,
-          // so a broken schedule costs one spin limit, not one per row.
-          if (spins > kSpinLimit ||
-              ((spins & 1023) == 0 && __hip_atomic_load(&ctl[2], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))) {
-            __hip_atomic_store(&ctl[2], 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            break;
-          }
-        }
-      }
+    const TA* blk0 = off + (size_t)jb * B * B;
+    const int ncol = (je - jb) * B;
+#pragma unroll 4
+    for (int tt = g; tt < ncol; tt += G) {
+      const int bk = tt / B, c = tt - bk * B;
+      const int k = ja[jb + bk];
+      const Acc av = (Acc)ldnt(blk0 + (size_t)bk * B * B + c * B + r);
+#pragma unroll
+      for (int q = 0; q < NR; ++q) acc[q] += av * (Acc)y[(size_t)q * n + (size_t)k * B + c];
     }
-    Acc acc[NR];
+  }
 #pragma unroll
-    for (int k = 0; k < NR; ++k) acc[k] = Acc(0);
-    if (g < G) {
-      const TA* blk0 = off + (size_t)jb * B * B;
-      const int ncol = (je - jb) * B;
-      for (int tt = g; tt < ncol; tt += G) {
-        const int bk = tt / B, c = tt - bk * B;
-        const int k = ja[jb + bk];
-        const Acc av = (Acc)ldnt(blk0 + (size_t)bk * B * B + c * B + r);
-        for (int q = 0; q < NR; ++q) acc[q] += av * (Acc)ld_coherent(&y[(size_t)q * n + (size_t)k * B + c]);
-      }
+  for (int q = 0; q < NR; ++q) {
+    Acc s = acc[q];
+#pragma unroll
+    for (int h = 1; h < G; ++h) s += __shfl(acc[q], lane + h * B);
+    if (row >= 0 && lane < B) vs[wave][q][lane] = (Acc)x[(size_t)q * n + (size_t)row * B + lane] - s;
+  }
+  __syncthreads();
+  Acc z[NR];
+#pragma unroll
+  for (int q = 0; q < NR; ++q) z[q] = Acc(0);
+  if (row >= 0 && g < G) {
+    const TA* d = diag + (size_t)row * B * B;
+#pragma unroll
+    for (int c = g; c < B; c += G) {
+      const Acc dv = (Acc)ldnt(d + c * B + r);
+#pragma unroll
+      for (int q = 0; q < NR; ++q) z[q] += dv * vs[wave][q][c];
     }
+  }
 #pragma unroll
-    for (int q = 0; q < NR; ++q) {
-      Acc s = acc[q];
+  for (int q = 0; q < NR; ++q) {
+    Acc s = z[q];
 #pragma unroll
-      for (int h = 1; h < G; ++h) s += __shfl(acc[q], lane + h * B);
-      if (lane < B) v[q][lane] = (Acc)x[(size_t)q * n + (size_t)row * B + lane] - s;
-    }
-    __syncthreads();
-    Acc z[NR];
-#pragma unroll
-    for (int q = 0; q < NR; ++q) z[q] = Acc(0);
-    if (g < G) {
-      const TA* d = diag + (size_t)row * B * B;
-#pragma unroll
-      for (int c = g; c < B; c += G) {
-        const Acc dv = (Acc)ldnt(d + c * B + r);
-#pragma unroll
-        for (int q = 0; q < NR; ++q) z[q] += dv * v[q][c];
-      }
-    }
-#pragma unroll
-    for (int q = 0; q < NR; ++q) {
-      Acc s = z[q];
-#pragma unroll
-      for (int h = 1; h < G; ++h) s += __shfl(z[q], lane + h * B);
-      if (lane < B) st_coherent(&y[(size_t)q * n + (size_t)row * B + lane], (TX)s);
-    }
-    __syncthreads();
-    if (lane == 0) __hip_atomic_store(&flags[row], epoch, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+    for (int h = 1; h < G; ++h) s += __shfl(z[q], lane + h * B);
+    if (row >= 0 && lane < B) y[(size_t)q * n + (size_t)row * B + lane] = (TX)s;
   }
 }
 
@@ -371,9 +344,10 @@ static void build_tri(const BlockCsr& f, bool forward, int prec, DevTri& t) {
           for (int c = 0; c < b; ++c) o[c * b + r] = s[r * b + c];
       }
   }
-  // level count of the dependency graph (reported, not used by the kernel)
+  // Dependency levels: a row's level is one more than the deepest row it
+  // depends on; rows are listed level by level for the launches.
   std::vector<int> lev(nsup, 0);
-  int mx = 0;
+  int mx = -1;
   for (int s = 0; s < nsup; ++s) {
     const int row = forward ? s : nsup - 1 - s;
     int l = 0;
@@ -381,7 +355,16 @@ static void build_tri(const BlockCsr& f, bool forward, int prec, DevTri& t) {
     lev[row] = l;
     mx = std::max(mx, l);
   }
-  t.levels = nsup ? mx + 1 : 0;
+  t.levels = mx + 1;
+  t.level_ptr.assign(t.levels + 1, 0);
+  for (int i = 0; i < nsup; ++i) t.level_ptr[lev[i] + 1]++;
+  for (int l = 0; l < t.levels; ++l) t.level_ptr[l + 1] += t.level_ptr[l];
+  std::vector<int> rows(nsup), fill(t.level_ptr.begin(), t.level_ptr.end() - 1);
+  for (int s = 0; s < nsup; ++s) {
+    const int row = forward ? s : nsup - 1 - s;
+    rows[fill[lev[row]]++] = row;
+  }
+  t.level_rows.upload(rows.data(), rows.size() * 4);
   t.ndep = ia[nsup];
   t.ia.upload(ia.data(), ia.size() * 4);
   t.ja.upload(ja.empty() ? ia.data() : ja.data(), std::max<size_t>(ja.size(), 1) * 4);
@@ -393,25 +376,14 @@ static void build_tri(const BlockCsr& f, bool forward, int prec, DevTri& t) {
 void dev_tri_forward(const BlockCsr& f, int prec, DevTri& t) { build_tri(f, true, prec, t); }
 void dev_tri_backward(const BlockCsr& f, int prec, DevTri& t) { build_tri(f, false, prec, t); }
 
-void SweepSync::ensure(int n) {
-  if (n <= nsup && flags.p) return;
-  flags.alloc((size_t)std::max(n, 1) * 4);
-  BCM_HIP(hipMemset(flags.p, 0, flags.bytes));
-  if (!ctl.p) {
-    ctl.alloc(64);
-    BCM_HIP(hipMemset(ctl.p, 0, 64));
-  } else {
-    // restart the epochs so no stale flag can match
-    BCM_HIP(hipMemset(ctl.p, 0, 64));
-  }
-  nsup = n;
-}
-
 void RedWork::ensure() {
   if (partials.p) return;
   partials.alloc((size_t)2048 * kMaxDots * 8);
   counter.alloc(64);
+  // hipMemset is ordered on the null stream only; the solver streams are
+  // non-blocking, so the reset is completed before any of them can run.
   BCM_HIP(hipMemset(counter.p, 0, 64));
+  BCM_HIP(hipDeviceSynchronize());
 }
 
 // ---------------------------------------------------------------------------
@@ -472,51 +444,52 @@ void launch_spmv(const DevBsr& A, const double* x, double* y, const double* bsub
 }
 
 template <int B, int NR, class TA, class TX>
-static void sweep_b(const DevTri& T, const void* x, void* y, SweepSync& sync, hipStream_t s, const int* skip) {
-  const int waves = std::min(T.nsup, std::max(1, device_cu_count() * 8));
-  hipLaunchKernelGGL(k_sweep_arm, dim3(1), dim3(1), 0, s, sync.ctl.as<int>());
-  hipLaunchKernelGGL((k_sweep<B, NR, TA, TX>), dim3(waves), dim3(64), 0, s, T.nsup, T.n, T.forward ? 1 : 0,
-                     T.ia.as<int>(), T.ja.as<int>(), T.off.as<TA>(), T.diag.as<TA>(),
-                     static_cast<const TX*>(x), static_cast<TX*>(y), sync.flags.as<int>(), sync.ctl.as<int>(), skip);
+static void sweep_b(const DevTri& T, const void* x, void* y, hipStream_t s, const int* skip) {
+  const int* rows = T.level_rows.as<int>();
+  for (int l = 0; l < T.levels; ++l) {
+    const int r0 = T.level_ptr[l], nr = T.level_ptr[l + 1] - r0;
+    hipLaunchKernelGGL((k_sweep_level<B, NR, TA, TX>), dim3((nr + 3) / 4), dim3(256), 0, s, nr, rows + r0, T.n,
+                       T.ia.as<int>(), T.ja.as<int>(), T.off.as<TA>(), T.diag.as<TA>(), static_cast<const TX*>(x),
+                       static_cast<TX*>(y), skip);
+  }
   BCM_HIP(hipGetLastError());
 }
 
 template <int B, int NR>
-static void sweep_prec(const DevTri& T, const void* x, void* y, int vec_prec, SweepSync& sync, hipStream_t s,
-                       const int* skip) {
-  if (T.prec == kF64 && vec_prec == kF64) sweep_b<B, NR, double, double>(T, x, y, sync, s, skip);
-  else if (T.prec == kF32 && vec_prec == kF32) sweep_b<B, NR, float, float>(T, x, y, sync, s, skip);
-  else if (T.prec == kBF16 && vec_prec == kF32) sweep_b<B, NR, uint16_t, float>(T, x, y, sync, s, skip);
+static void sweep_prec(const DevTri& T, const void* x, void* y, int vec_prec, hipStream_t s, const int* skip) {
+  if (T.prec == kF64 && vec_prec == kF64) sweep_b<B, NR, double, double>(T, x, y, s, skip);
+  else if (T.prec == kF32 && vec_prec == kF32) sweep_b<B, NR, float, float>(T, x, y, s, skip);
+  else if (T.prec == kBF16 && vec_prec == kF32) sweep_b<B, NR, uint16_t, float>(T, x, y, s, skip);
   else throw Error(kErrArg, "unsupported factor/vector precision pair (fp64/fp64, fp32/fp32, bf16/fp32)");
 }
 
 template <int B>
-static void sweep_nr(const DevTri& T, const void* x, void* y, int vec_prec, int nrhs, SweepSync& sync,
-                     hipStream_t s, const int* skip) {
+static void sweep_nr(const DevTri& T, const void* x, void* y, int vec_prec, int nrhs, hipStream_t s,
+                     const int* skip) {
   const size_t es = prec_bytes(vec_prec);
   int done = 0;
   while (done < nrhs) {
     const int left = nrhs - done;
     const char* xs = static_cast<const char*>(x) + (size_t)done * T.n * es;
     char* ys = static_cast<char*>(y) + (size_t)done * T.n * es;
-    if (left >= 6) { sweep_prec<B, 6>(T, xs, ys, vec_prec, sync, s, skip); done += 6; }
-    else if (left >= 3) { sweep_prec<B, 3>(T, xs, ys, vec_prec, sync, s, skip); done += 3; }
-    else { sweep_prec<B, 1>(T, xs, ys, vec_prec, sync, s, skip); done += 1; }
+    if (left >= 6) { sweep_prec<B, 6>(T, xs, ys, vec_prec, s, skip); done += 6; }
+    else if (left >= 3) { sweep_prec<B, 3>(T, xs, ys, vec_prec, s, skip); done += 3; }
+    else { sweep_prec<B, 1>(T, xs, ys, vec_prec, s, skip); done += 1; }
   }
 }
 
-void launch_sweep(const DevTri& T, const void* x, void* y, int vec_prec, int nrhs, SweepSync& sync,
-                  hipStream_t s, const int* skip) {
-  sync.ensure(T.nsup);
+void launch_sweep(const DevTri& T, const void* x, void* y, int vec_prec, int nrhs, hipStream_t s,
+                  const int* skip) {
+  if (x == y) throw Error(kErrArg, "triangular sweep: input and output must not alias");
   switch (T.b) {
-    case 3: sweep_nr<3>(T, x, y, vec_prec, nrhs, sync, s, skip); break;
-    case 6: sweep_nr<6>(T, x, y, vec_prec, nrhs, sync, s, skip); break;
-    case 8: sweep_nr<8>(T, x, y, vec_prec, nrhs, sync, s, skip); break;
-    case 9: sweep_nr<9>(T, x, y, vec_prec, nrhs, sync, s, skip); break;
-    case 12: sweep_nr<12>(T, x, y, vec_prec, nrhs, sync, s, skip); break;
-    case 16: sweep_nr<16>(T, x, y, vec_prec, nrhs, sync, s, skip); break;
-    case 24: sweep_nr<24>(T, x, y, vec_prec, nrhs, sync, s, skip); break;
-    case 32: sweep_nr<32>(T, x, y, vec_prec, nrhs, sync, s, skip); break;
+    case 3: sweep_nr<3>(T, x, y, vec_prec, nrhs, s, skip); break;
+    case 6: sweep_nr<6>(T, x, y, vec_prec, nrhs, s, skip); break;
+    case 8: sweep_nr<8>(T, x, y, vec_prec, nrhs, s, skip); break;
+    case 9: sweep_nr<9>(T, x, y, vec_prec, nrhs, s, skip); break;
+    case 12: sweep_nr<12>(T, x, y, vec_prec, nrhs, s, skip); break;
+    case 16: sweep_nr<16>(T, x, y, vec_prec, nrhs, s, skip); break;
+    case 24: sweep_nr<24>(T, x, y, vec_prec, nrhs, s, skip); break;
+    case 32: sweep_nr<32>(T, x, y, vec_prec, nrhs, s, skip); break;
     default: throw Error(kErrArg, "unsupported block size for the device sweep (use 3,6,8,9,12,16,24,32)");
   }
 }

@@ -242,7 +242,8 @@ Context::Context() {
   BCM_HIP(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
   own_stream = true;
   kstate.alloc(sizeof(KState));
-  BCM_HIP(hipMemset(kstate.p, 0, sizeof(KState)));
+  BCM_HIP(hipMemsetAsync(kstate.p, 0, sizeof(KState), stream));
+  BCM_HIP(hipStreamSynchronize(stream));
   BCM_HIP(hipHostMalloc((void**)&host_state, sizeof(KState), hipHostMallocDefault));
   std::memset(host_state, 0, sizeof(KState));
   rw.ensure();
@@ -250,6 +251,8 @@ Context::Context() {
 
 Context::~Context() {
   if (stream) (void)hipStreamSynchronize(stream);
+  for (auto& p : pending) { (void)hipEventDestroy(p.e0); (void)hipEventDestroy(p.e1); }
+  for (auto e : ev_pool) (void)hipEventDestroy(e);
   if (host_state) (void)hipHostFree(host_state);
   if (own_stream && stream) (void)hipStreamDestroy(stream);
 }
@@ -265,12 +268,36 @@ void Context::pull_state() {
   BCM_HIP(hipStreamSynchronize(stream));
 }
 
-void Context::check_sweep_error() {
-  if (!sync.ctl.p) return;
-  int ctl[4] = {0, 0, 0, 0};
-  BCM_HIP(hipMemcpyAsync(ctl, sync.ctl.p, sizeof(ctl), hipMemcpyDeviceToHost, stream));
-  BCM_HIP(hipStreamSynchronize(stream));
-  if (ctl[2]) throw Error(kErrInternal, "triangular sweep: dependency wait exceeded its spin limit");
+hipEvent_t Context::prof_begin() {
+  hipEvent_t e;
+  if (ev_pool.empty()) BCM_HIP(hipEventCreate(&e));
+  else { e = ev_pool.back(); ev_pool.pop_back(); }
+  BCM_HIP(hipEventRecord(e, stream));
+  return e;
+}
+
+void Context::prof_end(int kind, hipEvent_t e0, double bytes) {
+  hipEvent_t e1;
+  if (ev_pool.empty()) BCM_HIP(hipEventCreate(&e1));
+  else { e1 = ev_pool.back(); ev_pool.pop_back(); }
+  BCM_HIP(hipEventRecord(e1, stream));
+  pending.push_back(Pending{kind, e0, e1, bytes});
+  if (pending.size() > 4096) prof_flush();
+}
+
+void Context::prof_flush() {
+  if (pending.empty()) return;
+  BCM_HIP(hipEventSynchronize(pending.back().e1));
+  for (auto& p : pending) {
+    float ms = 0.f;
+    BCM_HIP(hipEventElapsedTime(&ms, p.e0, p.e1));
+    prof_acc[p.kind].ms += ms;
+    prof_acc[p.kind].launches += 1;
+    prof_acc[p.kind].bytes += p.bytes;
+    ev_pool.push_back(p.e0);
+    ev_pool.push_back(p.e1);
+  }
+  pending.clear();
 }
 
 void precond_apply(Factor& F, Context& ctx, const double* r, double* z, int prec, bool transpose, int nrhs,
@@ -281,14 +308,24 @@ void precond_apply(Factor& F, Context& ctx, const double* r, double* z, int prec
   const int64_t n = t1.n;
   if (prec == kF64) {
     double* tmp = ctx.vec(12, (size_t)n * nrhs * 8);
-    launch_sweep(t1, r, tmp, kF64, nrhs, ctx.sync, ctx.stream, skip);
-    launch_sweep(t2, tmp, z, kF64, nrhs, ctx.sync, ctx.stream, skip);
+    {
+      ProfScope ps(ctx, kProfSweep, sweep_bytes(t1, nrhs));
+      launch_sweep(t1, r, tmp, kF64, nrhs, ctx.stream, skip);
+    }
+    ProfScope ps(ctx, kProfSweep, sweep_bytes(t2, nrhs));
+    launch_sweep(t2, tmp, z, kF64, nrhs, ctx.stream, skip);
   } else {
     float* r32 = reinterpret_cast<float*>(ctx.vec(13, (size_t)n * nrhs * 4));
     float* t32 = reinterpret_cast<float*>(ctx.vec(14, (size_t)n * nrhs * 4));
     launch_convert(n * nrhs, r, kF64, r32, kF32, ctx.stream);
-    launch_sweep(t1, r32, t32, kF32, nrhs, ctx.sync, ctx.stream, skip);
-    launch_sweep(t2, t32, r32, kF32, nrhs, ctx.sync, ctx.stream, skip);
+    {
+      ProfScope ps(ctx, kProfSweep, sweep_bytes(t1, nrhs));
+      launch_sweep(t1, r32, t32, kF32, nrhs, ctx.stream, skip);
+    }
+    {
+      ProfScope ps(ctx, kProfSweep, sweep_bytes(t2, nrhs));
+      launch_sweep(t2, t32, r32, kF32, nrhs, ctx.stream, skip);
+    }
     launch_convert(n * nrhs, r32, kF32, z, kF64, ctx.stream);
   }
 }
@@ -347,7 +384,10 @@ void pcg(Matrix& A, Factor& F, Context& ctx, const SolveOptions& opt, const doub
         launch_multidot(n, 2, xs, ys, &ks->sc[kEnrsq], ctx.rw, s, done);  // (r,r) -> 8, (r,z) -> 9
         hipLaunchKernelGGL(k_scalar, dim3(1), dim3(1), 0, s, kOpPcgCheck, ks, eps2, opt.ittype, opt.niter - total);
         hipLaunchKernelGGL(k_pcg_p, dim3(ew_grid(n)), dim3(256), 0, s, (int64_t)n, ks, z, p);
-        launch_spmv(A.d, p, q, nullptr, 1, &ks->sc[kPq], p, ctx.rw, s, done);
+        {
+          ProfScope ps(ctx, kProfSpmv, spmv_bytes(A.d, 1));
+          launch_spmv(A.d, p, q, nullptr, 1, &ks->sc[kPq], p, ctx.rw, s, done);
+        }
         hipLaunchKernelGGL(k_scalar, dim3(1), dim3(1), 0, s, kOpPcgAlpha, ks, eps2, opt.ittype, opt.niter - total);
         hipLaunchKernelGGL(k_pcg_xr, dim3(ew_grid(n)), dim3(256), 0, s, (int64_t)n, ks, p, q, x, r);
         ++it;
@@ -355,7 +395,6 @@ void pcg(Matrix& A, Factor& F, Context& ctx, const SolveOptions& opt, const doub
       ctx.pull_state();
       if (ctx.host_state->st[0]) break;
     }
-    ctx.check_sweep_error();
     const KState& hs = *ctx.host_state;
     const int k = hs.st[1];
     total += std::max(0, k - 1);
@@ -407,14 +446,20 @@ void bicgstab(Matrix& A, Factor& F, Context& ctx, const SolveOptions& opt, const
       for (int c = 0; c < opt.check_every; ++c) {
         hipLaunchKernelGGL(k_bicg_p, dim3(ge), dim3(256), 0, s, (int64_t)n, ks, r, v, p);
         precond_apply(F, ctx, p, phat, opt.prec, false, 1, done);
-        launch_spmv(A.d, phat, v, nullptr, 1, &ks->sc[kRv], rhat, ctx.rw, s, done);
+        {
+          ProfScope ps(ctx, kProfSpmv, spmv_bytes(A.d, 1));
+          launch_spmv(A.d, phat, v, nullptr, 1, &ks->sc[kRv], rhat, ctx.rw, s, done);
+        }
         hipLaunchKernelGGL(k_scalar, dim3(1), dim3(1), 0, s, kOpBicgAlpha, ks, eps2, opt.ittype, opt.niter);
         hipLaunchKernelGGL(k_bicg_s, dim3(ge), dim3(256), 0, s, (int64_t)n, ks, r, v, sv,
                            ctx.rw.partials.as<double>(), ctx.rw.counter.as<unsigned>());
         hipLaunchKernelGGL(k_bicg_xs, dim3(ge), dim3(256), 0, s, (int64_t)n, ks, eps2, opt.ittype, phat, x);
         hipLaunchKernelGGL(k_scalar, dim3(1), dim3(1), 0, s, kOpBicgSCheck, ks, eps2, opt.ittype, opt.niter);
         precond_apply(F, ctx, sv, shat, opt.prec, false, 1, done);
-        launch_spmv(A.d, shat, t, nullptr, 1, nullptr, nullptr, ctx.rw, s, done);
+        {
+          ProfScope ps(ctx, kProfSpmv, spmv_bytes(A.d, 1));
+          launch_spmv(A.d, shat, t, nullptr, 1, nullptr, nullptr, ctx.rw, s, done);
+        }
         {
           const double* xs[2] = {t, t};
           const double* ys[2] = {sv, t};
@@ -430,7 +475,6 @@ void bicgstab(Matrix& A, Factor& F, Context& ctx, const SolveOptions& opt, const
       ctx.pull_state();
       if (ctx.host_state->st[0]) break;
     }
-    ctx.check_sweep_error();
     const KState& hs = *ctx.host_state;
     total += hs.st[1];
     st.breakdown = hs.st[2];

@@ -41,10 +41,17 @@ struct KState {
   int st[8];  // [0] done, [1] iteration, [2] breakdown, [3] converged-on-s (BiCGStab)
 };
 
+// Per-kernel-class timing with HIP events recorded on the context's stream.
+enum ProfKind : int { kProfSpmv = 0, kProfSweep = 1, kProfKinds = 2 };
+struct ProfAcc {
+  double ms = 0.0;
+  long long launches = 0;
+  double bytes = 0.0;
+};
+
 struct Context {
   hipStream_t stream = nullptr;
   bool own_stream = false;
-  SweepSync sync;
   RedWork rw;
   DevBuf kstate;        // KState
   KState* host_state = nullptr;  // pinned mirror
@@ -54,8 +61,37 @@ struct Context {
   KState* dstate() const { return kstate.as<KState>(); }
   double* vec(int i, size_t nbytes);  // returns work vector i of at least nbytes
   void pull_state();                  // copy state to the pinned mirror and wait
-  void check_sweep_error();
+  // profiling
+  bool prof = false;
+  ProfAcc prof_acc[kProfKinds];
+  std::vector<hipEvent_t> ev_pool;
+  struct Pending { int kind; hipEvent_t e0, e1; double bytes; };
+  std::vector<Pending> pending;
+  hipEvent_t prof_begin();
+  void prof_end(int kind, hipEvent_t e0, double bytes);
+  void prof_flush();
 };
+
+// Times the launches issued during its lifetime as one kernel of `kind`.
+struct ProfScope {
+  Context& c;
+  int kind;
+  double bytes;
+  hipEvent_t e0 = nullptr;
+  ProfScope(Context& ctx, int k, double b) : c(ctx), kind(k), bytes(b) {
+    if (c.prof) e0 = c.prof_begin();
+  }
+  ~ProfScope() {
+    if (c.prof && e0) c.prof_end(kind, e0, bytes);
+  }
+};
+
+inline double spmv_bytes(const DevBsr& A, int nrhs) {
+  return (double)A.value_bytes() + (double)A.index_bytes() + 16.0 * A.n * nrhs;
+}
+inline double sweep_bytes(const DevTri& T, int nrhs) {
+  return (double)T.value_bytes() + (double)T.index_bytes() + 2.0 * prec_bytes(T.prec == kF64 ? kF64 : kF32) * T.n * nrhs;
+}
 
 enum SolverKind : int {
   kPcg = 0,          // reference serial Cg (cg.cpp:31-349)

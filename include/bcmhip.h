@@ -128,6 +128,17 @@ int bcm_context_create(bcm_context* out);
 int bcm_context_destroy(bcm_context c);
 int bcm_context_sync(bcm_context c);
 
+/* Per-kernel-class timing with HIP events on the context's stream (the
+ * reference's clock() statistics per operator, cg.cpp:743-764). Times are
+ * the sum over launches; bytes are the algorithmic bytes of those launches. */
+typedef struct bcm_profile {
+  double spmv_ms, sweep_ms;
+  long long spmv_launches, sweep_launches;
+  double spmv_bytes, sweep_bytes;
+} bcm_profile;
+int bcm_context_profile(bcm_context c, int enable);
+int bcm_context_profile_read(bcm_context c, bcm_profile* out, int reset);
+
 /* ---- device operators: the CMVM / CSOLVEL / CSOLVEU / CSOLVELU plug-ins (svector.h:34-41) ---- */
 /* y = A x: CSMatrixRS::MvmA (mvm.cpp:2530); symmetric matrices: MvmASymm (mvm.cpp:3008). */
 int bcm_mvm(bcm_context c, bcm_matrix a, const double* x, double* y, int nrhs);

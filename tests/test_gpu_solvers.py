@@ -1,0 +1,140 @@
+"""GPU Krylov drivers through the C ABI against the CPU oracle.
+
+Parity bar: relative solution difference <= 1e-8 against the oracle on the
+same seeded inputs, FP64 true relative residual <= the requested tolerance,
+and the oracle's iteration count (PCG: exact; BiCGStab/FGMRES: +-1, the
+recurrences round differently).
+"""
+
+import ctypes as C
+
+import numpy as np
+import pytest
+
+from conftest import rel
+from helpers import heat, seepage
+
+pytestmark = pytest.mark.gpu
+
+
+def dev(B, x):
+    return B.DeviceArray.from_host(np.asfortranarray(x))
+
+
+def solve(B, ctx, mo, f, b, **opts):
+    db = dev(B, b)
+    dx = B.DeviceArray(len(b))
+    dx.zero()
+    st = ctx.solve(mo, f, db, dx, B.default_opts(**opts))
+    return dx.host(), st
+
+
+@pytest.mark.parametrize("ittype,eps", [(0, 1e-10), (3, 1e-10), (1, 1e-6)])
+def test_pcg_matches_oracle(B, O, ctx, ittype, eps):
+    h = heat(B, O, 16, 14, 9, seed=11)
+    print(f"pcg ittype={ittype} n={h['A'].n}", flush=True)
+    x, st = solve(B, ctx, h["mo"], h["f"], h["b"], kind=B.PCG, eps=eps, ittype=ittype, check_every=3)
+    xo, r = O.cg(h["A"], h["U"], h["b"], eps=eps, ittype=ittype)
+    assert st.converged == 1 and st.breakdown == 0
+    assert st.iterations == r.updates
+    assert rel(x, xo) < 1e-8 * max(1.0, eps / 1e-10)
+    if ittype == 3:
+        assert st.relres_true <= eps
+
+
+def test_bicgstab_matches_oracle(B, O, ctx):
+    s = seepage(B, O, 6, 5, 4, 24, seed=12)
+    x, st = solve(B, ctx, s["mo"], s["f"], s["b"], kind=B.BICGSTAB, eps=1e-10)
+    xo, r = O.bicgstab(s["A"], s["L"], s["U"], s["b"], eps=1e-10)
+    assert st.converged == 1 and st.relres_true <= 1e-10
+    assert abs(st.iterations - r.nitperf) <= 1
+    assert rel(x, xo) < 1e-8
+
+
+def test_pcg_warm_start_and_zero_rhs(B, O, ctx):
+    h = heat(B, O, 10, 10, 9, seed=13)
+    xd = np.linalg.solve(h["mo"].dense(), h["b"])
+    db, dx = dev(B, h["b"]), dev(B, xd * (1 + 1e-6))
+    st = ctx.solve(h["mo"], h["f"], db, dx, B.default_opts(kind=B.PCG, eps=1e-10))
+    assert st.converged and st.iterations < 15 and st.relres_true <= 1e-10
+    z = np.zeros(h["A"].n)
+    x, st = solve(B, ctx, h["mo"], h["f"], z, kind=B.PCG, eps=1e-10)
+    assert np.all(x == 0) and st.iterations == 0
+
+
+def _callbacks(B, m, rhs, store):
+    sprnd, ia, ja, a = m.arrays()
+    info = m.info()
+    nsup = info["nsup"]
+    rowstart = np.zeros(nsup + 1, np.int64)
+    for i in range(nsup):
+        bi = sprnd[i + 1] - sprnd[i]
+        rowstart[i + 1] = rowstart[i] + sum(bi * (sprnd[c + 1] - sprnd[c]) for c in ja[ia[i]:ia[i + 1]])
+    perm = {i: np.random.default_rng(i).permutation(ia[i + 1] - ia[i]) for i in range(nsup)}
+
+    def fnsup(obj, p):
+        p[0] = nsup
+
+    def fsprnds(obj, p):
+        for i in range(nsup + 1):
+            p[i] = int(sprnd[i])
+
+    def fiaja(obj, pia, pja):
+        ib = B.lib().bcm_alloc_ints(nsup + 1)
+        jb = B.lib().bcm_alloc_ints(len(ja))
+        for i in range(nsup + 1):
+            ib[i] = int(ia[i])
+        for i in range(nsup):   # callback order differs from sorted order: the library must sort
+            for k, src in enumerate(perm[i]):
+                jb[ia[i] + k] = int(ja[ia[i] + src])
+        pia[0] = ib
+        pja[0] = jb
+
+    def fblock(obj, i, out):
+        bi = sprnd[i + 1] - sprnd[i]
+        off = 0
+        offs = []
+        for c in ja[ia[i]:ia[i + 1]]:
+            offs.append(off)
+            off += bi * (sprnd[c + 1] - sprnd[c])
+        pos = 0
+        for src in perm[i]:
+            c = ja[ia[i] + src]
+            sz = bi * (sprnd[c + 1] - sprnd[c])
+            blk = a[rowstart[i] + offs[src]: rowstart[i] + offs[src] + sz]
+            for t in range(sz):
+                out[pos + t] = blk[t]
+            pos += sz
+
+    def frhs(obj, i, out):
+        for t in range(sprnd[i], sprnd[i + 1]):
+            out[t - sprnd[i]] = rhs[t]
+
+    def fini(obj, i, out):
+        for t in range(sprnd[i + 1] - sprnd[i]):
+            out[t] = 0.0
+
+    def fstore(obj, i, out):
+        for t in range(sprnd[i], sprnd[i + 1]):
+            store[t] = out[t - sprnd[i]]
+
+    sig = B.SIGNATURES["bcm_abstract_symm_solver"][1]
+    return [sig[2](fnsup), sig[3](fsprnds), sig[4](fiaja), sig[5](fblock), sig[6](frhs), sig[7](fini),
+            sig[8](fstore)]
+
+
+@pytest.mark.parametrize("symmetric", [True, False])
+def test_dropin_abstract_solvers(B, O, symmetric):
+    if symmetric:
+        m, rhs = B.gen_heat2d(9, 7, 9, 31)
+    else:
+        m, rhs = B.gen_seepage3d(4, 4, 3, 6, 0.3, 32)
+    x = np.zeros(m.n)
+    cbs = _callbacks(B, m, rhs, x)
+    p = B.default_params(ittype=3 if symmetric else 2, sttype=1, eps=1e-10, niter=500)
+    fn = B.lib().bcm_abstract_symm_solver if symmetric else B.lib().bcm_abstract_solver
+    code = fn(None, C.byref(p), *cbs)
+    assert code == 0, B.lib().bcm_last_error()
+    xd = np.linalg.solve(m.dense(), rhs)
+    assert rel(x, xd) < 1e-8
+    assert p.nitperf > 0 and p.timefct > 0 and p.density >= 1.0

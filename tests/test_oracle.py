@@ -1,0 +1,158 @@
+"""The CPU oracle, pinned before it is trusted.
+
+* Against the reference's own compiled solve-phase code (oracle/_ref, built
+  from Bslib's mvm.cpp, svector.cpp, cg.cpp, lanczos.cpp, rcm.cpp ...):
+  bit-for-bit equality.
+* Against the committed golden vectors (tests/golden/, written by the
+  reference build) where the reference build is absent.
+* Against closed forms: dense products and triangular solves, exact
+  preconditioners (one iteration), direct solves.
+"""
+
+import os
+
+import numpy as np
+import pytest
+
+from conftest import rel
+from helpers import dense_factor, heat, seepage
+
+GOLDEN = os.path.join(os.path.dirname(__file__), "golden", "solve_phase_small.npz")
+
+
+def need_ref(O):
+    if not O.ref_available():
+        pytest.skip("reference build (oracle/_ref) not present")
+
+
+@pytest.fixture(scope="module")
+def hs(B, O):
+    return heat(B, O)
+
+
+@pytest.fixture(scope="module")
+def ns(B, O):
+    return seepage(B, O)
+
+
+# ---- bit-exact against the reference build ----
+
+def test_mvm_family_bitexact(O, hs, ns):
+    need_ref(O)
+    rng = np.random.default_rng(0)
+    A = ns["A"]
+    x = rng.standard_normal((A.n, 3))
+    assert np.array_equal(O.mvma(A, x, 3), O.ref_mvma(A, x, 3))
+    assert np.array_equal(O.mvmat(A, x, 3), O.ref_mvmat(A, x, 3))
+    S = hs["A"]
+    xs = rng.standard_normal(S.n)
+    assert np.array_equal(O.mvmasymm(S, xs), O.ref_mvmasymm(S, xs))
+
+
+def test_triangular_bitexact(O, hs, ns):
+    need_ref(O)
+    rng = np.random.default_rng(1)
+    for f in (hs["U"], ns["L"], ns["U"]):
+        x = rng.standard_normal((f.n, 2))
+        assert np.array_equal(O.solvel(f, x, 2), O.ref_solvel(f, x, 2))
+        assert np.array_equal(O.solveu(f, x, 2), O.ref_solveu(f, x, 2))
+
+
+@pytest.mark.parametrize("ittype,eps", [(0, 1e-10), (1, 1e-6), (2, 1e-7)])
+def test_cg_bitexact(O, hs, ittype, eps):
+    need_ref(O)
+    x0, r = O.cg(hs["A"], hs["U"], hs["b"], eps=eps, ittype=ittype)
+    x1, nit, _ = O.ref_cg(hs["A"], hs["U"], hs["b"], eps=eps, ittype=ittype)
+    assert r.nitperf == nit
+    assert np.array_equal(x0, x1)
+
+
+@pytest.mark.parametrize("sttype,eps", [(0, 1e-8), (1, 1e-10), (2, 1e-9)])
+def test_lanczos_bitexact(O, ns, sttype, eps):
+    need_ref(O)
+    x0, r = O.lanczos(ns["A"], ns["L"], ns["U"], ns["b"], eps=eps, sttype=sttype)
+    x1, nit, _ = O.ref_lanczos(ns["A"], ns["L"], ns["U"], ns["b"], eps=eps, sttype=sttype)
+    assert r.nitperf == nit
+    assert np.array_equal(x0, x1)
+
+
+def test_scprod_bitexact(O):
+    need_ref(O)
+    rng = np.random.default_rng(2)
+    x, y = rng.standard_normal(1001), rng.standard_normal(1001)
+    assert O.orc().orc_scprod(1001, O._dp(x), O._dp(y)) == O.ref().ref_scprod(1001, O._dp(x), O._dp(y))
+
+
+# ---- golden vectors (written by the reference build) ----
+
+def test_golden_vectors(O):
+    if not os.path.exists(GOLDEN):
+        pytest.skip("golden fixture not generated")
+    g = np.load(GOLDEN, allow_pickle=False)
+    A = O.Block(g["a_sprnd"], g["a_ia"], g["a_ja"], g["a_a"])
+    U = O.Block(g["a_sprnd"], g["u_ia"], g["u_ja"], g["u_a"])
+    assert np.array_equal(O.mvmasymm(A, g["x"]), g["mvmasymm"])
+    assert np.array_equal(O.solvel(U, g["x"]), g["solvel"])
+    assert np.array_equal(O.solveu(U, g["x"]), g["solveu"])
+    x, r = O.cg(A, U, g["b"], eps=1e-10, ittype=0)
+    assert r.nitperf == int(g["cg_nit"])
+    assert np.array_equal(x, g["cg_x"])
+    N = O.Block(g["n_sprnd"], g["n_ia"], g["n_ja"], g["n_a"])
+    L = O.Block(g["n_sprnd"], g["l_ia"], g["l_ja"], g["l_a"])
+    Un = O.Block(g["n_sprnd"], g["nu_ia"], g["nu_ja"], g["nu_a"])
+    assert np.array_equal(O.mvma(N, g["xn"]), g["mvma"])
+    assert np.array_equal(O.mvmat(N, g["xn"]), g["mvmat"])
+    xl, rl = O.lanczos(N, L, Un, g["bn"], eps=1e-10, sttype=1)
+    assert rl.nitperf == int(g["lz_nit"])
+    assert np.array_equal(xl, g["lz_x"])
+
+
+# ---- closed forms ----
+
+def test_products_closed_form(B, O, hs, ns):
+    rng = np.random.default_rng(3)
+    Ad = ns["mo"].dense()
+    x = rng.standard_normal(Ad.shape[0])
+    assert rel(O.mvma(ns["A"], x), Ad @ x) < 1e-14
+    assert rel(O.mvmat(ns["A"], x), Ad.T @ x) < 1e-14
+    Sd = hs["mo"].dense()
+    xs = rng.standard_normal(Sd.shape[0])
+    assert rel(O.mvmasymm(hs["A"], xs), Sd @ xs) < 1e-14
+
+
+def test_triangular_closed_form(O, hs, ns):
+    rng = np.random.default_rng(4)
+    for f in (hs["U"], ns["L"], ns["U"]):
+        D = dense_factor(f)
+        x = rng.standard_normal(f.n)
+        assert rel(O.solvel(f, x), np.linalg.solve(D.T, x)) < 1e-12
+        assert rel(O.solveu(f, x), np.linalg.solve(D, x)) < 1e-12
+
+
+def test_exact_preconditioner_one_step(B, O):
+    """tau1 = tau2 = theta = 0 keeps every block: the factors are exact and every
+    Krylov method stops after one correction."""
+    h = heat(B, O, 5, 4, 6, seed=9, tau1=0.0, tau2=0.0, theta=0.0)
+    x, r = O.cg(h["A"], h["U"], h["b"], eps=1e-10, ittype=0)
+    assert r.updates == 1
+    assert rel(x, np.linalg.solve(h["mo"].dense(), h["b"])) < 1e-12
+    s = seepage(B, O, 4, 3, 3, 4, seed=5, tau1=0.0, tau2=0.0, theta=0.0)
+    xd = np.linalg.solve(s["mo"].dense(), s["b"])
+    xb, rb = O.bicgstab(s["A"], s["L"], s["U"], s["b"], eps=1e-10)
+    assert rb.nitperf == 1 and rel(xb, xd) < 1e-12
+    xg, rg = O.fgmres(s["A"], s["L"], s["U"], s["b"], eps=1e-10)
+    assert rg.nitperf == 1 and rel(xg, xd) < 1e-12
+
+
+def test_krylov_against_direct(O, hs, ns):
+    xd = np.linalg.solve(hs["mo"].dense(), hs["b"])
+    x, r = O.cg(hs["A"], hs["U"], hs["b"], eps=1e-12, ittype=3)
+    assert rel(x, xd) < 1e-9
+    Nd = ns["mo"].dense()
+    xd = np.linalg.solve(Nd, ns["b"])
+    for fn in (O.bicgstab, O.fgmres):
+        x, r = fn(ns["A"], ns["L"], ns["U"], ns["b"], eps=1e-12)
+        assert np.linalg.norm(ns["b"] - Nd @ x) <= 1.01e-12 * np.linalg.norm(ns["b"])
+        assert rel(x, xd) < 1e-9
+    x, r = O.lanczos(ns["A"], ns["L"], ns["U"], ns["b"], eps=1e-12, sttype=1)
+    assert rel(x, xd) < 1e-8
