@@ -111,6 +111,7 @@ SIGNATURES = {
     "bcm_context_create": (C.c_int, [C.POINTER(_P)]),
     "bcm_context_destroy": (C.c_int, [_P]),
     "bcm_context_sync": (C.c_int, [_P]),
+    "bcm_context_set_sweep_mode": (C.c_int, [_P, C.c_int]),
     "bcm_context_profile": (C.c_int, [_P, C.c_int]),
     "bcm_context_profile_read": (C.c_int, [_P, C.POINTER(Profile), C.c_int]),
     "bcm_mvm": (C.c_int, [_P, _P, C.c_void_p, C.c_void_p, C.c_int]),
@@ -376,6 +377,10 @@ class Context:
 
     def sync(self):
         _check(lib().bcm_context_sync(self.h))
+
+    def set_sweep_mode(self, mode: int):
+        """0: one launch per dependency level; 1: one sync-free launch per sweep."""
+        _check(lib().bcm_context_set_sweep_mode(self.h, mode))
 
     def profile(self, enable: bool = True):
         _check(lib().bcm_context_profile(self.h, int(enable)))

@@ -478,6 +478,15 @@ int bcm_context_sync(bcm_context c) {
   return guarded([&] {
     need(c != nullptr, "context_sync: null handle");
     BCM_HIP(hipStreamSynchronize(c->c.stream));
+    c->c.check_sweep_error();
+  });
+}
+
+int bcm_context_set_sweep_mode(bcm_context c, int mode) {
+  return guarded([&] {
+    need(c != nullptr && (mode == 0 || mode == 1), "context_set_sweep_mode: bad argument");
+    BCM_HIP(hipStreamSynchronize(c->c.stream));
+    c->c.sweep_mode = mode;
   });
 }
 
@@ -537,7 +546,7 @@ int bcm_solve_l(bcm_context c, bcm_factor f, int which, const double* x, double*
     need(x != y, "solve_l: x and y must not alias");
     const bool t = which == 1 && f->f.kind == 1;
     f->f.to_device(bcm::kF64, t);
-    bcm::launch_sweep(t ? f->f.fwdt : f->f.fwd, x, y, bcm::kF64, nrhs, c->c.stream);
+    bcm::launch_sweep(t ? f->f.fwdt : f->f.fwd, x, y, bcm::kF64, nrhs, c->c.stream, nullptr, c->c.sweep_sync());
   });
 }
 
@@ -547,7 +556,7 @@ int bcm_solve_u(bcm_context c, bcm_factor f, int which, const double* x, double*
     need(x != y, "solve_u: x and y must not alias");
     const bool t = which == 0 && f->f.kind == 1;
     f->f.to_device(bcm::kF64, t);
-    bcm::launch_sweep(t ? f->f.bwdt : f->f.bwd, x, y, bcm::kF64, nrhs, c->c.stream);
+    bcm::launch_sweep(t ? f->f.bwdt : f->f.bwd, x, y, bcm::kF64, nrhs, c->c.stream, nullptr, c->c.sweep_sync());
   });
 }
 

@@ -9,6 +9,9 @@
 namespace bcm {
 namespace {
 
+// Upper bound on the polls of one dependency wait of the sync-free sweep.
+constexpr int kSpinLimit = 1 << 22;
+
 // ---- value loaders ----
 __device__ __forceinline__ double ldv(const double* p) { return *p; }
 __device__ __forceinline__ float ldv(const float* p) { return *p; }

@@ -85,6 +85,15 @@ void dev_tri_forward(const BlockCsr& f, int prec, DevTri& t);
 // z_i = D_i (x_i - sum_{j>i} F_ij z_j).
 void dev_tri_backward(const BlockCsr& f, int prec, DevTri& t);
 
+// State of the sync-free sweeps on one stream: per-row completion epochs and
+// [0] epoch, [1] ticket, [2] error flag (a dependency wait hit its bound).
+struct SweepSync {
+  DevBuf flags, ctl;
+  int nsup = 0;
+  void ensure(int nsup, hipStream_t s);
+  int error(hipStream_t s);
+};
+
 // Reduction workspace: partial sums and the completion counter of the
 // single-pass (last block finalises) reductions.
 struct RedWork {
@@ -105,10 +114,11 @@ constexpr int kMaxDots = 8;
 void launch_spmv(const DevBsr& A, const double* x, double* y, const double* bsub, int nrhs,
                  double* dot_out, const double* wdot, RedWork& rw, hipStream_t s, const int* skip = nullptr);
 
-// Triangular sweep y = T^-1 x for the given precision of vectors (TX): one
-// launch per dependency level. x and y must not alias.
+// Triangular sweep y = T^-1 x for the given precision of vectors (TX). With
+// sync == nullptr: one launch per dependency level; otherwise one sync-free
+// launch using `sync`. x and y must not alias.
 void launch_sweep(const DevTri& T, const void* x, void* y, int vec_prec, int nrhs, hipStream_t s,
-                  const int* skip = nullptr);
+                  const int* skip = nullptr, SweepSync* sync = nullptr);
 
 // out[k] = sum_i x_k[i] * y_k[i] for K pairs (K <= kMaxDots), one pass.
 void launch_multidot(int n, int k, const double* const* xs, const double* const* ys, double* out,

@@ -48,7 +48,8 @@ __global__ __launch_bounds__(256) void k_spmv(int nsup, int n, const int* __rest
   const int lane = threadIdx.x & 63;
   const int r = lane % B, g = lane / B;
   const int tile = xcd_tile(blockIdx.x, gridDim.x);
-  const int wave_global = tile * (blockDim.x >> 6) + (threadIdx.x >> 6);
+  // wave-uniform by construction; readfirstlane makes the row loop a scalar loop
+  const int wave_global = __builtin_amdgcn_readfirstlane(tile * (blockDim.x >> 6) + (threadIdx.x >> 6));
   double dotv = 0.0;
   for (int q = 0; q < rows_per_wave; ++q) {
     const int row = wave_global * rows_per_wave + q;
@@ -64,7 +65,7 @@ __global__ __launch_bounds__(256) void k_spmv(int nsup, int n, const int* __rest
       for (int t = g; t < ncol; t += G) {
         const int bk = t / B, c = t - bk * B;
         const int col = ja[jb + bk];
-        const double av = (double)ldnt(arow + (size_t)bk * B * B + c * B + r);
+        const double av = (double)ldnt(arow + (size_t)t * B + r);  // block bk, column c
 #pragma unroll
         for (int k = 0; k < NR; ++k) acc[k] += av * x[(size_t)k * n + (size_t)col * B + c];
       }
@@ -143,7 +144,7 @@ __global__ __launch_bounds__(256) void k_sweep_level(int nrows, const int* __res
     for (int tt = g; tt < ncol; tt += G) {
       const int bk = tt / B, c = tt - bk * B;
       const int k = ja[jb + bk];
-      const Acc av = (Acc)ldnt(blk0 + (size_t)bk * B * B + c * B + r);
+      const Acc av = (Acc)ldnt(blk0 + (size_t)tt * B + r);
 #pragma unroll
       for (int q = 0; q < NR; ++q) acc[q] += av * (Acc)y[(size_t)q * n + (size_t)k * B + c];
     }
@@ -174,6 +175,141 @@ __global__ __launch_bounds__(256) void k_sweep_level(int nrows, const int* __res
 #pragma unroll
     for (int h = 1; h < G; ++h) s += __shfl(z[q], lane + h * B);
     if (row >= 0 && lane < B) y[(size_t)q * n + (size_t)row * B + lane] = (TX)s;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Sync-free triangular sweep (one cooperative launch per sweep). Wave w owns
+// rows w, w+W, w+2W, ... of the sweep order and processes them in that order;
+// every wave of the grid is resident (cooperative launch), so the smallest
+// unfinished row always belongs to a wave whose dependencies are complete and
+// the sweep cannot deadlock. A wave first loads its row's factor blocks into
+// registers, then waits (bounded) on the completion epochs of the rows it
+// depends on, then publishes its row.
+//
+// Cross-wave ordering follows the device-scope protocol for gfx942/gfx950
+// (as rocPRIM's look-back scan does): vector values and flags move through
+// device-scope relaxed atomic loads/stores, which are coherent across XCDs,
+// and workgroup-scope fences plus vmcnt waits only order them, with no L2
+// write-back or invalidation per row.
+//
+// The row loop is wave-uniform (its bounds are blockIdx/gridDim scalars);
+// every lane executes the publishing store.
+__global__ void k_sweep_arm(int* ctl) {
+  ctl[0] += 1;
+  ctl[1] = 0;
+}
+
+__device__ __forceinline__ void release_order_only() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+  __builtin_amdgcn_s_waitcnt(0 | (0x7 << 4) | (0xf << 8));  // vmcnt(0)
+}
+
+__device__ __forceinline__ void acquire_order_only() {
+  __builtin_amdgcn_s_waitcnt(0 | (0x7 << 4) | (0xf << 8));
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+}
+
+template <int B, int NR, class TA, class TX>
+__global__ __launch_bounds__(64) void k_sweep_free(int nsup, int n, int forward, const int* __restrict__ ia,
+                                                   const int* __restrict__ ja, const TA* __restrict__ off,
+                                                   const TA* __restrict__ diag, const TX* __restrict__ x, TX* y,
+                                                   int* flags, int* ctl, const int* skip) {
+  if (skip && *skip) return;
+  using Acc = typename AccOf<TX>::type;
+  constexpr int G = 64 / B;
+  constexpr int PF = (3 * B + G - 1) / G;   // off-diagonal values prefetched per lane (3 blocks)
+  constexpr int PD = (B + G - 1) / G;       // diagonal-block columns per lane
+  __shared__ Acc v[NR][B];
+  const int lane = threadIdx.x;
+  const int r = lane % B, g = lane / B;
+  const bool act = g < G;
+  const int epoch = __builtin_amdgcn_readfirstlane(ctl[0]);
+  for (int t = blockIdx.x; t < nsup; t += gridDim.x) {
+    const int row = forward ? t : nsup - 1 - t;
+    const int jb = ia[row], je = ia[row + 1];
+    const int ncol = (je - jb) * B;
+    const TA* blk0 = off + (size_t)jb * B * B;
+    const TA* d = diag + (size_t)row * B * B;
+    // factor values of this row do not depend on the wait: load them first
+    Acc pf[PF], pd[PD];
+    // column tt of the row's block stream (block tt/B, column tt%B) starts at blk0 + tt*B
+    const TA* lp = blk0 + g * B + r;
+#pragma unroll
+    for (int i = 0; i < PF; ++i) pf[i] = (act && g + i * G < ncol) ? (Acc)ldnt(lp + i * G * B) : Acc(0);
+#pragma unroll
+    for (int i = 0; i < PD; ++i) pd[i] = (act && g + i * G < B) ? (Acc)ldnt(d + (g + i * G) * B + r) : Acc(0);
+    // wait until every row this one depends on has been published
+    for (int base = jb; base < je; base += 64) {
+      const int j = base + lane;
+      if (j < je) {
+        const int k = ja[j];
+        int spins = 0;
+        while (__hip_atomic_load(&flags[k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != epoch) {
+          __builtin_amdgcn_s_sleep(1);
+          ++spins;
+          if (spins > kSpinLimit ||
+              ((spins & 1023) == 0 && __hip_atomic_load(&ctl[2], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))) {
+            __hip_atomic_store(&ctl[2], 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            break;
+          }
+        }
+      }
+    }
+    acquire_order_only();
+    Acc acc[NR];
+#pragma unroll
+    for (int q = 0; q < NR; ++q) acc[q] = Acc(0);
+    if (act) {
+#pragma unroll
+      for (int i = 0; i < PF; ++i) {
+        const int tt = g + i * G;
+        if (tt < ncol) {
+          const int bk = tt / B, c = tt - bk * B;
+          const int k = ja[jb + bk];
+#pragma unroll
+          for (int q = 0; q < NR; ++q)
+            acc[q] += pf[i] * (Acc)ld_coherent(&y[(size_t)q * n + (size_t)k * B + c]);
+        }
+      }
+      for (int tt = g + PF * G; tt < ncol; tt += G) {  // rows with more than three dependencies
+        const int bk = tt / B, c = tt - bk * B;
+        const int k = ja[jb + bk];
+        const Acc av = (Acc)ldnt(blk0 + (size_t)tt * B + r);
+        for (int q = 0; q < NR; ++q) acc[q] += av * (Acc)ld_coherent(&y[(size_t)q * n + (size_t)k * B + c]);
+      }
+    }
+#pragma unroll
+    for (int q = 0; q < NR; ++q) {
+      Acc s = acc[q];
+#pragma unroll
+      for (int h = 1; h < G; ++h) s += __shfl(acc[q], lane + h * B);
+      if (lane < B) v[q][lane] = (Acc)x[(size_t)q * n + (size_t)row * B + lane] - s;
+    }
+    __syncthreads();
+    Acc z[NR];
+#pragma unroll
+    for (int q = 0; q < NR; ++q) z[q] = Acc(0);
+    if (act) {
+#pragma unroll
+      for (int i = 0; i < PD; ++i) {
+        const int c = g + i * G;
+        if (c < B) {
+#pragma unroll
+          for (int q = 0; q < NR; ++q) z[q] += pd[i] * v[q][c];
+        }
+      }
+    }
+#pragma unroll
+    for (int q = 0; q < NR; ++q) {
+      Acc s = z[q];
+#pragma unroll
+      for (int h = 1; h < G; ++h) s += __shfl(z[q], lane + h * B);
+      if (lane < B) st_coherent(&y[(size_t)q * n + (size_t)row * B + lane], (TX)s);
+    }
+    __syncthreads();
+    release_order_only();
+    __hip_atomic_store(&flags[row], epoch, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   }
 }
 
@@ -376,6 +512,25 @@ static void build_tri(const BlockCsr& f, bool forward, int prec, DevTri& t) {
 void dev_tri_forward(const BlockCsr& f, int prec, DevTri& t) { build_tri(f, true, prec, t); }
 void dev_tri_backward(const BlockCsr& f, int prec, DevTri& t) { build_tri(f, false, prec, t); }
 
+void SweepSync::ensure(int n, hipStream_t s) {
+  if (n <= nsup && flags.p) return;
+  flags.alloc((size_t)std::max(n, 1) * 4);
+  if (!ctl.p) ctl.alloc(64);
+  // zero flags and epoch on the sweep's own stream, then wait: no stale epoch can match
+  BCM_HIP(hipMemsetAsync(flags.p, 0, flags.bytes, s));
+  BCM_HIP(hipMemsetAsync(ctl.p, 0, 64, s));
+  BCM_HIP(hipStreamSynchronize(s));
+  nsup = n;
+}
+
+int SweepSync::error(hipStream_t s) {
+  if (!ctl.p) return 0;
+  int h[4] = {0, 0, 0, 0};
+  BCM_HIP(hipMemcpyAsync(h, ctl.p, sizeof(h), hipMemcpyDeviceToHost, s));
+  BCM_HIP(hipStreamSynchronize(s));
+  return h[2];
+}
+
 void RedWork::ensure() {
   if (partials.p) return;
   partials.alloc((size_t)2048 * kMaxDots * 8);
@@ -444,7 +599,27 @@ void launch_spmv(const DevBsr& A, const double* x, double* y, const double* bsub
 }
 
 template <int B, int NR, class TA, class TX>
-static void sweep_b(const DevTri& T, const void* x, void* y, hipStream_t s, const int* skip) {
+static void sweep_b(const DevTri& T, const void* x, void* y, hipStream_t s, const int* skip, SweepSync* sync) {
+  if (sync) {
+    // every wave must be resident: cooperative launch of at most the occupancy limit
+    auto kern = k_sweep_free<B, NR, TA, TX>;
+    int per_cu = 0;
+    BCM_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, 64, 0));
+    const int waves = std::max(1, std::min(T.nsup, std::min(per_cu, 8) * device_cu_count()));
+    hipLaunchKernelGGL(k_sweep_arm, dim3(1), dim3(1), 0, s, sync->ctl.as<int>());
+    int nsup = T.nsup, n = T.n, fwd = T.forward ? 1 : 0;
+    const int* ia = T.ia.as<int>();
+    const int* ja = T.ja.as<int>();
+    const TA* off = T.off.as<TA>();
+    const TA* dg = T.diag.as<TA>();
+    const TX* xx = static_cast<const TX*>(x);
+    TX* yy = static_cast<TX*>(y);
+    int* fl = sync->flags.as<int>();
+    int* ct = sync->ctl.as<int>();
+    void* args[] = {&nsup, &n, &fwd, &ia, &ja, &off, &dg, &xx, &yy, &fl, &ct, (void*)&skip};
+    BCM_HIP(hipLaunchCooperativeKernel(reinterpret_cast<const void*>(kern), dim3(waves), dim3(64), args, 0, s));
+    return;
+  }
   const int* rows = T.level_rows.as<int>();
   for (int l = 0; l < T.levels; ++l) {
     const int r0 = T.level_ptr[l], nr = T.level_ptr[l + 1] - r0;
@@ -456,40 +631,42 @@ static void sweep_b(const DevTri& T, const void* x, void* y, hipStream_t s, cons
 }
 
 template <int B, int NR>
-static void sweep_prec(const DevTri& T, const void* x, void* y, int vec_prec, hipStream_t s, const int* skip) {
-  if (T.prec == kF64 && vec_prec == kF64) sweep_b<B, NR, double, double>(T, x, y, s, skip);
-  else if (T.prec == kF32 && vec_prec == kF32) sweep_b<B, NR, float, float>(T, x, y, s, skip);
-  else if (T.prec == kBF16 && vec_prec == kF32) sweep_b<B, NR, uint16_t, float>(T, x, y, s, skip);
+static void sweep_prec(const DevTri& T, const void* x, void* y, int vec_prec, hipStream_t s, const int* skip,
+                       SweepSync* sync) {
+  if (T.prec == kF64 && vec_prec == kF64) sweep_b<B, NR, double, double>(T, x, y, s, skip, sync);
+  else if (T.prec == kF32 && vec_prec == kF32) sweep_b<B, NR, float, float>(T, x, y, s, skip, sync);
+  else if (T.prec == kBF16 && vec_prec == kF32) sweep_b<B, NR, uint16_t, float>(T, x, y, s, skip, sync);
   else throw Error(kErrArg, "unsupported factor/vector precision pair (fp64/fp64, fp32/fp32, bf16/fp32)");
 }
 
 template <int B>
 static void sweep_nr(const DevTri& T, const void* x, void* y, int vec_prec, int nrhs, hipStream_t s,
-                     const int* skip) {
+                     const int* skip, SweepSync* sync) {
   const size_t es = prec_bytes(vec_prec);
   int done = 0;
   while (done < nrhs) {
     const int left = nrhs - done;
     const char* xs = static_cast<const char*>(x) + (size_t)done * T.n * es;
     char* ys = static_cast<char*>(y) + (size_t)done * T.n * es;
-    if (left >= 6) { sweep_prec<B, 6>(T, xs, ys, vec_prec, s, skip); done += 6; }
-    else if (left >= 3) { sweep_prec<B, 3>(T, xs, ys, vec_prec, s, skip); done += 3; }
-    else { sweep_prec<B, 1>(T, xs, ys, vec_prec, s, skip); done += 1; }
+    if (left >= 6) { sweep_prec<B, 6>(T, xs, ys, vec_prec, s, skip, sync); done += 6; }
+    else if (left >= 3) { sweep_prec<B, 3>(T, xs, ys, vec_prec, s, skip, sync); done += 3; }
+    else { sweep_prec<B, 1>(T, xs, ys, vec_prec, s, skip, sync); done += 1; }
   }
 }
 
 void launch_sweep(const DevTri& T, const void* x, void* y, int vec_prec, int nrhs, hipStream_t s,
-                  const int* skip) {
+                  const int* skip, SweepSync* sync) {
   if (x == y) throw Error(kErrArg, "triangular sweep: input and output must not alias");
+  if (sync) sync->ensure(T.nsup, s);
   switch (T.b) {
-    case 3: sweep_nr<3>(T, x, y, vec_prec, nrhs, s, skip); break;
-    case 6: sweep_nr<6>(T, x, y, vec_prec, nrhs, s, skip); break;
-    case 8: sweep_nr<8>(T, x, y, vec_prec, nrhs, s, skip); break;
-    case 9: sweep_nr<9>(T, x, y, vec_prec, nrhs, s, skip); break;
-    case 12: sweep_nr<12>(T, x, y, vec_prec, nrhs, s, skip); break;
-    case 16: sweep_nr<16>(T, x, y, vec_prec, nrhs, s, skip); break;
-    case 24: sweep_nr<24>(T, x, y, vec_prec, nrhs, s, skip); break;
-    case 32: sweep_nr<32>(T, x, y, vec_prec, nrhs, s, skip); break;
+    case 3: sweep_nr<3>(T, x, y, vec_prec, nrhs, s, skip, sync); break;
+    case 6: sweep_nr<6>(T, x, y, vec_prec, nrhs, s, skip, sync); break;
+    case 8: sweep_nr<8>(T, x, y, vec_prec, nrhs, s, skip, sync); break;
+    case 9: sweep_nr<9>(T, x, y, vec_prec, nrhs, s, skip, sync); break;
+    case 12: sweep_nr<12>(T, x, y, vec_prec, nrhs, s, skip, sync); break;
+    case 16: sweep_nr<16>(T, x, y, vec_prec, nrhs, s, skip, sync); break;
+    case 24: sweep_nr<24>(T, x, y, vec_prec, nrhs, s, skip, sync); break;
+    case 32: sweep_nr<32>(T, x, y, vec_prec, nrhs, s, skip, sync); break;
     default: throw Error(kErrArg, "unsupported block size for the device sweep (use 3,6,8,9,12,16,24,32)");
   }
 }

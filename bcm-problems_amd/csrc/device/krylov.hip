@@ -263,6 +263,11 @@ double* Context::vec(int i, size_t nbytes) {
   return vecs[i].as<double>();
 }
 
+void Context::check_sweep_error() {
+  if (sweep_mode == 1 && sync.error(stream))
+    throw Error(kErrInternal, "sync-free sweep: a dependency wait reached its bound");
+}
+
 void Context::pull_state() {
   BCM_HIP(hipMemcpyAsync(host_state, kstate.p, sizeof(KState), hipMemcpyDeviceToHost, stream));
   BCM_HIP(hipStreamSynchronize(stream));
@@ -310,21 +315,21 @@ void precond_apply(Factor& F, Context& ctx, const double* r, double* z, int prec
     double* tmp = ctx.vec(12, (size_t)n * nrhs * 8);
     {
       ProfScope ps(ctx, kProfSweep, sweep_bytes(t1, nrhs));
-      launch_sweep(t1, r, tmp, kF64, nrhs, ctx.stream, skip);
+      launch_sweep(t1, r, tmp, kF64, nrhs, ctx.stream, skip, ctx.sweep_sync());
     }
     ProfScope ps(ctx, kProfSweep, sweep_bytes(t2, nrhs));
-    launch_sweep(t2, tmp, z, kF64, nrhs, ctx.stream, skip);
+    launch_sweep(t2, tmp, z, kF64, nrhs, ctx.stream, skip, ctx.sweep_sync());
   } else {
     float* r32 = reinterpret_cast<float*>(ctx.vec(13, (size_t)n * nrhs * 4));
     float* t32 = reinterpret_cast<float*>(ctx.vec(14, (size_t)n * nrhs * 4));
     launch_convert(n * nrhs, r, kF64, r32, kF32, ctx.stream);
     {
       ProfScope ps(ctx, kProfSweep, sweep_bytes(t1, nrhs));
-      launch_sweep(t1, r32, t32, kF32, nrhs, ctx.stream, skip);
+      launch_sweep(t1, r32, t32, kF32, nrhs, ctx.stream, skip, ctx.sweep_sync());
     }
     {
       ProfScope ps(ctx, kProfSweep, sweep_bytes(t2, nrhs));
-      launch_sweep(t2, t32, r32, kF32, nrhs, ctx.stream, skip);
+      launch_sweep(t2, t32, r32, kF32, nrhs, ctx.stream, skip, ctx.sweep_sync());
     }
     launch_convert(n * nrhs, r32, kF32, z, kF64, ctx.stream);
   }
@@ -376,6 +381,7 @@ void pcg(Matrix& A, Factor& F, Context& ctx, const SolveOptions& opt, const doub
       launch_multidot(n, 2, xs, ys, &ks->sc[kBnorm2], ctx.rw, s);  // sc[20] = b.b, sc[21] = r0.r0
     }
     int it = 0;
+    const bool exact = opt.check_every == 1;
     while (true) {
       for (int c = 0; c < opt.check_every; ++c) {
         precond_apply(F, ctx, r, z, opt.prec, false, 1, done);
@@ -383,6 +389,10 @@ void pcg(Matrix& A, Factor& F, Context& ctx, const SolveOptions& opt, const doub
         const double* ys[2] = {r, z};
         launch_multidot(n, 2, xs, ys, &ks->sc[kEnrsq], ctx.rw, s, done);  // (r,r) -> 8, (r,z) -> 9
         hipLaunchKernelGGL(k_scalar, dim3(1), dim3(1), 0, s, kOpPcgCheck, ks, eps2, opt.ittype, opt.niter - total);
+        if (exact) {  // no launch is queued past the convergence test
+          ctx.pull_state();
+          if (ctx.host_state->st[0]) break;
+        }
         hipLaunchKernelGGL(k_pcg_p, dim3(ew_grid(n)), dim3(256), 0, s, (int64_t)n, ks, z, p);
         {
           ProfScope ps(ctx, kProfSpmv, spmv_bytes(A.d, 1));
@@ -442,6 +452,7 @@ void bicgstab(Matrix& A, Factor& F, Context& ctx, const SolveOptions& opt, const
       hipLaunchKernelGGL(k_move, dim3(1), dim3(1), 0, s, ks, 22, kRhoNew);
       hipLaunchKernelGGL(k_scalar, dim3(1), dim3(1), 0, s, kOpBicgInit, ks, eps2, opt.ittype, opt.niter);
     }
+    const bool exact = opt.check_every == 1;
     while (true) {
       for (int c = 0; c < opt.check_every; ++c) {
         hipLaunchKernelGGL(k_bicg_p, dim3(ge), dim3(256), 0, s, (int64_t)n, ks, r, v, p);
@@ -455,6 +466,10 @@ void bicgstab(Matrix& A, Factor& F, Context& ctx, const SolveOptions& opt, const
                            ctx.rw.partials.as<double>(), ctx.rw.counter.as<unsigned>());
         hipLaunchKernelGGL(k_bicg_xs, dim3(ge), dim3(256), 0, s, (int64_t)n, ks, eps2, opt.ittype, phat, x);
         hipLaunchKernelGGL(k_scalar, dim3(1), dim3(1), 0, s, kOpBicgSCheck, ks, eps2, opt.ittype, opt.niter);
+        if (exact) {  // no launch is queued past a convergence test
+          ctx.pull_state();
+          if (ctx.host_state->st[0]) break;
+        }
         precond_apply(F, ctx, sv, shat, opt.prec, false, 1, done);
         {
           ProfScope ps(ctx, kProfSpmv, spmv_bytes(A.d, 1));
@@ -471,6 +486,10 @@ void bicgstab(Matrix& A, Factor& F, Context& ctx, const SolveOptions& opt, const
         hipLaunchKernelGGL(k_move, dim3(1), dim3(1), 0, s, ks, kRr + 1, kRhoNew);
         hipLaunchKernelGGL(k_scalar, dim3(1), dim3(1), 0, s, kOpBicgNext, ks, eps2, opt.ittype,
                            opt.niter - total);
+        if (exact) {
+          ctx.pull_state();
+          if (ctx.host_state->st[0]) break;
+        }
       }
       ctx.pull_state();
       if (ctx.host_state->st[0]) break;
@@ -505,6 +524,7 @@ void solve(Matrix& A, Factor& F, Context& ctx, const SolveOptions& opt, const do
     default: throw Error(kErrArg, "solver kind not available");
   }
   BCM_HIP(hipStreamSynchronize(ctx.stream));
+  ctx.check_sweep_error();
   st.seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
 }
 

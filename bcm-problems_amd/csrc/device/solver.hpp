@@ -53,6 +53,10 @@ struct Context {
   hipStream_t stream = nullptr;
   bool own_stream = false;
   RedWork rw;
+  int sweep_mode = 1;   // 0: one launch per dependency level, 1: one sync-free launch per sweep
+  SweepSync sync;
+  SweepSync* sweep_sync() { return sweep_mode == 1 ? &sync : nullptr; }
+  void check_sweep_error();
   DevBuf kstate;        // KState
   KState* host_state = nullptr;  // pinned mirror
   std::vector<DevBuf> vecs;      // solver work vectors

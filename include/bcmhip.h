@@ -127,6 +127,10 @@ int bcm_dev_zero(void* p, unsigned long long bytes);
 int bcm_context_create(bcm_context* out);
 int bcm_context_destroy(bcm_context c);
 int bcm_context_sync(bcm_context c);
+/* Triangular sweep schedule: 0 = one launch per dependency level,
+ * 1 = one sync-free cooperative launch per sweep (default: rows owned
+ * round-robin by resident waves, bounded waits on per-row completion epochs). */
+int bcm_context_set_sweep_mode(bcm_context c, int mode);
 
 /* Per-kernel-class timing with HIP events on the context's stream (the
  * reference's clock() statistics per operator, cg.cpp:743-764). Times are

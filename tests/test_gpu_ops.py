@@ -14,6 +14,13 @@ from helpers import heat, seepage
 pytestmark = pytest.mark.gpu
 
 
+@pytest.fixture(scope="module", params=[0, 1], ids=["levels", "syncfree"])
+def ctx(B, request):
+    c = B.Context()
+    c.set_sweep_mode(request.param)
+    return c
+
+
 def dev(B, x):
     return B.DeviceArray.from_host(np.asfortranarray(x))
 
