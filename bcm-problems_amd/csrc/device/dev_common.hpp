@@ -39,6 +39,15 @@ template <class T> __device__ __forceinline__ void st_coherent(T* p, T v) {
 // Block-level reduction of K values followed by the last-block finalisation:
 // block sums go to partials[block*K + k]; the block that completes last adds
 // them in a fixed order (deterministic) into out[k] and re-arms the counter.
+// Orders this wave's LDS writes before its following LDS reads by other
+// lanes (and reads before later writes). LDS operations of one wave complete
+// in program order, so a wavefront-scope fence, which only stops the compiler
+// from moving memory operations across it, is enough.
+__device__ __forceinline__ void wave_lds_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+}
+
 template <int K>
 __device__ void reduce_finalize(double (&v)[K], double* partials, unsigned* counter, double* out) {
   __shared__ double sm[K][16];

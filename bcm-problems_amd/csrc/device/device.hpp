@@ -85,10 +85,10 @@ void dev_tri_forward(const BlockCsr& f, int prec, DevTri& t);
 // z_i = D_i (x_i - sum_{j>i} F_ij z_j).
 void dev_tri_backward(const BlockCsr& f, int prec, DevTri& t);
 
-// State of the sync-free sweeps on one stream: per-row completion epochs and
-// [0] epoch, [1] ticket, [2] error flag (a dependency wait hit its bound).
+// State of the sync-free sweeps on one stream: ctl[2] is the error flag (a
+// dependency wait reached its bound).
 struct SweepSync {
-  DevBuf flags, ctl;
+  DevBuf ctl;
   int nsup = 0;
   void ensure(int nsup, hipStream_t s);
   int error(hipStream_t s);

@@ -36,7 +36,41 @@ __device__ __forceinline__ int xcd_tile(int bid, int nblk) {
 
 // ---------------------------------------------------------------------------
 // SpMV: y = A x or y = bsub - A x, NR right-hand sides, optional dot(w, y).
-template <int B, int NR, class TA>
+//
+// One wavefront per block row. A block row's blocks are contiguous, column
+// after column, so the row is a stream of ncol = nblk*B columns of B values.
+// Each lane loads V consecutive values of a column with one vector load;
+// L = B/V lanes cover a column and G = 64/L lanes groups walk the columns
+// g, g+G, ... of the stream, so one wave instruction moves G*B values of
+// consecutive memory. The G partial sums of every row element are folded
+// through LDS.
+template <class TA, int V> struct VecOf;
+template <> struct VecOf<double, 1> { using T = double; };
+template <> struct VecOf<double, 2> { typedef double T __attribute__((ext_vector_type(2))); };
+template <> struct VecOf<double, 4> { typedef double T __attribute__((ext_vector_type(4))); };
+template <> struct VecOf<float, 1> { using T = float; };
+template <> struct VecOf<float, 2> { typedef float T __attribute__((ext_vector_type(2))); };
+template <> struct VecOf<float, 4> { typedef float T __attribute__((ext_vector_type(4))); };
+template <> struct VecOf<uint16_t, 1> { using T = uint16_t; };
+template <> struct VecOf<uint16_t, 2> { typedef uint16_t T __attribute__((ext_vector_type(2))); };
+template <> struct VecOf<uint16_t, 4> { typedef uint16_t T __attribute__((ext_vector_type(4))); };
+
+template <int V, class TA>
+__device__ __forceinline__ void load_vec(const TA* p, double (&out)[V]) {
+  using VT = typename VecOf<TA, V>::T;
+  const VT v = __builtin_nontemporal_load(reinterpret_cast<const VT*>(p));
+  if constexpr (V == 1) {
+    out[0] = (double)ldv(&v);
+  } else {
+#pragma unroll
+    for (int k = 0; k < V; ++k) {
+      const TA e = v[k];
+      out[k] = (double)ldv(&e);
+    }
+  }
+}
+
+template <int B, int V, int NR, class TA>
 __global__ __launch_bounds__(256) void k_spmv(int nsup, int n, const int* __restrict__ ia,
                                               const int* __restrict__ ja, const TA* __restrict__ a,
                                               const double* __restrict__ x, double* __restrict__ y,
@@ -44,48 +78,60 @@ __global__ __launch_bounds__(256) void k_spmv(int nsup, int n, const int* __rest
                                               double* partials, unsigned* counter, double* dot_out,
                                               int rows_per_wave, const int* skip) {
   if (skip && *skip) return;
-  constexpr int G = 64 / B;
-  const int lane = threadIdx.x & 63;
-  const int r = lane % B, g = lane / B;
+  constexpr int L = B / V;
+  constexpr int G = 64 / L;
+  __shared__ double red[4][G][NR][B];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int lr = lane % L, g = lane / L;
+  const bool act = g < G;
   const int tile = xcd_tile(blockIdx.x, gridDim.x);
   // wave-uniform by construction; readfirstlane makes the row loop a scalar loop
-  const int wave_global = __builtin_amdgcn_readfirstlane(tile * (blockDim.x >> 6) + (threadIdx.x >> 6));
+  const int wave_global = __builtin_amdgcn_readfirstlane(tile * (blockDim.x >> 6) + wave);
   double dotv = 0.0;
-  for (int q = 0; q < rows_per_wave; ++q) {
-    const int row = wave_global * rows_per_wave + q;
+  for (int q0 = 0; q0 < rows_per_wave; ++q0) {
+    const int row = wave_global * rows_per_wave + q0;
     if (row >= nsup) break;
-    double acc[NR];
+    double acc[NR][V];
 #pragma unroll
-    for (int k = 0; k < NR; ++k) acc[k] = 0.0;
+    for (int q = 0; q < NR; ++q)
+#pragma unroll
+      for (int k = 0; k < V; ++k) acc[q][k] = 0.0;
     const int jb = ia[row], je = ia[row + 1];
-    if (g < G) {
-      const TA* arow = a + (size_t)jb * B * B;
+    if (act) {
+      const TA* ap = a + (size_t)jb * B * B + lr * V;
       const int ncol = (je - jb) * B;
 #pragma unroll 4
       for (int t = g; t < ncol; t += G) {
         const int bk = t / B, c = t - bk * B;
         const int col = ja[jb + bk];
-        const double av = (double)ldnt(arow + (size_t)t * B + r);  // block bk, column c
+        double av[V];
+        load_vec<V>(ap + (size_t)t * B, av);
 #pragma unroll
-        for (int k = 0; k < NR; ++k) acc[k] += av * x[(size_t)k * n + (size_t)col * B + c];
+        for (int q = 0; q < NR; ++q) {
+          const double xv = x[(size_t)q * n + (size_t)col * B + c];
+#pragma unroll
+          for (int k = 0; k < V; ++k) acc[q][k] += av[k] * xv;
+        }
       }
-    }
 #pragma unroll
-    for (int k = 0; k < NR; ++k) {
-      double s = acc[k];
+      for (int q = 0; q < NR; ++q)
 #pragma unroll
-      for (int h = 1; h < G; ++h) s += __shfl(acc[k], lane + h * B);
-      acc[k] = s;
+        for (int k = 0; k < V; ++k) red[wave][g][q][lr * V + k] = acc[q][k];
     }
+    wave_lds_sync();
     if (lane < B) {
       const size_t e = (size_t)row * B + lane;
 #pragma unroll
-      for (int k = 0; k < NR; ++k) {
-        const double v = bsub ? bsub[(size_t)k * n + e] - acc[k] : acc[k];
-        y[(size_t)k * n + e] = v;
-        if (k == 0 && wdot) dotv += wdot[e] * v;
+      for (int q = 0; q < NR; ++q) {
+        double s = 0.0;
+#pragma unroll
+        for (int h = 0; h < G; ++h) s += red[wave][h][q][lane];
+        const double v = bsub ? bsub[(size_t)q * n + e] - s : s;
+        y[(size_t)q * n + e] = v;
+        if (q == 0 && wdot) dotv += wdot[e] * v;
       }
     }
+    wave_lds_sync();
   }
   if (dot_out) {
     double vv[1] = {dotv};
@@ -183,100 +229,138 @@ __global__ __launch_bounds__(256) void k_sweep_level(int nrows, const int* __res
 // rows w, w+W, w+2W, ... of the sweep order and processes them in that order;
 // every wave of the grid is resident (cooperative launch), so the smallest
 // unfinished row always belongs to a wave whose dependencies are complete and
-// the sweep cannot deadlock. A wave first loads its row's factor blocks into
-// registers, then waits (bounded) on the completion epochs of the rows it
-// depends on, then publishes its row.
+// the sweep cannot deadlock.
 //
-// Cross-wave ordering follows the device-scope protocol for gfx942/gfx950
-// (as rocPRIM's look-back scan does): vector values and flags move through
-// device-scope relaxed atomic loads/stores, which are coherent across XCDs,
-// and workgroup-scope fences plus vmcnt waits only order them, with no L2
-// write-back or invalidation per row.
+// The output vector is its own completion record: before the sweep it is
+// filled with a sentinel NaN pattern that no arithmetic produces (a signaling
+// NaN with a fixed payload); a row's values are final once they differ from
+// it. A wave loads its row's factor blocks into registers, then loads the
+// dependency values it needs and re-polls (bounded) those still holding the
+// sentinel, computes, and stores its row. Values cross XCDs through
+// device-scope relaxed atomic loads/stores (coherent across the 8 L2s, the
+// protocol rocPRIM's look-back scan uses on gfx942/gfx950); there is no
+// separate flag and no fence on the critical path. A wait that exceeds its
+// bound raises ctl[2], which stops every other wait; the host reports it.
 //
-// The row loop is wave-uniform (its bounds are blockIdx/gridDim scalars);
-// every lane executes the publishing store.
-__global__ void k_sweep_arm(int* ctl) {
-  ctl[0] += 1;
-  ctl[1] = 0;
+// The row loop is wave-uniform (blockIdx/gridDim scalars); lane-divergent
+// code inside holds no cross-lane operation.
+template <class T> struct Sentinel;
+template <> struct Sentinel<double> {
+  using Bits = unsigned long long;
+  static constexpr Bits bits = 0x7FF4DEADC0DE0001ull;
+};
+template <> struct Sentinel<float> {
+  using Bits = unsigned int;
+  static constexpr Bits bits = 0x7FA0DEADu;
+};
+
+template <class TX>
+__device__ __forceinline__ bool is_sentinel(TX v) {
+  using Bits = typename Sentinel<TX>::Bits;
+  return __builtin_bit_cast(Bits, v) == Sentinel<TX>::bits;
 }
 
-__device__ __forceinline__ void release_order_only() {
-  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
-  __builtin_amdgcn_s_waitcnt(0 | (0x7 << 4) | (0xf << 8));  // vmcnt(0)
-}
-
-__device__ __forceinline__ void acquire_order_only() {
-  __builtin_amdgcn_s_waitcnt(0 | (0x7 << 4) | (0xf << 8));
-  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+template <class TX>
+__global__ void k_sweep_fill(int64_t n, TX* __restrict__ y, const int* skip) {
+  if (skip && *skip) return;
+  using Bits = typename Sentinel<TX>::Bits;
+  Bits* yb = reinterpret_cast<Bits*>(y);
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) yb[i] = Sentinel<TX>::bits;
 }
 
 template <int B, int NR, class TA, class TX>
 __global__ __launch_bounds__(64) void k_sweep_free(int nsup, int n, int forward, const int* __restrict__ ia,
                                                    const int* __restrict__ ja, const TA* __restrict__ off,
                                                    const TA* __restrict__ diag, const TX* __restrict__ x, TX* y,
-                                                   int* flags, int* ctl, const int* skip) {
+                                                   int* ctl, const int* skip) {
   if (skip && *skip) return;
   using Acc = typename AccOf<TX>::type;
   constexpr int G = 64 / B;
-  constexpr int PF = (3 * B + G - 1) / G;   // off-diagonal values prefetched per lane (3 blocks)
+  // off-diagonal values prefetched per lane: three blocks (one per extra right-hand side set)
+  constexpr int PF = ((NR == 1 ? 3 : 1) * B + G - 1) / G;
   constexpr int PD = (B + G - 1) / G;       // diagonal-block columns per lane
   __shared__ Acc v[NR][B];
   const int lane = threadIdx.x;
   const int r = lane % B, g = lane / B;
   const bool act = g < G;
-  const int epoch = __builtin_amdgcn_readfirstlane(ctl[0]);
   for (int t = blockIdx.x; t < nsup; t += gridDim.x) {
     const int row = forward ? t : nsup - 1 - t;
     const int jb = ia[row], je = ia[row + 1];
     const int ncol = (je - jb) * B;
     const TA* blk0 = off + (size_t)jb * B * B;
     const TA* d = diag + (size_t)row * B * B;
-    // factor values of this row do not depend on the wait: load them first
-    Acc pf[PF], pd[PD];
+    // factor values of this row do not depend on the wait: load them first;
     // column tt of the row's block stream (block tt/B, column tt%B) starts at blk0 + tt*B
+    Acc pf[PF], pd[PD];
     const TA* lp = blk0 + g * B + r;
 #pragma unroll
     for (int i = 0; i < PF; ++i) pf[i] = (act && g + i * G < ncol) ? (Acc)ldnt(lp + i * G * B) : Acc(0);
 #pragma unroll
     for (int i = 0; i < PD; ++i) pd[i] = (act && g + i * G < B) ? (Acc)ldnt(d + (g + i * G) * B + r) : Acc(0);
-    // wait until every row this one depends on has been published
-    for (int base = jb; base < je; base += 64) {
-      const int j = base + lane;
-      if (j < je) {
-        const int k = ja[j];
-        int spins = 0;
-        while (__hip_atomic_load(&flags[k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != epoch) {
-          __builtin_amdgcn_s_sleep(1);
-          ++spins;
-          if (spins > kSpinLimit ||
-              ((spins & 1023) == 0 && __hip_atomic_load(&ctl[2], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))) {
-            __hip_atomic_store(&ctl[2], 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            break;
-          }
-        }
-      }
-    }
-    acquire_order_only();
     Acc acc[NR];
 #pragma unroll
     for (int q = 0; q < NR; ++q) acc[q] = Acc(0);
     if (act) {
+      int spins = 0;
+      // issue every dependency load at once, then re-poll only the values still
+      // holding the sentinel (one round trip when the dependencies are done)
+      int src[PF];  // element offsets in y (-1: column beyond the row)
+      TX yv[PF][NR];
 #pragma unroll
       for (int i = 0; i < PF; ++i) {
         const int tt = g + i * G;
-        if (tt < ncol) {
-          const int bk = tt / B, c = tt - bk * B;
-          const int k = ja[jb + bk];
+        const int bk = tt / B, c = tt - bk * B;
+        src[i] = tt < ncol ? ja[jb + bk] * B + c : -1;
+      }
+#pragma unroll
+      for (int i = 0; i < PF; ++i)
+#pragma unroll
+        for (int q = 0; q < NR; ++q) yv[i][q] = src[i] >= 0 ? ld_coherent(y + src[i] + (size_t)q * n) : TX(0);
+      bool miss = false;
+#pragma unroll
+      for (int i = 0; i < PF; ++i)
+#pragma unroll
+        for (int q = 0; q < NR; ++q) miss |= is_sentinel(yv[i][q]);
+      while (miss) {  // dependencies not all published yet
+        __builtin_amdgcn_s_sleep(1);
+        miss = false;
+#pragma unroll
+        for (int i = 0; i < PF; ++i)
 #pragma unroll
           for (int q = 0; q < NR; ++q)
-            acc[q] += pf[i] * (Acc)ld_coherent(&y[(size_t)q * n + (size_t)k * B + c]);
+            if (is_sentinel(yv[i][q])) {
+              yv[i][q] = ld_coherent(y + src[i] + (size_t)q * n);
+              miss |= is_sentinel(yv[i][q]);
+            }
+        if (++spins > kSpinLimit ||
+            ((spins & 1023) == 0 && __hip_atomic_load(&ctl[2], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))) {
+          __hip_atomic_store(&ctl[2], 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+          break;
         }
       }
-      for (int tt = g + PF * G; tt < ncol; tt += G) {  // rows with more than three dependencies
+#pragma unroll
+      for (int i = 0; i < PF; ++i)
+#pragma unroll
+        for (int q = 0; q < NR; ++q) acc[q] += pf[i] * (Acc)yv[i][q];
+      for (int tt = g + PF * G; tt < ncol; tt += G) {  // dependencies beyond the prefetched ones
         const int bk = tt / B, c = tt - bk * B;
         const int k = ja[jb + bk];
         const Acc av = (Acc)ldnt(blk0 + (size_t)tt * B + r);
-        for (int q = 0; q < NR; ++q) acc[q] += av * (Acc)ld_coherent(&y[(size_t)q * n + (size_t)k * B + c]);
+        for (int q = 0; q < NR; ++q) {
+          const TX* src = &y[(size_t)q * n + (size_t)k * B + c];
+          TX yv = ld_coherent(src);
+          while (is_sentinel(yv)) {
+            __builtin_amdgcn_s_sleep(1);
+            yv = ld_coherent(src);
+            if (++spins > kSpinLimit || ((spins & 1023) == 0 &&
+                __hip_atomic_load(&ctl[2], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))) {
+              __hip_atomic_store(&ctl[2], 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+              break;
+            }
+          }
+          acc[q] += av * (Acc)yv;
+        }
       }
     }
 #pragma unroll
@@ -308,8 +392,6 @@ __global__ __launch_bounds__(64) void k_sweep_free(int nsup, int n, int forward,
       if (lane < B) st_coherent(&y[(size_t)q * n + (size_t)row * B + lane], (TX)s);
     }
     __syncthreads();
-    release_order_only();
-    __hip_atomic_store(&flags[row], epoch, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   }
 }
 
@@ -513,11 +595,8 @@ void dev_tri_forward(const BlockCsr& f, int prec, DevTri& t) { build_tri(f, true
 void dev_tri_backward(const BlockCsr& f, int prec, DevTri& t) { build_tri(f, false, prec, t); }
 
 void SweepSync::ensure(int n, hipStream_t s) {
-  if (n <= nsup && flags.p) return;
-  flags.alloc((size_t)std::max(n, 1) * 4);
-  if (!ctl.p) ctl.alloc(64);
-  // zero flags and epoch on the sweep's own stream, then wait: no stale epoch can match
-  BCM_HIP(hipMemsetAsync(flags.p, 0, flags.bytes, s));
+  if (ctl.p) return;
+  ctl.alloc(64);
   BCM_HIP(hipMemsetAsync(ctl.p, 0, 64, s));
   BCM_HIP(hipStreamSynchronize(s));
   nsup = n;
@@ -544,6 +623,13 @@ void RedWork::ensure() {
 // ---------------------------------------------------------------------------
 // Launch wrappers
 
+// Vector width of the SpMV loads: four values where the block size allows
+// (two for six right-hand sides, to bound the LDS of the reduction).
+template <int B, int NR>
+constexpr int spmv_width() {
+  return (B % 4 == 0 && NR <= 3) ? 4 : (B % 2 == 0 ? 2 : 1);
+}
+
 template <int B, int NR, class TA>
 static void spmv_b(const DevBsr& A, const double* x, double* y, const double* bsub, double* dot_out,
                    const double* wdot, RedWork& rw, hipStream_t s, const int* skip) {
@@ -554,7 +640,7 @@ static void spmv_b(const DevBsr& A, const double* x, double* y, const double* bs
   int grid = (nwaves + kWavesPerBlock - 1) / kWavesPerBlock;
   if (grid > 8) grid = (grid + 7) / 8 * 8;  // whole XCD rounds, so the tile map is a permutation
   if (dot_out) rw.ensure();
-  hipLaunchKernelGGL((k_spmv<B, NR, TA>), dim3(grid), dim3(64 * kWavesPerBlock), 0, s, A.nsup, A.n,
+  hipLaunchKernelGGL((k_spmv<B, spmv_width<B, NR>(), NR, TA>), dim3(grid), dim3(64 * kWavesPerBlock), 0, s, A.nsup, A.n,
                      A.ia.as<int>(), A.ja.as<int>(), A.a.as<TA>(), x, y, bsub, wdot,
                      rw.partials.as<double>(), rw.counter.as<unsigned>(), dot_out, rows_per_wave, skip);
   BCM_HIP(hipGetLastError());
@@ -606,17 +692,18 @@ static void sweep_b(const DevTri& T, const void* x, void* y, hipStream_t s, cons
     int per_cu = 0;
     BCM_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, 64, 0));
     const int waves = std::max(1, std::min(T.nsup, std::min(per_cu, 8) * device_cu_count()));
-    hipLaunchKernelGGL(k_sweep_arm, dim3(1), dim3(1), 0, s, sync->ctl.as<int>());
+    const int64_t ny = (int64_t)T.n * NR;
+    TX* yy = static_cast<TX*>(y);
+    hipLaunchKernelGGL((k_sweep_fill<TX>), dim3(std::max<int64_t>(1, std::min<int64_t>((ny + 255) / 256, 2048))),
+                       dim3(256), 0, s, ny, yy, skip);
     int nsup = T.nsup, n = T.n, fwd = T.forward ? 1 : 0;
     const int* ia = T.ia.as<int>();
     const int* ja = T.ja.as<int>();
     const TA* off = T.off.as<TA>();
     const TA* dg = T.diag.as<TA>();
     const TX* xx = static_cast<const TX*>(x);
-    TX* yy = static_cast<TX*>(y);
-    int* fl = sync->flags.as<int>();
     int* ct = sync->ctl.as<int>();
-    void* args[] = {&nsup, &n, &fwd, &ia, &ja, &off, &dg, &xx, &yy, &fl, &ct, (void*)&skip};
+    void* args[] = {&nsup, &n, &fwd, &ia, &ja, &off, &dg, &xx, &yy, &ct, (void*)&skip};
     BCM_HIP(hipLaunchCooperativeKernel(reinterpret_cast<const void*>(kern), dim3(waves), dim3(64), args, 0, s));
     return;
   }
