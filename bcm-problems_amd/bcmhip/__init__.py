@@ -26,7 +26,7 @@ LIB_PATH = os.path.join(os.path.dirname(_HERE), "lib", "libbcmhip.so")
 
 OK = 0
 E_ARG, E_NOT_SPD, E_WORKSPACE, E_BREAKDOWN, E_HIP, E_NOCONV, E_NODEVICE, E_INTERNAL = range(1, 9)
-PCG, BICGSTAB, FGMRES, LANCZOS = 0, 1, 2, 3
+PCG, BICGSTAB, FGMRES, LANCZOS, BLOCKPCG = 0, 1, 2, 3, 5
 F64, F32, BF16 = 0, 1, 2
 
 
@@ -68,6 +68,20 @@ class Stats(C.Structure):
     _fields_ = [
         ("iterations", C.c_int), ("restarts", C.c_int), ("breakdown", C.c_int), ("converged", C.c_int),
         ("relres_true", C.c_double), ("relres_rec", C.c_double), ("seconds", C.c_double),
+    ]
+
+
+class BatchStats(C.Structure):
+    _fields_ = [
+        ("pairs", C.c_int), ("converged", C.c_int), ("breakdowns", C.c_int), ("iters_max", C.c_int),
+        ("restarts", C.c_int), ("iters_sum", C.c_longlong), ("relres_max", C.c_double), ("seconds", C.c_double),
+    ]
+
+
+class BatchTune(C.Structure):
+    _fields_ = [
+        ("sweep_mode", C.c_int), ("waves_per_cu", C.c_int), ("graph", C.c_int), ("check_every", C.c_int),
+        ("ms_per_iter", C.c_double),
     ]
 
 
@@ -122,6 +136,14 @@ SIGNATURES = {
     "bcm_precond": (C.c_int, [_P, _P, C.c_void_p, C.c_void_p, C.c_int, C.c_int, C.c_int]),
     "bcm_dot": (C.c_int, [_P, C.c_int, C.c_void_p, C.c_void_p, _DP]),
     "bcm_solve": (C.c_int, [_P, _P, _P, C.POINTER(SolveOpts), C.c_void_p, C.c_void_p, C.POINTER(Stats)]),
+    "bcm_batch_create": (C.c_int, [C.c_int, C.POINTER(_P), C.POINTER(_P), C.c_int, C.POINTER(_P)]),
+    "bcm_batch_destroy": (C.c_int, [_P]),
+    "bcm_batch_layout": (C.c_int, [_P, _IP, C.POINTER(C.c_longlong), C.POINTER(C.c_longlong)]),
+    "bcm_batch_solve": (C.c_int, [_P, _P, C.POINTER(SolveOpts), C.c_int, C.c_void_p, C.c_void_p, C.c_int,
+                                  C.POINTER(BatchStats), _IP, _DP]),
+    "bcm_batch_autotune": (C.c_int, [_P, _P, C.POINTER(SolveOpts), C.c_int, C.c_void_p, C.POINTER(BatchTune)]),
+    "bcm_batch_get_tune": (C.c_int, [_P, C.POINTER(BatchTune)]),
+    "bcm_batch_set_tune": (C.c_int, [_P, C.POINTER(BatchTune)]),
     "bcm_abstract_symm_solver": (C.c_int, [C.c_void_p, C.POINTER(Params), _CB_NSUP, _CB_SPRNDS, _CB_IAJA,
                                            _CB_BLOCK, _CB_BLOCK, _CB_BLOCK, _CB_BLOCK]),
     "bcm_abstract_solver": (C.c_int, [C.c_void_p, C.POINTER(Params), _CB_NSUP, _CB_SPRNDS, _CB_IAJA,
@@ -426,3 +448,68 @@ def device_count() -> int:
     n = C.c_int()
     _check(lib().bcm_device_count(C.byref(n)))
     return n.value
+
+
+class Batch:
+    """Block-diagonal union of cells (matrices + IC2 factors) on the device.
+
+    Vectors of a batch are n x nrhs (column-major); column q holds every
+    cell's q-th right-hand side at the cell's element offset (``elem0``).
+    """
+
+    def __init__(self, cells, prec_mask: int = 0b011):
+        self.cells = list(cells)  # keeps the handles alive while the batch exists
+        n = len(self.cells)
+        mats = (_P * n)(*[m.h for m, _ in self.cells])
+        facts = (_P * n)(*[f.h for _, f in self.cells])
+        out = C.c_void_p()
+        _check(lib().bcm_batch_create(n, mats, facts, prec_mask, C.byref(out)))
+        self.h = out.value
+        nc = C.c_int()
+        nn = C.c_longlong()
+        e0 = np.zeros(n + 1, np.int64)
+        _check(lib().bcm_batch_layout(self.h, C.byref(nc), C.byref(nn), e0.ctypes.data_as(C.POINTER(C.c_longlong))))
+        self.ncell, self.n, self.elem0 = nc.value, nn.value, e0
+
+    def gather(self, per_cell):
+        """Union layout (n x nrhs, column-major) from per-cell n_c x nrhs arrays."""
+        nrhs = np.asarray(per_cell[0]).reshape(len(per_cell[0]), -1).shape[1]
+        out = np.zeros((self.n, nrhs), order="F")
+        for c, v in enumerate(per_cell):
+            out[self.elem0[c]:self.elem0[c + 1], :] = np.asarray(v).reshape(-1, nrhs, order="F")
+        return out
+
+    def scatter(self, union):
+        """Per-cell n_c x nrhs arrays from a union-layout array."""
+        u = np.asarray(union).reshape(self.n, -1, order="F")
+        return [u[self.elem0[c]:self.elem0[c + 1], :] for c in range(self.ncell)]
+
+    def solve(self, ctx: "Context", b, x, opts: SolveOpts, nrhs: int, fallback: bool = False):
+        npair = self.ncell * nrhs
+        iters = np.zeros(npair, np.int32)
+        relres = np.zeros(npair)
+        st = BatchStats()
+        _check(lib().bcm_batch_solve(ctx.h, self.h, C.byref(opts), nrhs, ptr(b), ptr(x), int(fallback), C.byref(st),
+                                     _ip(iters), _dp(relres)))
+        return st, iters.reshape(self.ncell, nrhs), relres.reshape(self.ncell, nrhs)
+
+    def autotune(self, ctx: "Context", b, opts: SolveOpts, nrhs: int) -> BatchTune:
+        t = BatchTune()
+        _check(lib().bcm_batch_autotune(ctx.h, self.h, C.byref(opts), nrhs, ptr(b), C.byref(t)))
+        return t
+
+    def get_tune(self) -> BatchTune:
+        t = BatchTune()
+        _check(lib().bcm_batch_get_tune(self.h, C.byref(t)))
+        return t
+
+    def set_tune(self, **kw) -> None:
+        t = self.get_tune()
+        for k, v in kw.items():
+            setattr(t, k, v)
+        _check(lib().bcm_batch_set_tune(self.h, C.byref(t)))
+
+    def __del__(self):
+        if getattr(self, "h", None) and _lib is not None:
+            _lib.bcm_batch_destroy(self.h)
+            self.h = None

@@ -8,6 +8,7 @@
 #include <chrono>
 #include <cstdlib>
 #include <cstring>
+#include <memory>
 #include <new>
 #include <string>
 
@@ -23,6 +24,9 @@ struct bcm_factor_s {
 };
 struct bcm_context_s {
   bcm::Context c;
+};
+struct bcm_batch_s {
+  bcm::Batch b;
 };
 
 namespace {
@@ -545,8 +549,8 @@ int bcm_solve_l(bcm_context c, bcm_factor f, int which, const double* x, double*
     need(c && f && x && y && nrhs > 0 && (which == 0 || which == 1), "solve_l: bad argument");
     need(x != y, "solve_l: x and y must not alias");
     const bool t = which == 1 && f->f.kind == 1;
-    f->f.to_device(bcm::kF64, t);
-    bcm::launch_sweep(t ? f->f.fwdt : f->f.fwd, x, y, bcm::kF64, nrhs, c->c.stream, nullptr, c->c.sweep_sync());
+    const auto& im = f->f.to_device(bcm::kF64, t);
+    bcm::launch_sweep(t ? im.fwdt : im.fwd, x, y, bcm::kF64, nrhs, c->c.stream, nullptr, c->c.sweep_sync());
   });
 }
 
@@ -555,8 +559,8 @@ int bcm_solve_u(bcm_context c, bcm_factor f, int which, const double* x, double*
     need(c && f && x && y && nrhs > 0 && (which == 0 || which == 1), "solve_u: bad argument");
     need(x != y, "solve_u: x and y must not alias");
     const bool t = which == 0 && f->f.kind == 1;
-    f->f.to_device(bcm::kF64, t);
-    bcm::launch_sweep(t ? f->f.bwdt : f->f.bwd, x, y, bcm::kF64, nrhs, c->c.stream, nullptr, c->c.sweep_sync());
+    const auto& im = f->f.to_device(bcm::kF64, t);
+    bcm::launch_sweep(t ? im.bwdt : im.bwd, x, y, bcm::kF64, nrhs, c->c.stream, nullptr, c->c.sweep_sync());
   });
 }
 
@@ -589,6 +593,100 @@ int bcm_solve(bcm_context c, bcm_matrix a, bcm_factor f, const bcm_solve_opts* o
     bcm::solve(a->m, f->f, c->c, opt, b, x, s);
     fill_stats(s, st);
     if (s.breakdown) throw bcm::Error(bcm::kErrBreakdown, "Krylov breakdown");
+  });
+}
+
+int bcm_batch_create(int ncell, const bcm_matrix* mats, const bcm_factor* facts, int prec_mask, bcm_batch* out) {
+  return guarded([&] {
+    need(ncell > 0 && mats && facts && out, "batch_create: bad argument");
+    need((prec_mask & 7) != 0, "batch_create: prec_mask selects no precision");
+    need_device();
+    std::vector<const bcm::Matrix*> m(ncell);
+    std::vector<const bcm::Factor*> f(ncell);
+    for (int i = 0; i < ncell; ++i) {
+      need(mats[i] && facts[i], "batch_create: null handle");
+      m[i] = &mats[i]->m;
+      f[i] = &facts[i]->f;
+    }
+    std::unique_ptr<bcm_batch_s> b(new bcm_batch_s());
+    bcm::batch_build(b->b, m, f, prec_mask & 7);
+    *out = b.release();
+  });
+}
+
+int bcm_batch_destroy(bcm_batch b) {
+  delete b;
+  return BCM_OK;
+}
+
+int bcm_batch_layout(bcm_batch b, int* ncell, long long* n, long long* elem0) {
+  return guarded([&] {
+    need(b != nullptr, "batch_layout: bad argument");
+    if (ncell) *ncell = b->b.ncell;
+    if (n) *n = b->b.n;
+    if (elem0)
+      for (int i = 0; i <= b->b.ncell; ++i) elem0[i] = b->b.elem0[i];
+  });
+}
+
+int bcm_batch_solve(bcm_context c, bcm_batch b, const bcm_solve_opts* o, int nrhs, const double* rhs, double* x,
+                    int fallback, bcm_batch_stats* st, int* iters, double* relres) {
+  return guarded([&] {
+    need(c && b && rhs && x, "batch_solve: bad argument");
+    bcm::SolveOptions opt = to_opts(o);
+    if (opt.kind != bcm::kPcg) opt.kind = bcm::kBlockPcg;
+    bcm::BatchStats s;
+    bcm::batch_solve(b->b, c->c, opt, nrhs, rhs, x, fallback != 0, s, iters, relres);
+    if (st) {
+      st->pairs = s.pairs;
+      st->converged = s.converged;
+      st->breakdowns = s.breakdowns;
+      st->iters_max = s.iters_max;
+      st->restarts = s.restarts;
+      st->iters_sum = s.iters_sum;
+      st->relres_max = s.relres_max;
+      st->seconds = s.seconds;
+    }
+  });
+}
+
+static void tune_out(const bcm::BatchTune& t, bcm_batch_tune* out) {
+  out->sweep_mode = t.sweep_mode;
+  out->waves_per_cu = t.waves_per_cu;
+  out->graph = t.graph;
+  out->check_every = t.check_every;
+  out->ms_per_iter = t.ms_per_iter;
+}
+
+int bcm_batch_autotune(bcm_context c, bcm_batch b, const bcm_solve_opts* o, int nrhs, const double* rhs,
+                       bcm_batch_tune* out) {
+  return guarded([&] {
+    need(c && b && rhs, "batch_autotune: bad argument");
+    bcm::SolveOptions opt = to_opts(o);
+    opt.kind = bcm::kBlockPcg;
+    bcm::BatchTune t = bcm::batch_tune(b->b, c->c, opt, nrhs, rhs);
+    if (out) tune_out(t, out);
+  });
+}
+
+int bcm_batch_get_tune(bcm_batch b, bcm_batch_tune* t) {
+  return guarded([&] {
+    need(b && t, "batch_get_tune: bad argument");
+    tune_out(b->b.tune, t);
+  });
+}
+
+int bcm_batch_set_tune(bcm_batch b, const bcm_batch_tune* t) {
+  return guarded([&] {
+    need(b && t, "batch_set_tune: bad argument");
+    need(t->sweep_mode == 0 || t->sweep_mode == 1, "batch_set_tune: sweep_mode is 0 or 1");
+    need(t->waves_per_cu >= 1 && t->waves_per_cu <= 32, "batch_set_tune: waves_per_cu in 1..32");
+    need(t->check_every >= 1 && t->check_every <= 1024, "batch_set_tune: check_every in 1..1024");
+    b->b.tune.sweep_mode = t->sweep_mode;
+    b->b.tune.waves_per_cu = t->waves_per_cu;
+    b->b.tune.graph = t->graph ? 1 : 0;
+    b->b.tune.check_every = t->check_every;
+    b->b.drop_graph();
   });
 }
 

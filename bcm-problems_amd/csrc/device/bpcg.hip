@@ -1,0 +1,628 @@
+// Batched preconditioned CG over many independent cells and right-hand sides.
+//
+// The cells' matrices and IC2 factors are uploaded as one block-diagonal
+// union, so a single SpMV launch (nrhs columns) and a single pair of
+// triangular sweeps serve every cell: dependency chains of different cells
+// are independent and the sync-free sweep interleaves them. Each
+// (cell, right-hand side) pair runs the reference's preconditioned CG
+// recurrence (cg.cpp:137-289) with its own scalars; a pair that meets its
+// tolerance is frozen (its x, r, p no longer change) and the launch sequence
+// ends when no pair is active.
+//
+// Reductions are segmented by "chunks" -- runs of at most kChunk elements
+// inside one cell: a chunk kernel writes one partial per (chunk, column,
+// value), and one thread per pair adds its cell's chunks in a fixed order.
+//
+// With an FP32/BF16 factor the preconditioner is an inexact operator, so the
+// recurrence uses the flexible (Polak-Ribiere) beta = (z, r - r_old) /
+// (z_old, r_old), which keeps CG convergent to the FP64 tolerance.
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+
+#include "dev_common.hpp"
+#include "solver.hpp"
+
+namespace bcm {
+
+namespace {
+
+constexpr int kChunk = 4096;
+constexpr int kPairThreads = 1024;
+
+// Per-pair device state (struct of arrays, P = ncell * nrhs entries).
+struct BPtrs {
+  double *gamma, *alpha, *beta, *rr0, *bb, *pq, *relres;
+  int *active, *iters, *fresh, *flag, *conv;
+  int* state;  // [0] done (no active pair), [1] breakdowns
+};
+
+struct VecSet {
+  const double* x[3];
+  const double* y[3];
+};
+
+__device__ __forceinline__ double pair_thr2(const BPtrs& s, int p, int ittype, double eps2) {
+  if (ittype == 0) return eps2 * s.rr0[p];
+  if (ittype == 1) return eps2;
+  return eps2 * s.bb[p];
+}
+
+// Sum over the chunks of cell c of value k of column q (partials laid out
+// [chunk][q][k] with nrhs columns and K values).
+__device__ __forceinline__ double chunk_sum(const double* partials, const int* cell_chunk0, int c, int q, int nrhs,
+                                            int K, int k) {
+  double s = 0.0;
+  for (int ch = cell_chunk0[c]; ch < cell_chunk0[c + 1]; ++ch) s += partials[((size_t)ch * nrhs + q) * K + k];
+  return s;
+}
+
+// Partial dots of K vector pairs over one chunk, for NR columns.
+template <int NR, int K>
+__global__ __launch_bounds__(256) void k_bp_dots(int64_t n, const int* __restrict__ lo, const int* __restrict__ hi,
+                                                 VecSet v, double* partials, const int* skip) {
+  if (skip && *skip) return;
+  double acc[NR * K];
+#pragma unroll
+  for (int j = 0; j < NR * K; ++j) acc[j] = 0.0;
+  const int a = lo[blockIdx.x], e = hi[blockIdx.x];
+#pragma unroll
+  for (int q = 0; q < NR; ++q) {
+    const size_t off = (size_t)q * n;
+    for (int i = a + threadIdx.x; i < e; i += blockDim.x) {
+#pragma unroll
+      for (int k = 0; k < K; ++k) acc[q * K + k] += v.x[k][off + i] * v.y[k][off + i];
+    }
+  }
+  block_partials<NR * K>(acc, partials);
+}
+
+// Start of a solve: partials hold (r.r, b.b) per pair. Optional fallback to
+// x0 = 0 for pairs whose initial residual is not below ||b||.
+__global__ __launch_bounds__(kPairThreads) void k_bp_start(int npair, int nrhs, const int* cell_chunk0,
+                                                           const double* partials, BPtrs s, double eps2, int ittype,
+                                                           int fallback) {
+  int any = 0;
+  for (int p = threadIdx.x; p < npair; p += blockDim.x) {
+    const int c = p / nrhs, q = p - c * nrhs;
+    double rr = chunk_sum(partials, cell_chunk0, c, q, nrhs, 2, 0);
+    const double bb = chunk_sum(partials, cell_chunk0, c, q, nrhs, 2, 1);
+    s.flag[p] = 0;
+    if (fallback && !(rr < bb)) {  // the guess is no better than zero
+      s.flag[p] = 1;
+      rr = bb;
+    }
+    s.bb[p] = bb;
+    s.rr0[p] = rr;
+    s.iters[p] = 0;
+    s.fresh[p] = 1;
+    s.conv[p] = 0;
+    s.gamma[p] = 0.0;
+    s.alpha[p] = 0.0;
+    s.beta[p] = 0.0;
+    const int act = rr > pair_thr2(s, p, ittype, eps2) ? 1 : 0;
+    s.active[p] = act;
+    if (!act) s.conv[p] = 1;
+    any |= act;
+  }
+  any = __syncthreads_or(any);
+  if (threadIdx.x == 0) {
+    s.state[0] = any ? 0 : 1;
+    s.state[1] = 0;
+  }
+}
+
+// x = 0, r = b for the pairs flagged by k_bp_start.
+template <int NR>
+__global__ __launch_bounds__(256) void k_bp_zero(int64_t n, const int* __restrict__ ccell, const int* __restrict__ lo,
+                                                 const int* __restrict__ hi, BPtrs s, const double* __restrict__ b,
+                                                 double* __restrict__ x, double* __restrict__ r) {
+  const int c = ccell[blockIdx.x], a = lo[blockIdx.x], e = hi[blockIdx.x];
+#pragma unroll
+  for (int q = 0; q < NR; ++q) {
+    if (!s.flag[c * NR + q]) continue;
+    const size_t off = (size_t)q * n;
+    for (int i = a + threadIdx.x; i < e; i += blockDim.x) {
+      x[off + i] = 0.0;
+      r[off + i] = b[off + i];
+    }
+  }
+}
+
+// Convergence test and beta. Partials hold (r.z, r.r[, r_old.z]) per pair.
+__global__ __launch_bounds__(kPairThreads) void k_bp_beta(int npair, int nrhs, const int* cell_chunk0,
+                                                          const double* partials, int K, BPtrs s, double eps2,
+                                                          int ittype, int niter) {
+  if (s.state[0]) return;
+  int any = 0;
+  for (int p = threadIdx.x; p < npair; p += blockDim.x) {
+    if (!s.active[p]) {
+      s.beta[p] = 0.0;
+      continue;
+    }
+    const int c = p / nrhs, q = p - c * nrhs;
+    const double rz = chunk_sum(partials, cell_chunk0, c, q, nrhs, K, 0);
+    const double rr = chunk_sum(partials, cell_chunk0, c, q, nrhs, K, 1);
+    if (!(rr > pair_thr2(s, p, ittype, eps2))) {  // converged
+      s.active[p] = 0;
+      s.conv[p] = 1;
+      continue;
+    }
+    if (s.iters[p] >= niter || !(rz > 0.0)) {  // budget spent, or M^-1 not positive definite
+      if (!(rz > 0.0)) atomicAdd(&s.state[1], 1);
+      s.active[p] = 0;
+      continue;
+    }
+    double beta = 0.0;
+    if (!s.fresh[p]) {
+      const double num = K == 3 ? rz - chunk_sum(partials, cell_chunk0, c, q, nrhs, K, 2) : rz;
+      beta = num / s.gamma[p];
+    }
+    s.fresh[p] = 0;
+    s.beta[p] = beta;
+    s.gamma[p] = rz;
+    any = 1;
+  }
+  any = __syncthreads_or(any);
+  if (threadIdx.x == 0 && !any) s.state[0] = 1;
+}
+
+// p = z + beta p for active pairs.
+template <int NR>
+__global__ __launch_bounds__(256) void k_bp_p(int64_t n, const int* __restrict__ ccell, const int* __restrict__ lo,
+                                              const int* __restrict__ hi, BPtrs s, const double* __restrict__ z,
+                                              double* __restrict__ pv) {
+  if (s.state[0]) return;
+  const int c = ccell[blockIdx.x], a = lo[blockIdx.x], e = hi[blockIdx.x];
+#pragma unroll
+  for (int q = 0; q < NR; ++q) {
+    const int pr = c * NR + q;
+    if (!s.active[pr]) continue;
+    const double beta = s.beta[pr];
+    const size_t off = (size_t)q * n;
+    for (int i = a + threadIdx.x; i < e; i += blockDim.x) pv[off + i] = pv[off + i] * beta + 1.0 * z[off + i];
+  }
+}
+
+// alpha = gamma / (p, A p); partials hold (p.q) per pair.
+__global__ __launch_bounds__(kPairThreads) void k_bp_alpha(int npair, int nrhs, const int* cell_chunk0,
+                                                           const double* partials, BPtrs s) {
+  if (s.state[0]) return;
+  for (int p = threadIdx.x; p < npair; p += blockDim.x) {
+    if (!s.active[p]) continue;
+    const int c = p / nrhs, q = p - c * nrhs;
+    const double pq = chunk_sum(partials, cell_chunk0, c, q, nrhs, 1, 0);
+    const double alpha = s.gamma[p] / pq;
+    if (!(alpha >= 0.0) || !isfinite(alpha)) {  // "Matrix is not positive definite"
+      atomicAdd(&s.state[1], 1);
+      s.active[p] = 0;
+      s.alpha[p] = 0.0;
+      continue;
+    }
+    s.pq[p] = pq;
+    s.alpha[p] = alpha;
+    s.iters[p] += 1;
+  }
+}
+
+// x += alpha p; (r_old = r;) r -= alpha q for active pairs.
+template <int NR, bool FLEX>
+__global__ __launch_bounds__(256) void k_bp_xr(int64_t n, const int* __restrict__ ccell, const int* __restrict__ lo,
+                                               const int* __restrict__ hi, BPtrs s, const double* __restrict__ pv,
+                                               const double* __restrict__ qv, double* __restrict__ x,
+                                               double* __restrict__ r, double* __restrict__ rold) {
+  if (s.state[0]) return;
+  const int c = ccell[blockIdx.x], a = lo[blockIdx.x], e = hi[blockIdx.x];
+#pragma unroll
+  for (int q = 0; q < NR; ++q) {
+    const int pr = c * NR + q;
+    if (!s.active[pr]) continue;
+    const double alpha = s.alpha[pr];
+    const size_t off = (size_t)q * n;
+    for (int i = a + threadIdx.x; i < e; i += blockDim.x) {
+      const double ri = r[off + i];
+      if (FLEX) rold[off + i] = ri;
+      r[off + i] = ri + (-alpha) * qv[off + i];
+      x[off + i] = x[off + i] + alpha * pv[off + i];
+    }
+  }
+}
+
+// End of a pass: partials hold (r.r, b.b) of the true residual b - A x.
+// Pairs above the tolerance that did not break down are re-armed for a
+// restart from their current x (state[0] stays 0 when there is one).
+__global__ __launch_bounds__(kPairThreads) void k_bp_final(int npair, int nrhs, const int* cell_chunk0,
+                                                           const double* partials, BPtrs s, double eps2, int ittype,
+                                                           int niter, int rearm) {
+  int any = 0;
+  for (int p = threadIdx.x; p < npair; p += blockDim.x) {
+    const int c = p / nrhs, q = p - c * nrhs;
+    const double rr = chunk_sum(partials, cell_chunk0, c, q, nrhs, 2, 0);
+    const double bb = chunk_sum(partials, cell_chunk0, c, q, nrhs, 2, 1);
+    s.relres[p] = bb > 0.0 ? sqrt(rr / bb) : sqrt(rr);
+    const bool ok = !(rr > pair_thr2(s, p, ittype, eps2));
+    s.conv[p] = ok ? 1 : 0;
+    if (!ok && rearm && s.iters[p] < niter && s.alpha[p] != 0.0) {
+      s.active[p] = 1;
+      s.fresh[p] = 1;
+      any = 1;
+    } else {
+      s.active[p] = 0;
+    }
+  }
+  any = __syncthreads_or(any);
+  if (threadIdx.x == 0) s.state[0] = any ? 0 : 1;
+}
+
+__global__ void k_copy_vec(int64_t n, const double* __restrict__ a, double* __restrict__ b) {
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) b[i] = a[i];
+}
+
+__global__ void k_zero_vec(int64_t n, double* __restrict__ a) {
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) a[i] = 0.0;
+}
+
+int grid_vec(int64_t n) { return (int)std::max<int64_t>(1, std::min<int64_t>((n + 255) / 256, 4096)); }
+
+// Work vectors of a batched solve (all n x nrhs, fp64) and the pair state.
+struct BWork {
+  double *x, *r, *z, *p, *q, *rold, *b, *partials;
+  BPtrs s;
+  int* state_host;
+};
+
+BWork work(Batch& bt, Context& ctx, int nrhs) {
+  const size_t vb = (size_t)bt.n * nrhs * 8;
+  const int npair = bt.ncell * nrhs;
+  BWork w;
+  w.x = ctx.vec(40, vb);
+  w.r = ctx.vec(41, vb);
+  w.z = ctx.vec(42, vb);
+  w.p = ctx.vec(43, vb);
+  w.q = ctx.vec(44, vb);
+  w.rold = ctx.vec(45, vb);
+  w.b = ctx.vec(46, vb);
+  w.partials = ctx.vec(47, (size_t)bt.nchunks * nrhs * 3 * 8 + 64);
+  const size_t pd = ((size_t)npair + 1) & ~size_t(1);  // keep the int arrays 8-byte aligned
+  char* base = reinterpret_cast<char*>(ctx.vec(48, pd * 8 * 7 + pd * 4 * 5 + 64));
+  double* d = reinterpret_cast<double*>(base);
+  w.s.gamma = d;
+  w.s.alpha = d + pd;
+  w.s.beta = d + 2 * pd;
+  w.s.rr0 = d + 3 * pd;
+  w.s.bb = d + 4 * pd;
+  w.s.pq = d + 5 * pd;
+  w.s.relres = d + 6 * pd;
+  int* iv = reinterpret_cast<int*>(d + 7 * pd);
+  w.s.active = iv;
+  w.s.iters = iv + pd;
+  w.s.fresh = iv + 2 * pd;
+  w.s.flag = iv + 3 * pd;
+  w.s.conv = iv + 4 * pd;
+  w.s.state = iv + 5 * pd;
+  w.state_host = reinterpret_cast<int*>(ctx.host_state->st);
+  return w;
+}
+
+template <int NR, int K>
+void dots(Batch& bt, Context& ctx, const BWork& w, const double* const* xs, const double* const* ys, const int* skip) {
+  VecSet v{};
+  for (int k = 0; k < K; ++k) { v.x[k] = xs[k]; v.y[k] = ys[k]; }
+  hipLaunchKernelGGL((k_bp_dots<NR, K>), dim3(bt.nchunks), dim3(256), 0, ctx.stream, (int64_t)bt.n,
+                     bt.chunk_lo.as<int>(), bt.chunk_hi.as<int>(), v, w.partials, skip);
+}
+
+// r = b - A x and the (r.r, b.b) partials.
+template <int NR>
+void residual_dots(Batch& bt, Context& ctx, const BWork& w) {
+  launch_spmv(bt.A, w.x, w.r, w.b, NR, nullptr, nullptr, ctx.rw, ctx.stream);
+  const double* xs[2] = {w.r, w.b};
+  dots<NR, 2>(bt, ctx, w, xs, xs, nullptr);
+}
+
+// One PCG iteration for every active pair.
+template <int NR>
+void iteration(Batch& bt, Context& ctx, const BWork& w, const SolveOptions& opt, const DevTri& t1, const DevTri& t2,
+               bool flex) {
+  hipStream_t s = ctx.stream;
+  const int* done = &w.s.state[0];
+  const int npair = bt.ncell * NR;
+  const double eps2 = opt.eps * opt.eps;
+  precond_sweeps(t1, t2, ctx, w.r, w.z, opt.prec, NR, done);
+  if (flex) {
+    const double* xs[3] = {w.r, w.r, w.rold};
+    const double* ys[3] = {w.z, w.r, w.z};
+    dots<NR, 3>(bt, ctx, w, xs, ys, done);
+  } else {
+    const double* xs[2] = {w.r, w.r};
+    const double* ys[2] = {w.z, w.r};
+    dots<NR, 2>(bt, ctx, w, xs, ys, done);
+  }
+  hipLaunchKernelGGL(k_bp_beta, dim3(1), dim3(kPairThreads), 0, s, npair, NR, bt.dcell_chunk0.as<int>(), w.partials,
+                     flex ? 3 : 2, w.s, eps2, opt.ittype, opt.niter);
+  hipLaunchKernelGGL(k_bp_p<NR>, dim3(bt.nchunks), dim3(256), 0, s, (int64_t)bt.n, bt.chunk_cell.as<int>(),
+                     bt.chunk_lo.as<int>(), bt.chunk_hi.as<int>(), w.s, w.z, w.p);
+  {
+    ProfScope ps(ctx, kProfSpmv, spmv_bytes(bt.A, NR));
+    launch_spmv(bt.A, w.p, w.q, nullptr, NR, nullptr, nullptr, ctx.rw, s, done);
+  }
+  {
+    const double* xs[1] = {w.p};
+    const double* ys[1] = {w.q};
+    dots<NR, 1>(bt, ctx, w, xs, ys, done);
+  }
+  hipLaunchKernelGGL(k_bp_alpha, dim3(1), dim3(kPairThreads), 0, s, npair, NR, bt.dcell_chunk0.as<int>(), w.partials,
+                     w.s);
+  if (flex)
+    hipLaunchKernelGGL((k_bp_xr<NR, true>), dim3(bt.nchunks), dim3(256), 0, s, (int64_t)bt.n, bt.chunk_cell.as<int>(),
+                       bt.chunk_lo.as<int>(), bt.chunk_hi.as<int>(), w.s, w.p, w.q, w.x, w.r, w.rold);
+  else
+    hipLaunchKernelGGL((k_bp_xr<NR, false>), dim3(bt.nchunks), dim3(256), 0, s, (int64_t)bt.n,
+                       bt.chunk_cell.as<int>(), bt.chunk_lo.as<int>(), bt.chunk_hi.as<int>(), w.s, w.p, w.q, w.x,
+                       w.r, w.rold);
+}
+
+}  // namespace
+
+// Captured block of check_every iterations, valid for one set of launch
+// parameters and work-vector addresses.
+struct BatchGraph {
+  hipGraph_t g = nullptr;
+  hipGraphExec_t exec = nullptr;
+  int nrhs = 0, prec = -1, iters = 0, sweep_mode = -1, waves = 0, flex = 0, niter = 0, ittype = -1;
+  double eps = 0.0;
+  std::vector<const void*> key;  // device buffers the captured launches use
+  ~BatchGraph() {
+    if (exec) (void)hipGraphExecDestroy(exec);
+    if (g) (void)hipGraphDestroy(g);
+  }
+};
+
+Batch::Batch() = default;
+Batch::~Batch() = default;
+void Batch::drop_graph() { graph.reset(); }
+
+void batch_build(Batch& bt, const std::vector<const Matrix*>& mats, const std::vector<const Factor*>& facts,
+                 int prec_mask) {
+  if (mats.empty() || mats.size() != facts.size()) throw Error(kErrArg, "batch: one factor per matrix");
+  bt.ncell = (int)mats.size();
+  bt.kind = facts[0]->kind;
+  std::vector<BlockCsr> full;
+  full.reserve(mats.size());
+  std::vector<const BlockCsr*> parts, lower, upper;
+  for (size_t c = 0; c < mats.size(); ++c) {
+    if (facts[c]->kind != bt.kind) throw Error(kErrArg, "batch: mixed IC2 and ILU2 factors");
+    if (facts[c]->u.n() != mats[c]->h.n()) throw Error(kErrArg, "batch: matrix and factor sizes differ");
+    if (mats[c]->symmetric) {
+      full.push_back(symm_mtr(mats[c]->h));
+      parts.push_back(&full.back());
+    } else {
+      parts.push_back(&mats[c]->h);
+    }
+    lower.push_back(&facts[c]->lower());
+    upper.push_back(&facts[c]->u);
+  }
+  dev_bsr_upload_parts(parts, kF64, bt.A);
+  full.clear();
+  bt.b = bt.A.b;
+  bt.n = bt.A.n;
+  bt.elem0.assign(bt.ncell + 1, 0);
+  for (int c = 0; c < bt.ncell; ++c) bt.elem0[c + 1] = bt.elem0[c] + mats[c]->h.n();
+  for (int p = 0; p < 3; ++p) {
+    if (!(prec_mask >> p & 1)) continue;
+    Factor::Image& im = bt.img[p];
+    dev_tri_forward_parts(lower, p, im.fwd);
+    dev_tri_backward_parts(upper, p, im.bwd);
+    im.has = true;
+  }
+  // reduction chunks: runs of at most kChunk elements inside one cell
+  std::vector<int> ccell, clo, chi;
+  bt.cell_chunk0.assign(bt.ncell + 1, 0);
+  for (int c = 0; c < bt.ncell; ++c) {
+    for (int64_t a = bt.elem0[c]; a < bt.elem0[c + 1]; a += kChunk) {
+      ccell.push_back(c);
+      clo.push_back((int)a);
+      chi.push_back((int)std::min<int64_t>(a + kChunk, bt.elem0[c + 1]));
+    }
+    bt.cell_chunk0[c + 1] = (int)ccell.size();
+  }
+  bt.nchunks = (int)ccell.size();
+  if (bt.nchunks == 0) throw Error(kErrArg, "batch: empty cells");
+  bt.chunk_cell.upload(ccell.data(), ccell.size() * 4);
+  bt.chunk_lo.upload(clo.data(), clo.size() * 4);
+  bt.chunk_hi.upload(chi.data(), chi.size() * 4);
+  bt.dcell_chunk0.upload(bt.cell_chunk0.data(), bt.cell_chunk0.size() * 4);
+  bt.graph.reset();
+  bt.tuned = false;
+}
+
+namespace {
+
+template <int NR>
+void solve_nr(Batch& bt, Context& ctx, const SolveOptions& opt, const double* b, double* x, bool fallback,
+              BatchStats& st, int* iters_out, double* relres_out) {
+  const Factor::Image& im = bt.img[opt.prec];
+  if (!im.has) throw Error(kErrArg, "batch: factor sweeps of the requested precision were not built");
+  const int npair = bt.ncell * NR;
+  if (npair > 1 << 20) throw Error(kErrArg, "batch: too many (cell, right-hand side) pairs");
+  hipStream_t s = ctx.stream;
+  BWork w = work(bt, ctx, NR);
+  const int64_t nn = bt.n * NR;
+  const double eps2 = opt.eps * opt.eps;
+  const bool flex = opt.prec != kF64;
+  const BatchTune& tu = bt.tune;
+  const int saved_mode = ctx.sweep_mode;
+  const int saved_waves = ctx.sync.waves_per_cu;
+  ctx.sweep_mode = tu.sweep_mode;
+  ctx.sync.waves_per_cu = tu.waves_per_cu;
+  auto restore = [&] {
+    ctx.sweep_mode = saved_mode;
+    ctx.sync.waves_per_cu = saved_waves;
+    ctx.sync.cooperative = true;
+  };
+  try {
+    ctx.sync.ensure(im.fwd.nsup, s);
+    ctx.rw.ensure();
+    hipLaunchKernelGGL(k_copy_vec, dim3(grid_vec(nn)), dim3(256), 0, s, nn, b, w.b);
+    hipLaunchKernelGGL(k_copy_vec, dim3(grid_vec(nn)), dim3(256), 0, s, nn, (const double*)x, w.x);
+    hipLaunchKernelGGL(k_zero_vec, dim3(grid_vec(nn)), dim3(256), 0, s, nn, w.p);
+    residual_dots<NR>(bt, ctx, w);
+    hipLaunchKernelGGL(k_bp_start, dim3(1), dim3(kPairThreads), 0, s, npair, NR, bt.dcell_chunk0.as<int>(),
+                       w.partials, w.s, eps2, opt.ittype, fallback ? 1 : 0);
+    if (fallback)
+      hipLaunchKernelGGL(k_bp_zero<NR>, dim3(bt.nchunks), dim3(256), 0, s, (int64_t)bt.n, bt.chunk_cell.as<int>(),
+                         bt.chunk_lo.as<int>(), bt.chunk_hi.as<int>(), w.s, w.b, w.x, w.r);
+    const int every = std::max(1, tu.check_every);
+    const bool use_graph = tu.graph && !ctx.prof;
+    if (use_graph) {
+      // every buffer the iteration touches exists before capture (no allocation inside it)
+      std::vector<const void*> key = {w.x, w.r, w.z, w.p, w.q, w.rold, w.partials, w.s.state, ctx.sync.ctl.p};
+      if (opt.prec == kF64) {
+        key.push_back(ctx.vec(12, (size_t)nn * 8));
+      } else {
+        key.push_back(ctx.vec(13, (size_t)nn * 4));
+        key.push_back(ctx.vec(14, (size_t)nn * 4));
+      }
+      BatchGraph* g = bt.graph.get();
+      const bool valid = g && g->nrhs == NR && g->prec == opt.prec && g->iters == every &&
+                         g->sweep_mode == tu.sweep_mode && g->waves == tu.waves_per_cu && g->flex == flex &&
+                         g->niter == opt.niter && g->ittype == opt.ittype && g->eps == opt.eps && g->key == key;
+      if (!valid) {
+        bt.graph.reset(new BatchGraph());
+        g = bt.graph.get();
+        ctx.sync.cooperative = false;
+        BCM_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+        try {
+          for (int it = 0; it < every; ++it) iteration<NR>(bt, ctx, w, opt, im.fwd, im.bwd, flex);
+        } catch (...) {
+          hipGraph_t junk = nullptr;
+          (void)hipStreamEndCapture(s, &junk);
+          if (junk) (void)hipGraphDestroy(junk);
+          throw;
+        }
+        BCM_HIP(hipStreamEndCapture(s, &g->g));
+        BCM_HIP(hipGraphInstantiate(&g->exec, g->g, nullptr, nullptr, 0));
+        ctx.sync.cooperative = true;
+        g->nrhs = NR; g->prec = opt.prec; g->iters = every; g->sweep_mode = tu.sweep_mode;
+        g->waves = tu.waves_per_cu; g->flex = flex; g->niter = opt.niter; g->ittype = opt.ittype; g->eps = opt.eps;
+        g->key = key;
+      }
+    }
+    int passes = 0;
+    while (true) {
+      while (true) {
+        if (use_graph) {
+          BCM_HIP(hipGraphLaunch(bt.graph->exec, s));
+        } else {
+          for (int it = 0; it < every; ++it) iteration<NR>(bt, ctx, w, opt, im.fwd, im.bwd, flex);
+        }
+        BCM_HIP(hipMemcpyAsync(w.state_host, w.s.state, 8, hipMemcpyDeviceToHost, s));
+        BCM_HIP(hipStreamSynchronize(s));
+        if (w.state_host[0]) break;
+      }
+      // true residual of every pair; re-arm the ones above the tolerance
+      residual_dots<NR>(bt, ctx, w);
+      const int rearm = passes < opt.max_restarts ? 1 : 0;
+      hipLaunchKernelGGL(k_bp_final, dim3(1), dim3(kPairThreads), 0, s, npair, NR, bt.dcell_chunk0.as<int>(),
+                         w.partials, w.s, eps2, opt.ittype, opt.niter, rearm);
+      BCM_HIP(hipMemcpyAsync(w.state_host, w.s.state, 8, hipMemcpyDeviceToHost, s));
+      BCM_HIP(hipStreamSynchronize(s));
+      if (w.state_host[0]) break;
+      ++passes;
+      hipLaunchKernelGGL(k_zero_vec, dim3(grid_vec(nn)), dim3(256), 0, s, nn, w.p);
+    }
+    st.restarts = passes;
+    st.breakdowns = w.state_host[1];
+    hipLaunchKernelGGL(k_copy_vec, dim3(grid_vec(nn)), dim3(256), 0, s, nn, (const double*)w.x, x);
+    std::vector<int> it(npair), cv(npair);
+    std::vector<double> rr(npair);
+    BCM_HIP(hipMemcpyAsync(it.data(), w.s.iters, npair * 4, hipMemcpyDeviceToHost, s));
+    BCM_HIP(hipMemcpyAsync(cv.data(), w.s.conv, npair * 4, hipMemcpyDeviceToHost, s));
+    BCM_HIP(hipMemcpyAsync(rr.data(), w.s.relres, npair * 8, hipMemcpyDeviceToHost, s));
+    BCM_HIP(hipStreamSynchronize(s));
+    st.pairs = npair;
+    for (int p = 0; p < npair; ++p) {
+      st.converged += cv[p];
+      st.iters_max = std::max(st.iters_max, it[p]);
+      st.iters_sum += it[p];
+      st.relres_max = std::max(st.relres_max, rr[p]);
+    }
+    if (iters_out) std::memcpy(iters_out, it.data(), npair * 4);
+    if (relres_out) std::memcpy(relres_out, rr.data(), npair * 8);
+  } catch (...) {
+    restore();
+    throw;
+  }
+  restore();
+}
+
+}  // namespace
+
+void batch_solve(Batch& bt, Context& ctx, const SolveOptions& opt, int nrhs, const double* b, double* x,
+                 bool fallback, BatchStats& st, int* iters, double* relres) {
+  auto t0 = std::chrono::steady_clock::now();
+  st = BatchStats();
+  if (opt.prec < kF64 || opt.prec > kBF16) throw Error(kErrArg, "batch: unknown precision");
+  if (opt.kind != kPcg && opt.kind != kBlockPcg) throw Error(kErrArg, "batch: block PCG only");
+  if (bt.kind != 0) throw Error(kErrArg, "batch: block PCG needs IC2 factors");
+  switch (nrhs) {
+    case 1: solve_nr<1>(bt, ctx, opt, b, x, fallback, st, iters, relres); break;
+    case 3: solve_nr<3>(bt, ctx, opt, b, x, fallback, st, iters, relres); break;
+    case 6: solve_nr<6>(bt, ctx, opt, b, x, fallback, st, iters, relres); break;
+    default: throw Error(kErrArg, "batch: nrhs must be 1, 3 or 6");
+  }
+  BCM_HIP(hipStreamSynchronize(ctx.stream));
+  ctx.check_sweep_error();
+  st.seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+}
+
+BatchTune batch_tune(Batch& bt, Context& ctx, const SolveOptions& opt, int nrhs, const double* b) {
+  // candidates: sweep schedule x launch shape x graph replay x polling interval
+  std::vector<BatchTune> cand;
+  for (int graph = 0; graph <= 1; ++graph)
+    for (int mode = 0; mode <= 1; ++mode)
+      for (int waves : {4, 8, 16}) {
+        if (mode == 0 && waves != 8) continue;
+        for (int every : {4, 16}) {
+          BatchTune t;
+          t.sweep_mode = mode;
+          t.waves_per_cu = waves;
+          t.graph = graph;
+          t.check_every = every;
+          cand.push_back(t);
+        }
+      }
+  const size_t vb = (size_t)bt.n * nrhs * 8;
+  double* xs = ctx.vec(49, vb);
+  SolveOptions o = opt;
+  o.eps = 0.0;         // never converges: every candidate runs the same iterations
+  o.max_restarts = 0;
+  const BatchTune keep = bt.tune;
+  BatchTune best = keep;
+  best.ms_per_iter = 1e300;
+  o.niter = 32;  // a multiple of every polling interval: all candidates run the same iterations
+  for (const BatchTune& c : cand) {
+    bt.tune = c;
+    BatchStats st;
+    BCM_HIP(hipMemsetAsync(xs, 0, vb, ctx.stream));
+    batch_solve(bt, ctx, o, nrhs, b, xs, false, st, nullptr, nullptr);  // warm-up (and graph capture)
+    BCM_HIP(hipMemsetAsync(xs, 0, vb, ctx.stream));
+    auto t0 = std::chrono::steady_clock::now();
+    batch_solve(bt, ctx, o, nrhs, b, xs, false, st, nullptr, nullptr);
+    const double ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    const double per = ms / std::max(1, st.iters_max);
+    if (per < best.ms_per_iter) {
+      best = c;
+      best.ms_per_iter = per;
+    }
+  }
+  bt.tune = best;
+  bt.tuned = true;
+  bt.graph.reset();
+  return best;
+}
+
+}  // namespace bcm

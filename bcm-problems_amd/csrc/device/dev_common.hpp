@@ -36,9 +36,6 @@ template <class T> __device__ __forceinline__ void st_coherent(T* p, T v) {
   __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
-// Block-level reduction of K values followed by the last-block finalisation:
-// block sums go to partials[block*K + k]; the block that completes last adds
-// them in a fixed order (deterministic) into out[k] and re-arms the counter.
 // Orders this wave's LDS writes before its following LDS reads by other
 // lanes (and reads before later writes). LDS operations of one wave complete
 // in program order, so a wavefront-scope fence, which only stops the compiler
@@ -48,50 +45,44 @@ __device__ __forceinline__ void wave_lds_sync() {
   __builtin_amdgcn_wave_barrier();
 }
 
+// Block sums without any cross-block synchronisation: block b writes
+// partials[b * K + k]. A later kernel on the same stream adds them with
+// sum_partials (the kernel boundary makes the stores visible on every XCD),
+// so no block pays for a device-scope fence.
 template <int K>
-__device__ void reduce_finalize(double (&v)[K], double* partials, unsigned* counter, double* out) {
-  __shared__ double sm[K][16];
-  __shared__ int is_last;
+__device__ void block_partials(const double (&v)[K], double* partials) {
+  __shared__ double bp_sm[K][16];
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nw = blockDim.x >> 6;
 #pragma unroll
   for (int k = 0; k < K; ++k) {
     double x = v[k];
     for (int off = 32; off > 0; off >>= 1) x += __shfl_down(x, off);
-    if (lane == 0) sm[k][wave] = x;
+    if (lane == 0) bp_sm[k][wave] = x;
   }
   __syncthreads();
-  if (threadIdx.x == 0) {
-#pragma unroll
-    for (int k = 0; k < K; ++k) {
-      double s = 0.0;
-      for (int w = 0; w < nw; ++w) s += sm[k][w];
-      partials[(size_t)blockIdx.x * K + k] = s;
-    }
-    __threadfence();
-    unsigned prev = atomicAdd(counter, 1u);
-    is_last = prev == gridDim.x - 1;
-  }
-  __syncthreads();
-  if (!is_last) return;
-  __threadfence();
-  const int nb = gridDim.x;
-#pragma unroll
-  for (int k = 0; k < K; ++k) {
+  for (int k = threadIdx.x; k < K; k += blockDim.x) {
     double s = 0.0;
-    for (int i = threadIdx.x; i < nb; i += blockDim.x)
-      s += __hip_atomic_load(&partials[(size_t)i * K + k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    for (int off = 32; off > 0; off >>= 1) s += __shfl_down(s, off);
-    __syncthreads();
-    if (lane == 0) sm[k][wave] = s;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-      double t = 0.0;
-      for (int w = 0; w < nw; ++w) t += sm[k][w];
-      out[k] = t;
-    }
+    for (int w = 0; w < nw; ++w) s += bp_sm[k][w];
+    partials[(size_t)blockIdx.x * K + k] = s;
   }
-  if (threadIdx.x == 0) *counter = 0u;
 }
+
+// Sum over nb blocks of value k of per-block partials with stride K, in a
+// fixed order; called by every thread of one block, returns the sum to all.
+__device__ __forceinline__ double sum_partials(const double* partials, int nb, int K, int k) {
+  __shared__ double sp_sm[16];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nw = blockDim.x >> 6;
+  double s = 0.0;
+  for (int i = threadIdx.x; i < nb; i += blockDim.x) s += partials[(size_t)i * K + k];
+  for (int off = 32; off > 0; off >>= 1) s += __shfl_down(s, off);
+  __syncthreads();
+  if (lane == 0) sp_sm[wave] = s;
+  __syncthreads();
+  double t = 0.0;
+  for (int w = 0; w < nw; ++w) t += sp_sm[w];
+  return t;
+}
+
 
 }  // namespace
 }  // namespace bcm

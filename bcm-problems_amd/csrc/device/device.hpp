@@ -63,6 +63,8 @@ struct DevBsr {
 
 // Builds a device block-CSR from host storage (uniform block size required).
 void dev_bsr_upload(const BlockCsr& h, int prec, DevBsr& d);
+// Block-diagonal union of several matrices (same block size), part after part.
+void dev_bsr_upload_parts(const std::vector<const BlockCsr*>& parts, int prec, DevBsr& d);
 
 struct DevTri {
   int nsup = 0, n = 0, b = 0, ndep = 0;
@@ -84,31 +86,48 @@ void dev_tri_forward(const BlockCsr& f, int prec, DevTri& t);
 // Backward sweep through a factor stored by rows (reference SolveU):
 // z_i = D_i (x_i - sum_{j>i} F_ij z_j).
 void dev_tri_backward(const BlockCsr& f, int prec, DevTri& t);
+// The same sweeps through the block-diagonal union of several factors.
+void dev_tri_forward_parts(const std::vector<const BlockCsr*>& f, int prec, DevTri& t);
+void dev_tri_backward_parts(const std::vector<const BlockCsr*>& f, int prec, DevTri& t);
 
 // State of the sync-free sweeps on one stream: ctl[2] is the error flag (a
-// dependency wait reached its bound).
+// dependency wait reached its bound). Launch shape: at most waves_per_cu resident waves
+// per CU (never more than the occupancy limit), launched cooperatively
+// (residency guaranteed by the runtime) or, for stream capture into a graph,
+// as an ordinary launch of a grid that fits the device at once.
 struct SweepSync {
   DevBuf ctl;
   int nsup = 0;
+  int waves_per_cu = 8;
+  bool cooperative = true;
   void ensure(int nsup, hipStream_t s);
   int error(hipStream_t s);
 };
 
-// Reduction workspace: partial sums and the completion counter of the
-// single-pass (last block finalises) reductions.
+// Per-block partial sums of the reductions. A reduction kernel writes one
+// partial per block and value; a later kernel on the stream adds them in a
+// fixed order (deterministic, and no block needs a device-scope fence).
+// pending_blocks / pending_k describe the partials not yet consumed.
 struct RedWork {
   DevBuf partials;  // kRedBlocks * kMaxDots doubles
-  DevBuf counter;   // unsigned
+  int pending_blocks = 0, pending_k = 0;
   void ensure();
 };
-constexpr int kRedBlocks = 512;
+constexpr int kRedBlocks = 2048;
 constexpr int kMaxDots = 8;
+// Passed as the output of a reduction: leave the partials in the RedWork for
+// the next scalar kernel to add (no finalize launch).
+inline double* const kDeferred = reinterpret_cast<double*>(static_cast<uintptr_t>(1));
+
+// out[k] = sum of the pending partials of value k; clears the pending state.
+void launch_finalize(RedWork& rw, double* out, hipStream_t s, const int* skip = nullptr);
 
 // ---- launches (all asynchronous on `s`) ----
 
 // y = A x (mode 0) or y = bsub - A x (mode 1); nrhs columns with leading
-// dimension n. Optionally accumulates dot(wdot, y) over all rows into
-// *dot_out (first column only) when dot_out != nullptr.
+// dimension n. Optionally computes dot(wdot, y) over all rows (first column
+// only) when dot_out != nullptr: into *dot_out, or left as pending partials
+// in rw when dot_out == kDeferred.
 // Every launch takes an optional device flag `skip`: when *skip != 0 the
 // kernel returns at once (a finished solver keeps its queued iterations inert).
 void launch_spmv(const DevBsr& A, const double* x, double* y, const double* bsub, int nrhs,
@@ -120,12 +139,14 @@ void launch_spmv(const DevBsr& A, const double* x, double* y, const double* bsub
 void launch_sweep(const DevTri& T, const void* x, void* y, int vec_prec, int nrhs, hipStream_t s,
                   const int* skip = nullptr, SweepSync* sync = nullptr);
 
-// out[k] = sum_i x_k[i] * y_k[i] for K pairs (K <= kMaxDots), one pass.
+// out[k] = sum_i x_k[i] * y_k[i] for K pairs (K <= kMaxDots), one pass plus a
+// finalize launch; with out == kDeferred the partials stay pending in rw.
 void launch_multidot(int n, int k, const double* const* xs, const double* const* ys, double* out,
                      RedWork& rw, hipStream_t s, const int* skip = nullptr);
 
 // Converts between fp64 and fp32 vectors.
-void launch_convert(int64_t n, const void* src, int src_prec, void* dst, int dst_prec, hipStream_t s);
+void launch_convert(int64_t n, const void* src, int src_prec, void* dst, int dst_prec, hipStream_t s,
+                    const int* skip = nullptr);
 
 // Generic element-wise kernels used by the drivers (defined with the kernels).
 void launch_axpby(int64_t n, double alpha, const double* x, double beta, double* y, hipStream_t s);

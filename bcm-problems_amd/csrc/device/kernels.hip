@@ -75,8 +75,7 @@ __global__ __launch_bounds__(256) void k_spmv(int nsup, int n, const int* __rest
                                               const int* __restrict__ ja, const TA* __restrict__ a,
                                               const double* __restrict__ x, double* __restrict__ y,
                                               const double* __restrict__ bsub, const double* __restrict__ wdot,
-                                              double* partials, unsigned* counter, double* dot_out,
-                                              int rows_per_wave, const int* skip) {
+                                              double* partials, int rows_per_wave, const int* skip) {
   if (skip && *skip) return;
   constexpr int L = B / V;
   constexpr int G = 64 / L;
@@ -133,30 +132,9 @@ __global__ __launch_bounds__(256) void k_spmv(int nsup, int n, const int* __rest
     }
     wave_lds_sync();
   }
-  if (dot_out) {
-    double vv[1] = {dotv};
-    reduce_finalize<1>(vv, partials, counter, dot_out);
-  }
-}
-
-// Variable block sizes: one thread per scalar row (correctness path).
-__global__ void k_spmv_var(int nsup, int n, const int* __restrict__ sprnd, const int* __restrict__ ia,
-                           const int* __restrict__ ja, const int64_t* __restrict__ bsa,
-                           const double* __restrict__ a, const double* __restrict__ x, double* __restrict__ y,
-                           const double* __restrict__ bsub, int nrhs) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  int lo = 0, hi = nsup;
-  while (hi - lo > 1) { int m = (lo + hi) / 2; if (sprnd[m] <= i) lo = m; else hi = m; }
-  const int row = lo, r = i - sprnd[row], bi = sprnd[row + 1] - sprnd[row];
-  for (int k = 0; k < nrhs; ++k) {
-    double acc = 0.0;
-    for (int j = ia[row]; j < ia[row + 1]; ++j) {
-      const int col = ja[j], c0 = sprnd[col], bj = sprnd[col + 1] - c0;
-      const double* blk = a + bsa[j];
-      for (int c = 0; c < bj; ++c) acc += blk[(size_t)c * bi + r] * x[(size_t)k * n + c0 + c];
-    }
-    y[(size_t)k * n + i] = bsub ? bsub[(size_t)k * n + i] - acc : acc;
+  if (partials) {
+    const double vv[1] = {dotv};
+    block_partials<1>(vv, partials);
   }
 }
 
@@ -404,8 +382,7 @@ struct DotArgs {
 };
 
 template <int K>
-__global__ __launch_bounds__(256) void k_multidot(int64_t n, DotArgs args, double* partials, unsigned* counter,
-                                                  double* out, const int* skip) {
+__global__ __launch_bounds__(256) void k_multidot(int64_t n, DotArgs args, double* partials, const int* skip) {
   if (skip && *skip) return;
   double v[K];
 #pragma unroll
@@ -415,7 +392,15 @@ __global__ __launch_bounds__(256) void k_multidot(int64_t n, DotArgs args, doubl
 #pragma unroll
     for (int k = 0; k < K; ++k) v[k] += args.x[k][i] * args.y[k][i];
   }
-  reduce_finalize<K>(v, partials, counter, out);
+  block_partials<K>(v, partials);
+}
+
+// out[k] = sum of the per-block partials of value k (one block per value).
+__global__ __launch_bounds__(256) void k_finalize(const double* partials, int nb, int K, double* out,
+                                                  const int* skip) {
+  if (skip && *skip) return;
+  const double v = sum_partials(partials, nb, K, blockIdx.x);
+  if (threadIdx.x == 0) out[blockIdx.x] = v;
 }
 
 __global__ void k_axpby(int64_t n, double alpha, const double* __restrict__ x, double beta, double* __restrict__ y) {
@@ -435,7 +420,8 @@ __global__ void k_fill(int64_t n, double v, double* __restrict__ y) {
 }
 
 template <class S, class D>
-__global__ void k_convert(int64_t n, const S* __restrict__ x, D* __restrict__ y) {
+__global__ void k_convert(int64_t n, const S* __restrict__ x, D* __restrict__ y, const int* skip) {
+  if (skip && *skip) return;
   const int64_t stride = (int64_t)gridDim.x * blockDim.x;
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) y[i] = (D)x[i];
 }
@@ -484,16 +470,18 @@ static uint16_t to_bf16(float f) {
   return (uint16_t)((u + rounding) >> 16);
 }
 
-static void upload_values(const std::vector<double>& v, int prec, DevBuf& d) {
+// Converts `count` fp64 values to `prec` and copies them to dst (device).
+static void upload_values_at(const double* v, size_t count, int prec, void* dst) {
+  if (count == 0) return;
   if (prec == kF64) {
-    d.upload(v.data(), v.size() * 8);
+    BCM_HIP(hipMemcpy(dst, v, count * 8, hipMemcpyHostToDevice));
   } else if (prec == kF32) {
-    std::vector<float> f(v.begin(), v.end());
-    d.upload(f.data(), f.size() * 4);
+    std::vector<float> f(v, v + count);
+    BCM_HIP(hipMemcpy(dst, f.data(), count * 4, hipMemcpyHostToDevice));
   } else {
-    std::vector<uint16_t> h(v.size());
-    for (size_t i = 0; i < v.size(); ++i) h[i] = to_bf16((float)v[i]);
-    d.upload(h.data(), h.size() * 2);
+    std::vector<uint16_t> h(count);
+    for (size_t i = 0; i < count; ++i) h[i] = to_bf16((float)v[i]);
+    BCM_HIP(hipMemcpy(dst, h.data(), count * 2, hipMemcpyHostToDevice));
   }
 }
 
@@ -504,95 +492,149 @@ static int uniform_block(const BlockCsr& h) {
   return b;
 }
 
-void dev_bsr_upload(const BlockCsr& h, int prec, DevBsr& d) {
-  d.nsup = h.nsup;
-  d.n = h.n();
-  d.b = uniform_block(h);
-  if (d.b == 0 || d.b > 64) throw Error(kErrArg, "device matrices need a uniform block size of at most 64");
-  d.nzja = h.nzja();
-  d.prec = prec;
-  d.ia.upload(h.ia.data(), h.ia.size() * 4);
-  d.ja.upload(h.ja.data(), h.ja.size() * 4);
-  upload_values(h.a, prec, d.a);
+static int parts_block(const std::vector<const BlockCsr*>& parts, const char* what) {
+  int b = -1;
+  for (const BlockCsr* p : parts) {
+    const int bp = uniform_block(*p);
+    if (bp == 0 || bp > 64 || (b >= 0 && bp != b))
+      throw Error(kErrArg, std::string(what) + " need one uniform block size of at most 64");
+    b = bp;
+  }
+  if (b <= 0) throw Error(kErrArg, std::string(what) + ": empty");
+  return b;
 }
 
-static void build_tri(const BlockCsr& f, bool forward, int prec, DevTri& t) {
-  const int b = uniform_block(f);
-  if (b == 0 || b > 64) throw Error(kErrArg, "device factors need a uniform block size of at most 64");
-  const int nsup = f.nsup, bb = b * b;
-  t.nsup = nsup;
-  t.n = f.n();
+void dev_bsr_upload_parts(const std::vector<const BlockCsr*>& parts, int prec, DevBsr& d) {
+  const int b = parts_block(parts, "device matrices");
+  int64_t nsup = 0, nzja = 0;
+  for (const BlockCsr* p : parts) { nsup += p->nsup; nzja += p->nzja(); }
+  if (nsup * b > INT32_MAX || nzja > INT32_MAX) throw Error(kErrArg, "device matrix exceeds 32-bit indexing");
+  d.nsup = (int)nsup;
+  d.n = (int)(nsup * b);
+  d.b = b;
+  d.nzja = (int)nzja;
+  d.prec = prec;
+  const size_t bb = (size_t)b * b, es = prec_bytes(prec);
+  std::vector<int> ia(nsup + 1, 0), ja(std::max<int64_t>(nzja, 1), 0);
+  d.a.alloc(std::max<size_t>(nzja * bb * es, es));
+  int64_t r0 = 0, j0 = 0;
+  for (const BlockCsr* p : parts) {
+    for (int i = 0; i < p->nsup; ++i) ia[r0 + i + 1] = (int)(j0 + p->ia[i + 1]);
+    for (int j = 0; j < p->nzja(); ++j) ja[j0 + j] = (int)(r0 + p->ja[j]);
+    upload_values_at(p->a.data(), p->a.size(), prec, static_cast<char*>(d.a.p) + (size_t)j0 * bb * es);
+    r0 += p->nsup;
+    j0 += p->nzja();
+  }
+  d.ia.upload(ia.data(), ia.size() * 4);
+  d.ja.upload(ja.data(), ja.size() * 4);
+}
+
+void dev_bsr_upload(const BlockCsr& h, int prec, DevBsr& d) { dev_bsr_upload_parts({&h}, prec, d); }
+
+// Sweep image of the block-diagonal union of factors (one per part, rows of
+// part p after those of parts < p). Forward: rows gather the transposed
+// blocks of the column-stored factor; backward: rows of the row-stored factor.
+// Diagonal blocks are stored transposed for the forward sweep. Dependency
+// levels (a row's level is one more than the deepest row it depends on) are
+// merged over the parts for the level-scheduled launches.
+static void build_tri_parts(const std::vector<const BlockCsr*>& parts, bool forward, int prec, DevTri& t) {
+  const int b = parts_block(parts, "device factors");
+  const size_t bb = (size_t)b * b, es = prec_bytes(prec);
+  int64_t nsup = 0, ndep = 0;
+  for (const BlockCsr* f : parts) { nsup += f->nsup; ndep += f->nzja() - f->nsup; }
+  if (nsup * b > INT32_MAX || ndep > INT32_MAX) throw Error(kErrArg, "device factor exceeds 32-bit indexing");
+  t.nsup = (int)nsup;
+  t.n = (int)(nsup * b);
   t.b = b;
   t.forward = forward;
   t.prec = prec;
-  std::vector<int> ia(nsup + 1, 0), ja;
-  std::vector<double> off, diag((size_t)nsup * bb);
-  for (int i = 0; i < nsup; ++i) {
-    if (f.ia[i] == f.ia[i + 1] || f.ja[f.ia[i]] != i) throw Error(kErrArg, "factor rows must start with the diagonal block");
-    const double* d = &f.a[f.bsa[f.ia[i]]];
-    double* o = &diag[(size_t)i * bb];
-    for (int r = 0; r < b; ++r)
-      for (int c = 0; c < b; ++c) o[c * b + r] = forward ? d[r * b + c] : d[c * b + r];
-  }
-  if (!forward) {
-    for (int i = 0; i < nsup; ++i) {
-      for (int j = f.ia[i] + 1; j < f.ia[i + 1]; ++j) {
-        ja.push_back(f.ja[j]);
-        off.insert(off.end(), f.a.begin() + f.bsa[j], f.a.begin() + f.bsa[j] + bb);
-      }
-      ia[i + 1] = (int)ja.size();
+  t.ndep = (int)ndep;
+  std::vector<int> ia_all(nsup + 1, 0), ja_all(std::max<int64_t>(ndep, 1), 0), lev_all(nsup, 0);
+  t.off.alloc(std::max<size_t>(ndep * bb * es, es));
+  t.diag.alloc(nsup * bb * es);
+  int64_t r0 = 0, d0 = 0;
+  int mx = 0;
+  for (const BlockCsr* fp : parts) {
+    const BlockCsr& f = *fp;
+    const int ns = f.nsup;
+    std::vector<int> ia(ns + 1, 0), ja;
+    std::vector<double> off, diag((size_t)ns * bb);
+    for (int i = 0; i < ns; ++i) {
+      if (f.ia[i] == f.ia[i + 1] || f.ja[f.ia[i]] != i)
+        throw Error(kErrArg, "factor rows must start with the diagonal block");
+      const double* d = &f.a[f.bsa[f.ia[i]]];
+      double* o = &diag[(size_t)i * bb];
+      for (int r = 0; r < b; ++r)
+        for (int c = 0; c < b; ++c) o[c * b + r] = forward ? d[r * b + c] : d[c * b + r];
     }
-  } else {
-    // Row k gathers the transposed blocks F_ik of the column-stored factor.
-    std::vector<int> cnt(nsup + 1, 0);
-    for (int i = 0; i < nsup; ++i)
-      for (int j = f.ia[i] + 1; j < f.ia[i + 1]; ++j) cnt[f.ja[j] + 1]++;
-    for (int k = 0; k < nsup; ++k) ia[k + 1] = ia[k] + cnt[k + 1];
-    ja.resize(ia[nsup]);
-    off.resize((size_t)ia[nsup] * bb);
-    std::vector<int> pos(ia.begin(), ia.end() - 1);
-    for (int i = 0; i < nsup; ++i)
-      for (int j = f.ia[i] + 1; j < f.ia[i + 1]; ++j) {
-        const int k = f.ja[j];
-        const int p = pos[k]++;
-        ja[p] = i;
-        const double* s = &f.a[f.bsa[j]];
-        double* o = &off[(size_t)p * bb];
-        for (int r = 0; r < b; ++r)
-          for (int c = 0; c < b; ++c) o[c * b + r] = s[r * b + c];
+    if (!forward) {
+      for (int i = 0; i < ns; ++i) {
+        for (int j = f.ia[i] + 1; j < f.ia[i + 1]; ++j) {
+          ja.push_back(f.ja[j]);
+          off.insert(off.end(), f.a.begin() + f.bsa[j], f.a.begin() + f.bsa[j] + bb);
+        }
+        ia[i + 1] = (int)ja.size();
       }
+    } else {
+      std::vector<int> cnt(ns + 1, 0);
+      for (int i = 0; i < ns; ++i)
+        for (int j = f.ia[i] + 1; j < f.ia[i + 1]; ++j) cnt[f.ja[j] + 1]++;
+      for (int k = 0; k < ns; ++k) ia[k + 1] = ia[k] + cnt[k + 1];
+      ja.resize(ia[ns]);
+      off.resize((size_t)ia[ns] * bb);
+      std::vector<int> pos(ia.begin(), ia.end() - 1);
+      for (int i = 0; i < ns; ++i)
+        for (int j = f.ia[i] + 1; j < f.ia[i + 1]; ++j) {
+          const int k = f.ja[j];
+          const int p = pos[k]++;
+          ja[p] = i;
+          const double* src = &f.a[f.bsa[j]];
+          double* o = &off[(size_t)p * bb];
+          for (int r = 0; r < b; ++r)
+            for (int c = 0; c < b; ++c) o[c * b + r] = src[r * b + c];
+        }
+    }
+    for (int s = 0; s < ns; ++s) {
+      const int row = forward ? s : ns - 1 - s;
+      int l = 0;
+      for (int j = ia[row]; j < ia[row + 1]; ++j) l = std::max(l, lev_all[r0 + ja[j]] + 1);
+      lev_all[r0 + row] = l;
+      mx = std::max(mx, l);
+    }
+    for (int i = 0; i < ns; ++i) ia_all[r0 + i + 1] = (int)(d0 + ia[i + 1]);
+    for (size_t j = 0; j < ja.size(); ++j) ja_all[d0 + j] = (int)(r0 + ja[j]);
+    upload_values_at(off.data(), off.size(), prec, static_cast<char*>(t.off.p) + (size_t)d0 * bb * es);
+    upload_values_at(diag.data(), diag.size(), prec, static_cast<char*>(t.diag.p) + (size_t)r0 * bb * es);
+    r0 += ns;
+    d0 += (int64_t)ja.size();
   }
-  // Dependency levels: a row's level is one more than the deepest row it
-  // depends on; rows are listed level by level for the launches.
-  std::vector<int> lev(nsup, 0);
-  int mx = -1;
-  for (int s = 0; s < nsup; ++s) {
-    const int row = forward ? s : nsup - 1 - s;
-    int l = 0;
-    for (int j = ia[row]; j < ia[row + 1]; ++j) l = std::max(l, lev[ja[j]] + 1);
-    lev[row] = l;
-    mx = std::max(mx, l);
-  }
-  t.levels = mx + 1;
+  t.levels = nsup ? mx + 1 : 0;
   t.level_ptr.assign(t.levels + 1, 0);
-  for (int i = 0; i < nsup; ++i) t.level_ptr[lev[i] + 1]++;
+  for (int64_t i = 0; i < nsup; ++i) t.level_ptr[lev_all[i] + 1]++;
   for (int l = 0; l < t.levels; ++l) t.level_ptr[l + 1] += t.level_ptr[l];
   std::vector<int> rows(nsup), fill(t.level_ptr.begin(), t.level_ptr.end() - 1);
-  for (int s = 0; s < nsup; ++s) {
-    const int row = forward ? s : nsup - 1 - s;
-    rows[fill[lev[row]]++] = row;
+  r0 = 0;
+  for (const BlockCsr* fp : parts) {
+    const int ns = fp->nsup;
+    for (int s = 0; s < ns; ++s) {
+      const int row = (int)r0 + (forward ? s : ns - 1 - s);
+      rows[fill[lev_all[row]]++] = row;
+    }
+    r0 += ns;
   }
-  t.level_rows.upload(rows.data(), rows.size() * 4);
-  t.ndep = ia[nsup];
-  t.ia.upload(ia.data(), ia.size() * 4);
-  t.ja.upload(ja.empty() ? ia.data() : ja.data(), std::max<size_t>(ja.size(), 1) * 4);
-  std::vector<double> offv = off.empty() ? std::vector<double>(1, 0.0) : off;
-  upload_values(offv, prec, t.off);
-  upload_values(diag, prec, t.diag);
+  t.level_rows.upload(rows.data(), std::max<size_t>(rows.size(), 1) * 4);
+  t.ia.upload(ia_all.data(), ia_all.size() * 4);
+  t.ja.upload(ja_all.data(), ja_all.size() * 4);
 }
 
-void dev_tri_forward(const BlockCsr& f, int prec, DevTri& t) { build_tri(f, true, prec, t); }
-void dev_tri_backward(const BlockCsr& f, int prec, DevTri& t) { build_tri(f, false, prec, t); }
+void dev_tri_forward(const BlockCsr& f, int prec, DevTri& t) { build_tri_parts({&f}, true, prec, t); }
+void dev_tri_backward(const BlockCsr& f, int prec, DevTri& t) { build_tri_parts({&f}, false, prec, t); }
+void dev_tri_forward_parts(const std::vector<const BlockCsr*>& f, int prec, DevTri& t) {
+  build_tri_parts(f, true, prec, t);
+}
+void dev_tri_backward_parts(const std::vector<const BlockCsr*>& f, int prec, DevTri& t) {
+  build_tri_parts(f, false, prec, t);
+}
 
 void SweepSync::ensure(int n, hipStream_t s) {
   if (ctl.p) return;
@@ -612,12 +654,15 @@ int SweepSync::error(hipStream_t s) {
 
 void RedWork::ensure() {
   if (partials.p) return;
-  partials.alloc((size_t)2048 * kMaxDots * 8);
-  counter.alloc(64);
-  // hipMemset is ordered on the null stream only; the solver streams are
-  // non-blocking, so the reset is completed before any of them can run.
-  BCM_HIP(hipMemset(counter.p, 0, 64));
-  BCM_HIP(hipDeviceSynchronize());
+  partials.alloc((size_t)kRedBlocks * kMaxDots * 8);
+}
+
+void launch_finalize(RedWork& rw, double* out, hipStream_t s, const int* skip) {
+  if (rw.pending_k < 1) throw Error(kErrInternal, "finalize: no pending reduction");
+  hipLaunchKernelGGL(k_finalize, dim3(rw.pending_k), dim3(256), 0, s, rw.partials.as<double>(), rw.pending_blocks,
+                     rw.pending_k, out, skip);
+  BCM_HIP(hipGetLastError());
+  rw.pending_k = 0;
 }
 
 // ---------------------------------------------------------------------------
@@ -639,11 +684,20 @@ static void spmv_b(const DevBsr& A, const double* x, double* y, const double* bs
   const int nwaves = (A.nsup + rows_per_wave - 1) / rows_per_wave;
   int grid = (nwaves + kWavesPerBlock - 1) / kWavesPerBlock;
   if (grid > 8) grid = (grid + 7) / 8 * 8;  // whole XCD rounds, so the tile map is a permutation
-  if (dot_out) rw.ensure();
+  double* partials = nullptr;
+  if (dot_out) {
+    rw.ensure();
+    if (grid > kRedBlocks * kMaxDots) throw Error(kErrInternal, "spmv: too many blocks for the fused dot");
+    partials = rw.partials.as<double>();
+  }
   hipLaunchKernelGGL((k_spmv<B, spmv_width<B, NR>(), NR, TA>), dim3(grid), dim3(64 * kWavesPerBlock), 0, s, A.nsup, A.n,
-                     A.ia.as<int>(), A.ja.as<int>(), A.a.as<TA>(), x, y, bsub, wdot,
-                     rw.partials.as<double>(), rw.counter.as<unsigned>(), dot_out, rows_per_wave, skip);
+                     A.ia.as<int>(), A.ja.as<int>(), A.a.as<TA>(), x, y, bsub, wdot, partials, rows_per_wave, skip);
   BCM_HIP(hipGetLastError());
+  if (dot_out) {
+    rw.pending_blocks = grid;
+    rw.pending_k = 1;
+    if (dot_out != kDeferred) launch_finalize(rw, dot_out, s, skip);
+  }
 }
 
 template <int B, int NR>
@@ -691,7 +745,7 @@ static void sweep_b(const DevTri& T, const void* x, void* y, hipStream_t s, cons
     auto kern = k_sweep_free<B, NR, TA, TX>;
     int per_cu = 0;
     BCM_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, 64, 0));
-    const int waves = std::max(1, std::min(T.nsup, std::min(per_cu, 8) * device_cu_count()));
+    const int waves = std::max(1, std::min(T.nsup, std::min(per_cu, sync->waves_per_cu) * device_cu_count()));
     const int64_t ny = (int64_t)T.n * NR;
     TX* yy = static_cast<TX*>(y);
     hipLaunchKernelGGL((k_sweep_fill<TX>), dim3(std::max<int64_t>(1, std::min<int64_t>((ny + 255) / 256, 2048))),
@@ -704,7 +758,10 @@ static void sweep_b(const DevTri& T, const void* x, void* y, hipStream_t s, cons
     const TX* xx = static_cast<const TX*>(x);
     int* ct = sync->ctl.as<int>();
     void* args[] = {&nsup, &n, &fwd, &ia, &ja, &off, &dg, &xx, &yy, &ct, (void*)&skip};
-    BCM_HIP(hipLaunchCooperativeKernel(reinterpret_cast<const void*>(kern), dim3(waves), dim3(64), args, 0, s));
+    if (sync->cooperative)
+      BCM_HIP(hipLaunchCooperativeKernel(reinterpret_cast<const void*>(kern), dim3(waves), dim3(64), args, 0, s));
+    else
+      BCM_HIP(hipLaunchKernel(reinterpret_cast<const void*>(kern), dim3(waves), dim3(64), args, 0, s));
     return;
   }
   const int* rows = T.level_rows.as<int>();
@@ -764,30 +821,32 @@ void launch_multidot(int n, int k, const double* const* xs, const double* const*
   rw.ensure();
   DotArgs args{};
   for (int i = 0; i < k; ++i) { args.x[i] = xs[i]; args.y[i] = ys[i]; }
-  const int grid = std::min(grid_for(n, 256), 1024);
+  const int grid = std::min(grid_for(n, 256), kRedBlocks);
   double* p = rw.partials.as<double>();
-  unsigned* c = rw.counter.as<unsigned>();
   switch (k) {
-    case 1: hipLaunchKernelGGL(k_multidot<1>, dim3(grid), dim3(256), 0, s, (int64_t)n, args, p, c, out, skip); break;
-    case 2: hipLaunchKernelGGL(k_multidot<2>, dim3(grid), dim3(256), 0, s, (int64_t)n, args, p, c, out, skip); break;
-    case 3: hipLaunchKernelGGL(k_multidot<3>, dim3(grid), dim3(256), 0, s, (int64_t)n, args, p, c, out, skip); break;
-    case 4: hipLaunchKernelGGL(k_multidot<4>, dim3(grid), dim3(256), 0, s, (int64_t)n, args, p, c, out, skip); break;
-    case 5: hipLaunchKernelGGL(k_multidot<5>, dim3(grid), dim3(256), 0, s, (int64_t)n, args, p, c, out, skip); break;
-    case 6: hipLaunchKernelGGL(k_multidot<6>, dim3(grid), dim3(256), 0, s, (int64_t)n, args, p, c, out, skip); break;
-    case 7: hipLaunchKernelGGL(k_multidot<7>, dim3(grid), dim3(256), 0, s, (int64_t)n, args, p, c, out, skip); break;
-    default: hipLaunchKernelGGL(k_multidot<8>, dim3(grid), dim3(256), 0, s, (int64_t)n, args, p, c, out, skip); break;
+    case 1: hipLaunchKernelGGL(k_multidot<1>, dim3(grid), dim3(256), 0, s, (int64_t)n, args, p, skip); break;
+    case 2: hipLaunchKernelGGL(k_multidot<2>, dim3(grid), dim3(256), 0, s, (int64_t)n, args, p, skip); break;
+    case 3: hipLaunchKernelGGL(k_multidot<3>, dim3(grid), dim3(256), 0, s, (int64_t)n, args, p, skip); break;
+    case 4: hipLaunchKernelGGL(k_multidot<4>, dim3(grid), dim3(256), 0, s, (int64_t)n, args, p, skip); break;
+    case 5: hipLaunchKernelGGL(k_multidot<5>, dim3(grid), dim3(256), 0, s, (int64_t)n, args, p, skip); break;
+    case 6: hipLaunchKernelGGL(k_multidot<6>, dim3(grid), dim3(256), 0, s, (int64_t)n, args, p, skip); break;
+    case 7: hipLaunchKernelGGL(k_multidot<7>, dim3(grid), dim3(256), 0, s, (int64_t)n, args, p, skip); break;
+    default: hipLaunchKernelGGL(k_multidot<8>, dim3(grid), dim3(256), 0, s, (int64_t)n, args, p, skip); break;
   }
   BCM_HIP(hipGetLastError());
+  rw.pending_blocks = grid;
+  rw.pending_k = k;
+  if (out != kDeferred) launch_finalize(rw, out, s, skip);
 }
 
-void launch_convert(int64_t n, const void* src, int sp, void* dst, int dp, hipStream_t s) {
+void launch_convert(int64_t n, const void* src, int sp, void* dst, int dp, hipStream_t s, const int* skip) {
   const int grid = grid_for(n, 256);
   if (sp == kF64 && dp == kF32)
-    hipLaunchKernelGGL((k_convert<double, float>), dim3(grid), dim3(256), 0, s, n, (const double*)src, (float*)dst);
+    hipLaunchKernelGGL((k_convert<double, float>), dim3(grid), dim3(256), 0, s, n, (const double*)src, (float*)dst,
+                       skip);
   else if (sp == kF32 && dp == kF64)
-    hipLaunchKernelGGL((k_convert<float, double>), dim3(grid), dim3(256), 0, s, n, (const float*)src, (double*)dst);
-  else if (sp == dp)
-    BCM_HIP(hipMemcpyAsync(dst, src, (size_t)n * prec_bytes(sp), hipMemcpyDeviceToDevice, s));
+    hipLaunchKernelGGL((k_convert<float, double>), dim3(grid), dim3(256), 0, s, n, (const float*)src, (double*)dst,
+                       skip);
   else
     throw Error(kErrArg, "convert: unsupported precision pair");
   BCM_HIP(hipGetLastError());

@@ -55,10 +55,20 @@ __global__ void k_state_reset(KState* ks, int keep_scalars) {
   if (t < 8) ks->st[t] = 0;
 }
 
-__global__ void k_scalar(int op, KState* ks, double eps2, int ittype, int niter) {
+// One-block scalar step. It first adds the pending per-block partials of the
+// preceding reduction (K values, stored from slot0 on), then thread 0 runs
+// the recurrence update `op`.
+__global__ __launch_bounds__(256) void k_scalar(int op, KState* ks, double eps2, int ittype, int niter,
+                                                const double* partials, int nb, int K, int slot0, int stamp) {
   double* sc = ks->sc;
   int* st = ks->st;
   if (st[0]) return;
+  double v[kMaxDots];
+#pragma unroll
+  for (int k = 0; k < kMaxDots; ++k)
+    if (k < K) v[k] = sum_partials(partials, nb, K, k);
+  if (threadIdx.x != 0) return;
+  for (int k = 0; k < K && k < kMaxDots; ++k) sc[slot0 + k] = v[k];
   switch (op) {
     case kOpPcgCheck: {
       const int k = ++st[1];
@@ -77,12 +87,12 @@ __global__ void k_scalar(int op, KState* ks, double eps2, int ittype, int niter)
       sc[kGamold] = sc[kGamma];
       break;
     }
-    case kOpBicgInit: {
-      sc[kRho] = sc[kRhoNew];
+    case kOpBicgInit: {  // slots 20..22 hold b.b, r0.r0, rhat.r0
+      sc[kRr] = sc[kR0norm2];
+      sc[kRho] = sc[kTrue0];
       sc[kRhoOld] = 1.0;
       sc[kBAlpha] = 1.0;
       sc[kOmega] = 1.0;
-      sc[kR0norm2] = sc[kRr];
       if (!(sc[kRr] > threshold(ks, ittype, eps2))) { st[0] = 1; st[4] = 1; }
       break;
     }
@@ -92,8 +102,8 @@ __global__ void k_scalar(int op, KState* ks, double eps2, int ittype, int niter)
       sc[kBAlpha] = a;
       break;
     }
-    case kOpBicgSCheck: {
-      if (!(sc[kSs] > threshold(ks, ittype, eps2))) { st[0] = 1; st[3] = 1; st[4] = 1; ++st[1]; }
+    case kOpBicgSCheck: {  // converged on s: the x update by alpha alone (k_bicg_xs with this stamp) ends it
+      if (!(sc[kSs] > threshold(ks, ittype, eps2))) { st[0] = 1; st[3] = stamp; st[4] = 1; ++st[1]; }
       break;
     }
     case kOpBicgOmega: {
@@ -102,10 +112,10 @@ __global__ void k_scalar(int op, KState* ks, double eps2, int ittype, int niter)
       sc[kOmega] = w;
       break;
     }
-    case kOpBicgNext: {
+    case kOpBicgNext: {  // slots kRr, kRr + 1 hold r.r and rhat.r
       const int k = ++st[1];
       sc[kRhoOld] = sc[kRho];
-      sc[kRho] = sc[kRhoNew];
+      sc[kRho] = sc[kRr + 1];
       if (!(sc[kRr] > threshold(ks, ittype, eps2))) { st[0] = 1; st[4] = 1; return; }
       if (k >= niter) { st[0] = 1; return; }
       if (sc[kRho] == 0.0 || !std::isfinite(sc[kRho])) { st[0] = 1; st[2] = 1; return; }
@@ -146,9 +156,9 @@ __global__ void k_bicg_p(int64_t n, const KState* ks, const double* __restrict__
     p[i] = r[i] + beta * (p[i] - w * v[i]);
 }
 
-__global__ __launch_bounds__(256) void k_bicg_s(int64_t n, KState* ks, const double* __restrict__ r,
+__global__ __launch_bounds__(256) void k_bicg_s(int64_t n, const KState* ks, const double* __restrict__ r,
                                                 const double* __restrict__ v, double* __restrict__ s,
-                                                double* partials, unsigned* counter) {
+                                                double* partials) {
   if (ks->st[0]) return;
   const double a = ks->sc[kBAlpha];
   double acc[1] = {0.0};
@@ -158,23 +168,22 @@ __global__ __launch_bounds__(256) void k_bicg_s(int64_t n, KState* ks, const dou
     s[i] = si;
     acc[0] += si * si;
   }
-  reduce_finalize<1>(acc, partials, counter, &ks->sc[kSs]);
+  block_partials<1>(acc, partials);
 }
 
-__global__ void k_bicg_xs(int64_t n, const KState* ks, double thr_eps2, int ittype, const double* __restrict__ phat,
+__global__ void k_bicg_xs(int64_t n, const KState* ks, int stamp, const double* __restrict__ phat,
                           double* __restrict__ x) {
-  if (ks->st[0]) return;
-  if (ks->sc[kSs] > threshold(ks, ittype, thr_eps2)) return;
+  if (ks->st[3] != stamp) return;  // only in the iteration that converged on s
   const double a = ks->sc[kBAlpha];
   const int64_t stride = (int64_t)gridDim.x * blockDim.x;
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) x[i] += a * phat[i];
 }
 
-__global__ __launch_bounds__(256) void k_bicg_xr(int64_t n, KState* ks, const double* __restrict__ phat,
+__global__ __launch_bounds__(256) void k_bicg_xr(int64_t n, const KState* ks, const double* __restrict__ phat,
                                                  const double* __restrict__ shat, const double* __restrict__ s,
                                                  const double* __restrict__ t, const double* __restrict__ rhat,
-                                                 double* __restrict__ x, double* __restrict__ r, double* partials,
-                                                 unsigned* counter) {
+                                                 double* __restrict__ x, double* __restrict__ r,
+                                                 double* partials) {
   if (ks->st[0]) return;
   const double a = ks->sc[kBAlpha], w = ks->sc[kOmega];
   double acc[2] = {0.0, 0.0};
@@ -186,15 +195,28 @@ __global__ __launch_bounds__(256) void k_bicg_xr(int64_t n, KState* ks, const do
     acc[0] += ri * ri;
     acc[1] += rhat[i] * ri;
   }
-  // finalises into sc[kRr] and sc[kRr + 1]; kRhoNew is copied by the scalar step
-  reduce_finalize<2>(acc, partials, counter, &ks->sc[kRr]);
+  block_partials<2>(acc, partials);  // r.r and rhat.r, added by the next scalar step
 }
-
-__global__ void k_move(KState* ks, int from, int to) { ks->sc[to] = ks->sc[from]; }
 
 int ew_grid(int64_t n) {
   int64_t g = (n + 255) / 256;
   return (int)std::max<int64_t>(1, std::min<int64_t>(g, 1024));
+}
+
+// Launches the scalar step `op`, consuming the pending partials of ctx.rw
+// into sc[slot0..].
+void scalar_step(Context& ctx, int op, double eps2, int ittype, int niter, int slot0, int stamp = 0) {
+  RedWork& rw = ctx.rw;
+  const int K = rw.pending_k, nb = rw.pending_blocks;
+  rw.pending_k = 0;
+  hipLaunchKernelGGL(k_scalar, dim3(1), dim3(256), 0, ctx.stream, op, ctx.dstate(), eps2, ittype, niter,
+                     rw.partials.as<double>(), nb, K, slot0, stamp);
+}
+
+// Marks the partials a solver vector kernel left in ctx.rw as pending.
+void set_pending(Context& ctx, int blocks, int k) {
+  ctx.rw.pending_blocks = blocks;
+  ctx.rw.pending_k = k;
 }
 
 }  // namespace
@@ -219,23 +241,25 @@ void Matrix::to_device(bool with_transpose) {
   }
 }
 
-void Factor::to_device(int prec, bool with_transpose) {
-  if (dev_prec != prec) {
-    dev_tri_forward(lower(), prec, fwd);
-    dev_tri_backward(u, prec, bwd);
-    dev_prec = prec;
-    has_t = false;
+const Factor::Image& Factor::to_device(int prec, bool with_transpose) {
+  if (prec < kF64 || prec > kBF16) throw Error(kErrArg, "unknown factor precision");
+  Image& im = img[prec];
+  if (!im.has) {
+    dev_tri_forward(lower(), prec, im.fwd);
+    dev_tri_backward(u, prec, im.bwd);
+    im.has = true;
   }
-  if (with_transpose && !has_t) {
+  if (with_transpose && !im.has_t) {
     if (kind == 0) {
-      dev_tri_forward(u, prec, fwdt);
-      dev_tri_backward(u, prec, bwdt);
+      dev_tri_forward(u, prec, im.fwdt);
+      dev_tri_backward(u, prec, im.bwdt);
     } else {
-      dev_tri_forward(u, prec, fwdt);
-      dev_tri_backward(l, prec, bwdt);
+      dev_tri_forward(u, prec, im.fwdt);
+      dev_tri_backward(l, prec, im.bwdt);
     }
-    has_t = true;
+    im.has_t = true;
   }
+  return im;
 }
 
 Context::Context() {
@@ -305,11 +329,8 @@ void Context::prof_flush() {
   pending.clear();
 }
 
-void precond_apply(Factor& F, Context& ctx, const double* r, double* z, int prec, bool transpose, int nrhs,
-                   const int* skip) {
-  F.to_device(prec, transpose);
-  const DevTri& t1 = transpose ? F.fwdt : F.fwd;
-  const DevTri& t2 = transpose ? F.bwdt : F.bwd;
+void precond_sweeps(const DevTri& t1, const DevTri& t2, Context& ctx, const double* r, double* z, int prec, int nrhs,
+                    const int* skip) {
   const int64_t n = t1.n;
   if (prec == kF64) {
     double* tmp = ctx.vec(12, (size_t)n * nrhs * 8);
@@ -322,7 +343,7 @@ void precond_apply(Factor& F, Context& ctx, const double* r, double* z, int prec
   } else {
     float* r32 = reinterpret_cast<float*>(ctx.vec(13, (size_t)n * nrhs * 4));
     float* t32 = reinterpret_cast<float*>(ctx.vec(14, (size_t)n * nrhs * 4));
-    launch_convert(n * nrhs, r, kF64, r32, kF32, ctx.stream);
+    launch_convert(n * nrhs, r, kF64, r32, kF32, ctx.stream, skip);
     {
       ProfScope ps(ctx, kProfSweep, sweep_bytes(t1, nrhs));
       launch_sweep(t1, r32, t32, kF32, nrhs, ctx.stream, skip, ctx.sweep_sync());
@@ -331,8 +352,14 @@ void precond_apply(Factor& F, Context& ctx, const double* r, double* z, int prec
       ProfScope ps(ctx, kProfSweep, sweep_bytes(t2, nrhs));
       launch_sweep(t2, t32, r32, kF32, nrhs, ctx.stream, skip, ctx.sweep_sync());
     }
-    launch_convert(n * nrhs, r32, kF32, z, kF64, ctx.stream);
+    launch_convert(n * nrhs, r32, kF32, z, kF64, ctx.stream, skip);
   }
+}
+
+void precond_apply(Factor& F, Context& ctx, const double* r, double* z, int prec, bool transpose, int nrhs,
+                   const int* skip) {
+  const Factor::Image& im = F.to_device(prec, transpose);
+  precond_sweeps(transpose ? im.fwdt : im.fwd, transpose ? im.bwdt : im.bwd, ctx, r, z, prec, nrhs, skip);
 }
 
 // ---------------------------------------------------------------------------
@@ -387,8 +414,8 @@ void pcg(Matrix& A, Factor& F, Context& ctx, const SolveOptions& opt, const doub
         precond_apply(F, ctx, r, z, opt.prec, false, 1, done);
         const double* xs[2] = {r, r};
         const double* ys[2] = {r, z};
-        launch_multidot(n, 2, xs, ys, &ks->sc[kEnrsq], ctx.rw, s, done);  // (r,r) -> 8, (r,z) -> 9
-        hipLaunchKernelGGL(k_scalar, dim3(1), dim3(1), 0, s, kOpPcgCheck, ks, eps2, opt.ittype, opt.niter - total);
+        launch_multidot(n, 2, xs, ys, kDeferred, ctx.rw, s, done);
+        scalar_step(ctx, kOpPcgCheck, eps2, opt.ittype, opt.niter - total, kEnrsq);  // (r,r) -> 8, (r,z) -> 9
         if (exact) {  // no launch is queued past the convergence test
           ctx.pull_state();
           if (ctx.host_state->st[0]) break;
@@ -396,9 +423,9 @@ void pcg(Matrix& A, Factor& F, Context& ctx, const SolveOptions& opt, const doub
         hipLaunchKernelGGL(k_pcg_p, dim3(ew_grid(n)), dim3(256), 0, s, (int64_t)n, ks, z, p);
         {
           ProfScope ps(ctx, kProfSpmv, spmv_bytes(A.d, 1));
-          launch_spmv(A.d, p, q, nullptr, 1, &ks->sc[kPq], p, ctx.rw, s, done);
+          launch_spmv(A.d, p, q, nullptr, 1, kDeferred, p, ctx.rw, s, done);
         }
-        hipLaunchKernelGGL(k_scalar, dim3(1), dim3(1), 0, s, kOpPcgAlpha, ks, eps2, opt.ittype, opt.niter - total);
+        scalar_step(ctx, kOpPcgAlpha, eps2, opt.ittype, opt.niter - total, kPq);
         hipLaunchKernelGGL(k_pcg_xr, dim3(ew_grid(n)), dim3(256), 0, s, (int64_t)n, ks, p, q, x, r);
         ++it;
       }
@@ -438,6 +465,7 @@ void bicgstab(Matrix& A, Factor& F, Context& ctx, const SolveOptions& opt, const
   hipStream_t s = ctx.stream;
   const int ge = ew_grid(n);
   int total = 0;
+  int stamp = 0;  // identifies each queued iteration (k_bicg_xs acts only for the one that converged on s)
   for (int attempt = 0; attempt <= opt.max_restarts; ++attempt) {
     hipLaunchKernelGGL(k_state_reset, dim3(1), dim3(64), 0, s, ks, 0);
     launch_spmv(A.d, x, r, b, 1, nullptr, nullptr, ctx.rw, s);
@@ -447,25 +475,24 @@ void bicgstab(Matrix& A, Factor& F, Context& ctx, const SolveOptions& opt, const
     {
       const double* xs[3] = {b, r, rhat};
       const double* ys[3] = {b, r, r};
-      launch_multidot(n, 3, xs, ys, &ks->sc[kBnorm2], ctx.rw, s);  // 20: b.b 21: r.r 22: rhat.r
-      hipLaunchKernelGGL(k_move, dim3(1), dim3(1), 0, s, ks, kR0norm2, kRr);
-      hipLaunchKernelGGL(k_move, dim3(1), dim3(1), 0, s, ks, 22, kRhoNew);
-      hipLaunchKernelGGL(k_scalar, dim3(1), dim3(1), 0, s, kOpBicgInit, ks, eps2, opt.ittype, opt.niter);
+      launch_multidot(n, 3, xs, ys, kDeferred, ctx.rw, s);
+      scalar_step(ctx, kOpBicgInit, eps2, opt.ittype, opt.niter, kBnorm2);  // 20: b.b 21: r.r 22: rhat.r
     }
     const bool exact = opt.check_every == 1;
     while (true) {
       for (int c = 0; c < opt.check_every; ++c) {
+        ++stamp;
         hipLaunchKernelGGL(k_bicg_p, dim3(ge), dim3(256), 0, s, (int64_t)n, ks, r, v, p);
         precond_apply(F, ctx, p, phat, opt.prec, false, 1, done);
         {
           ProfScope ps(ctx, kProfSpmv, spmv_bytes(A.d, 1));
-          launch_spmv(A.d, phat, v, nullptr, 1, &ks->sc[kRv], rhat, ctx.rw, s, done);
+          launch_spmv(A.d, phat, v, nullptr, 1, kDeferred, rhat, ctx.rw, s, done);
         }
-        hipLaunchKernelGGL(k_scalar, dim3(1), dim3(1), 0, s, kOpBicgAlpha, ks, eps2, opt.ittype, opt.niter);
-        hipLaunchKernelGGL(k_bicg_s, dim3(ge), dim3(256), 0, s, (int64_t)n, ks, r, v, sv,
-                           ctx.rw.partials.as<double>(), ctx.rw.counter.as<unsigned>());
-        hipLaunchKernelGGL(k_bicg_xs, dim3(ge), dim3(256), 0, s, (int64_t)n, ks, eps2, opt.ittype, phat, x);
-        hipLaunchKernelGGL(k_scalar, dim3(1), dim3(1), 0, s, kOpBicgSCheck, ks, eps2, opt.ittype, opt.niter);
+        scalar_step(ctx, kOpBicgAlpha, eps2, opt.ittype, opt.niter, kRv);
+        hipLaunchKernelGGL(k_bicg_s, dim3(ge), dim3(256), 0, s, (int64_t)n, ks, r, v, sv, ctx.rw.partials.as<double>());
+        set_pending(ctx, ge, 1);
+        scalar_step(ctx, kOpBicgSCheck, eps2, opt.ittype, opt.niter, kSs, stamp);
+        hipLaunchKernelGGL(k_bicg_xs, dim3(ge), dim3(256), 0, s, (int64_t)n, ks, stamp, phat, x);
         if (exact) {  // no launch is queued past a convergence test
           ctx.pull_state();
           if (ctx.host_state->st[0]) break;
@@ -478,14 +505,13 @@ void bicgstab(Matrix& A, Factor& F, Context& ctx, const SolveOptions& opt, const
         {
           const double* xs[2] = {t, t};
           const double* ys[2] = {sv, t};
-          launch_multidot(n, 2, xs, ys, &ks->sc[kTs], ctx.rw, s, done);
+          launch_multidot(n, 2, xs, ys, kDeferred, ctx.rw, s, done);
         }
-        hipLaunchKernelGGL(k_scalar, dim3(1), dim3(1), 0, s, kOpBicgOmega, ks, eps2, opt.ittype, opt.niter);
+        scalar_step(ctx, kOpBicgOmega, eps2, opt.ittype, opt.niter, kTs);
         hipLaunchKernelGGL(k_bicg_xr, dim3(ge), dim3(256), 0, s, (int64_t)n, ks, phat, shat, sv, t, rhat, x, r,
-                           ctx.rw.partials.as<double>(), ctx.rw.counter.as<unsigned>());
-        hipLaunchKernelGGL(k_move, dim3(1), dim3(1), 0, s, ks, kRr + 1, kRhoNew);
-        hipLaunchKernelGGL(k_scalar, dim3(1), dim3(1), 0, s, kOpBicgNext, ks, eps2, opt.ittype,
-                           opt.niter - total);
+                           ctx.rw.partials.as<double>());
+        set_pending(ctx, ge, 2);
+        scalar_step(ctx, kOpBicgNext, eps2, opt.ittype, opt.niter - total, kRr);
         if (exact) {
           ctx.pull_state();
           if (ctx.host_state->st[0]) break;

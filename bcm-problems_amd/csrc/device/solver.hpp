@@ -28,10 +28,13 @@ struct Factor {
   BlockCsr l, u;           // L by columns, U by rows, inverted diagonal blocks
   FactorStats stats;
   std::vector<int> order;  // ordering the factor was computed in (order[old] = new), may be empty
-  DevTri fwd, bwd, fwdt, bwdt;
-  int dev_prec = -1;
-  bool has_t = false;
-  void to_device(int prec, bool with_transpose);
+  // One device image per storage precision (fp64, fp32, bf16), built on first use.
+  struct Image {
+    DevTri fwd, bwd, fwdt, bwdt;
+    bool has = false, has_t = false;
+  };
+  Image img[3];
+  const Image& to_device(int prec, bool with_transpose);
   const BlockCsr& lower() const { return kind == 0 ? u : l; }
 };
 
@@ -136,9 +139,68 @@ void solve(Matrix& A, Factor& F, Context& ctx, const SolveOptions& opt, const do
 void precond_apply(Factor& F, Context& ctx, const double* r, double* z, int prec, bool transpose, int nrhs,
                    const int* skip = nullptr);
 
-// FGMRES(m) driver (fgmres.cpp).
+// z = M^-1 r through the sweep pair (t1 then t2) in the given precision
+// (fp64 vectors, or fp32 vectors converted on the way in and out).
+void precond_sweeps(const DevTri& t1, const DevTri& t2, Context& ctx, const double* r, double* z, int prec, int nrhs,
+                    const int* skip = nullptr);
+
+// FGMRES(m) driver (fgmres.hip).
 void solve_fgmres(Matrix& A, Factor& F, Context& ctx, const SolveOptions& opt, const double* b, double* x,
                   SolveStats& st);
+
+// ---------------------------------------------------------------------------
+// Batched solves: many independent cells held as one block-diagonal union on
+// the device, nrhs right-hand sides per cell, and one launch sequence for all
+// (cell, right-hand side) pairs, each pair with its own recurrence scalars.
+
+// Schedule of a batched solve, picked by batch_tune.
+struct BatchTune {
+  int sweep_mode = 1;     // 0: level-scheduled sweeps, 1: sync-free sweeps
+  int waves_per_cu = 8;   // sync-free sweeps: resident waves per CU
+  int graph = 0;          // 1: replay check_every iterations as one captured hipGraph
+  int check_every = 8;    // iterations between host polls of the done flag
+  double ms_per_iter = 0.0;
+};
+
+struct BatchGraph;  // captured iteration block (bpcg.hip)
+
+struct Batch {
+  int ncell = 0, b = 0, kind = 0;
+  int64_t n = 0;                      // union length (elements per right-hand side)
+  std::vector<int64_t> elem0;         // ncell + 1 element offsets of the cells
+  DevBsr A;                           // union of the cell matrices (full storage, fp64)
+  Factor::Image img[3];               // union sweeps per factor precision
+  int nchunks = 0;                    // reduction chunks (each inside one cell)
+  std::vector<int> cell_chunk0;       // ncell + 1
+  DevBuf chunk_cell, chunk_lo, chunk_hi, dcell_chunk0;
+  BatchTune tune;
+  bool tuned = false;
+  std::unique_ptr<BatchGraph> graph;
+  Batch();
+  ~Batch();
+  void drop_graph();  // after a schedule change
+};
+
+struct BatchStats {
+  int pairs = 0, converged = 0, breakdowns = 0, iters_max = 0, restarts = 0;
+  long long iters_sum = 0;
+  double relres_max = 0.0, seconds = 0.0;
+};
+
+// Builds the device union of the cells' matrices and factors (all IC2 or all
+// ILU2, one block size). prec_mask: bit p uploads the factor sweeps in
+// precision p (fp64 = bit 0, fp32 = bit 1, bf16 = bit 2).
+void batch_build(Batch& bt, const std::vector<const Matrix*>& mats, const std::vector<const Factor*>& facts,
+                 int prec_mask);
+// Block PCG over all pairs. b, x: device, union layout, nrhs columns of
+// length bt.n; x holds x0. With opt_fallback, a pair whose ||b - A x0|| >=
+// ||b|| restarts from x0 = 0. iters/relres (host, ncell * nrhs, cell-major)
+// may be null.
+void batch_solve(Batch& bt, Context& ctx, const SolveOptions& opt, int nrhs, const double* b, double* x,
+                 bool fallback, BatchStats& st, int* iters, double* relres);
+// Times the candidate schedules on this batch (a few iterations each, on
+// scratch copies of b) and keeps the fastest in bt.tune.
+BatchTune batch_tune(Batch& bt, Context& ctx, const SolveOptions& opt, int nrhs, const double* b);
 
 // Two-sided Lanczos driver (lanczos.hip).
 void solve_lanczos(Matrix& A, Factor& F, Context& ctx, const SolveOptions& opt, const double* b, double* x,

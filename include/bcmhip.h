@@ -30,6 +30,7 @@ extern "C" {
 #define BCM_BICGSTAB 1   /* no reference counterpart (nonsymmetric drop-in) */
 #define BCM_FGMRES 2     /* flexible GMRES(m); reference block GMRES is gmres.cpp:337 */
 #define BCM_LANCZOS 3    /* CSVector::Lanczos, lanczos.cpp:34-371 */
+#define BCM_BLOCKPCG 5   /* batched PCG over many cells and right-hand sides (bcm_batch_solve) */
 
 #define BCM_F64 0
 #define BCM_F32 1
@@ -162,6 +163,44 @@ int bcm_dot(bcm_context c, int n, const double* x, const double* y, double* resu
 /* ---- GPU solve: x (device) holds the initial guess on entry ---- */
 int bcm_solve(bcm_context c, bcm_matrix a, bcm_factor f, const bcm_solve_opts* o, const double* b, double* x,
               bcm_stats* st);
+
+/* ---- batched solves: many independent cells (block-diagonal union on the
+ *      device), nrhs right-hand sides each; one launch sequence for every
+ *      (cell, right-hand side) pair. Block counterpart of the reference's
+ *      per-system Cg (cg.cpp:31-349) as the homogenization drivers call it
+ *      once per cell and load case. ---- */
+typedef struct bcm_batch_s* bcm_batch;
+typedef struct bcm_batch_stats {
+  int pairs, converged, breakdowns, iters_max, restarts;
+  long long iters_sum;
+  double relres_max, seconds;
+} bcm_batch_stats;
+typedef struct bcm_batch_tune {
+  int sweep_mode;    /* 0 level-scheduled sweeps, 1 sync-free sweeps */
+  int waves_per_cu;  /* sync-free sweeps: resident waves per CU */
+  int graph;         /* 1: replay check_every iterations as one captured hipGraph */
+  int check_every;   /* iterations between host polls */
+  double ms_per_iter;
+} bcm_batch_tune;
+/* Builds the union of ncell matrices and their IC2 factors (one block size).
+ * prec_mask: bit p keeps the factor sweeps in precision p (bit 0 F64, 1 F32, 2 BF16). */
+int bcm_batch_create(int ncell, const bcm_matrix* mats, const bcm_factor* facts, int prec_mask, bcm_batch* out);
+int bcm_batch_destroy(bcm_batch b);
+/* Union length n and the element offset of every cell (elem0: ncell + 1 entries, may be NULL). */
+int bcm_batch_layout(bcm_batch b, int* ncell, long long* n, long long* elem0);
+/* Block PCG for all pairs; rhs and x are device arrays of n x nrhs (column q
+ * holds every cell's q-th right-hand side at the cell's offsets); x holds x0.
+ * o->precision selects the factor sweeps (F32/BF16: flexible CG with an FP64
+ * outer recurrence). fallback != 0: a pair with ||b - A x0|| >= ||b||
+ * restarts from x0 = 0. iters/relres: host arrays of ncell * nrhs entries
+ * (cell-major) or NULL. nrhs is 1, 3 or 6. */
+int bcm_batch_solve(bcm_context c, bcm_batch b, const bcm_solve_opts* o, int nrhs, const double* rhs, double* x,
+                    int fallback, bcm_batch_stats* st, int* iters, double* relres);
+/* Times the candidate schedules on this batch and keeps the fastest. */
+int bcm_batch_autotune(bcm_context c, bcm_batch b, const bcm_solve_opts* o, int nrhs, const double* rhs,
+                       bcm_batch_tune* out);
+int bcm_batch_get_tune(bcm_batch b, bcm_batch_tune* t);
+int bcm_batch_set_tune(bcm_batch b, const bcm_batch_tune* t);
 
 /* ---- drop-in entry points (callback ABI of smatrix.h:15-21; C++ references
  *      are passed as pointers, which is the same machine-level convention) ---- */

@@ -1,0 +1,151 @@
+"""Batched block PCG over uneven elasticity cells x 6 load cases against the oracle.
+
+Parity bar: every (cell, load case) pair of the batched FP64 solve matches
+the oracle's Cg on that cell alone (cg.cpp:31-349 restated in the oracle):
+iteration count within one (the batched dots add chunk partials in a
+different order) and relative solution difference <= 1e-8; FP32/BF16 factor
+sweeps (flexible CG, FP64 outer recurrence) reach FP64 true relative
+residual <= 1e-10 and stay within 1e-8 of the FP64 oracle solution. All
+launch schedules (level-scheduled or sync-free sweeps, eager or hipGraph
+replay) give bit-identical results.
+"""
+
+import numpy as np
+import pytest
+
+from conftest import rel
+
+pytestmark = pytest.mark.gpu
+
+SIZES = [3, 5, 4, 2, 6]  # uneven periodic cells (m^3 block rows of 24)
+
+
+def cell(B, O, m, seed):
+    rng = np.random.default_rng(seed)
+    radius, contrast, nu = rng.uniform(0.1, 0.45), 10 ** rng.uniform(0, 3), rng.uniform(0.2, 0.45)
+    mat, rhs = B.gen_elastic_cell(m, radius, contrast, nu, 0.0, 20261205 + m)
+    order = mat.rcm_order()
+    mo = mat.reorder(order)
+    f = B.Factor.ic2(mo)
+    b = mo.reorder_vector(order, rhs)
+    A = O.Block(*mo.arrays())
+    U = O.Block(A.sprnd, *f.arrays(1, A.nsup))
+    return dict(mo=mo, f=f, b=b, A=A, U=U)
+
+
+@pytest.fixture(scope="module")
+def cells(B, O):
+    return [cell(B, O, m, 100 + i) for i, m in enumerate(SIZES)]
+
+
+@pytest.fixture(scope="module")
+def batch(B, cells):
+    return B.Batch([(c["mo"], c["f"]) for c in cells], prec_mask=0b111)
+
+
+@pytest.fixture(scope="module")
+def oracle_sol(O, cells):
+    out = []
+    for c in cells:
+        xs, its = [], []
+        for q in range(6):
+            x, r = O.cg(c["A"], c["U"], c["b"][:, q], eps=1e-10, ittype=3)
+            xs.append(x)
+            its.append(r.updates)
+        out.append((np.stack(xs, axis=1), np.array(its)))
+    return out
+
+
+def run(B, ctx, bt, cells, nrhs=6, x0=None, fallback=False, **opts):
+    b = bt.gather([c["b"][:, :nrhs] for c in cells])
+    db = B.DeviceArray.from_host(b)
+    dx = B.DeviceArray.from_host(x0) if x0 is not None else B.DeviceArray(b.shape)
+    if x0 is None:
+        dx.zero()
+    o = B.default_opts(kind=B.BLOCKPCG, **opts)
+    st, its, rr = bt.solve(ctx, db, dx, o, nrhs, fallback)
+    return bt.scatter(dx.host()), st, its, rr
+
+
+def test_batch_layout(B, batch, cells):
+    assert batch.ncell == len(SIZES)
+    assert batch.n == sum(24 * m ** 3 for m in SIZES)
+    assert list(np.diff(batch.elem0)) == [24 * m ** 3 for m in SIZES]
+
+
+def test_batch_fp64_matches_oracle_per_pair(B, ctx, batch, cells, oracle_sol):
+    print(f"batch fp64 cells={SIZES} n={batch.n}", flush=True)
+    xs, st, its, rr = run(B, ctx, batch, cells, eps=1e-10, precision=B.F64)
+    assert st.converged == st.pairs == 6 * len(SIZES) and st.breakdowns == 0
+    assert rr.max() <= 1e-10
+    for c, (x, (xo, ito)) in enumerate(zip(xs, oracle_sol)):
+        assert np.all(np.abs(its[c] - ito) <= 1), (c, its[c], ito)
+        for q in range(6):
+            assert rel(x[:, q], xo[:, q]) < 1e-8, (c, q)
+
+
+@pytest.mark.parametrize("prec", ["F32", "BF16"])
+def test_batch_mixed_precision(B, ctx, batch, cells, oracle_sol, prec):
+    print(f"batch {prec}", flush=True)
+    xs, st, its, rr = run(B, ctx, batch, cells, eps=1e-10, precision=getattr(B, prec))
+    assert st.converged == st.pairs and rr.max() <= 1e-10
+    for c, (x, (xo, ito)) in enumerate(zip(xs, oracle_sol)):
+        assert rel(x, xo) < 1e-8, c
+        assert np.all(its[c] <= ito + (3 if prec == "F32" else 25)), (c, its[c], ito)
+
+
+def test_batch_schedules_are_bit_identical(B, ctx, batch, cells):
+    ref = None
+    for mode, graph, every in [(1, 0, 8), (0, 0, 8), (1, 1, 4), (0, 1, 16), (1, 0, 1)]:
+        print(f"schedule mode={mode} graph={graph} every={every}", flush=True)
+        batch.set_tune(sweep_mode=mode, graph=graph, check_every=every, waves_per_cu=8)
+        for prec in (B.F64, B.F32):
+            xs, st, its, rr = run(B, ctx, batch, cells, eps=1e-10, precision=prec)
+            assert st.converged == st.pairs
+            got = (np.concatenate([x.ravel() for x in xs]), its.copy())
+            if ref is None:
+                ref = {}
+            if prec in ref:
+                assert np.array_equal(got[0], ref[prec][0]) and np.array_equal(got[1], ref[prec][1]), (mode, graph)
+            else:
+                ref[prec] = got
+    batch.set_tune(sweep_mode=1, graph=0, check_every=8, waves_per_cu=8)
+
+
+def test_batch_warm_start_fallback_and_zero_rhs(B, ctx, batch, cells, oracle_sol):
+    good = batch.gather([xo * (1 + 1e-7) for xo, _ in oracle_sol])
+    xs, st, its, rr = run(B, ctx, batch, cells, x0=good, fallback=True, eps=1e-10)
+    assert st.converged == st.pairs
+    for c, (xo, ito) in enumerate(oracle_sol):
+        assert its[c].max() < ito.min(), (c, its[c], ito)
+    bad = good * -50.0  # ||b - A x0|| >= ||b||: every pair falls back to x0 = 0
+    xs, st, its2, rr = run(B, ctx, batch, cells, x0=bad, fallback=True, eps=1e-10)
+    assert st.converged == st.pairs
+    for c, (xo, ito) in enumerate(oracle_sol):
+        assert np.all(np.abs(its2[c] - ito) <= 1)
+    zero = [dict(c, b=np.zeros_like(c["b"])) for c in cells]
+    xs, st, its3, rr = run(B, ctx, batch, zero, eps=1e-10)
+    assert np.all(its3 == 0) and all(np.all(x == 0) for x in xs)
+
+
+@pytest.mark.parametrize("nrhs", [1, 3])
+def test_batch_fewer_rhs(B, ctx, batch, cells, oracle_sol, nrhs):
+    xs, st, its, rr = run(B, ctx, batch, cells, nrhs=nrhs, eps=1e-10)
+    assert st.pairs == nrhs * len(SIZES) and st.converged == st.pairs
+    for c, (x, (xo, ito)) in enumerate(zip(xs, oracle_sol)):
+        assert rel(x, xo[:, :nrhs]) < 1e-8
+
+
+def test_batch_autotune_and_errors(B, ctx, batch, cells):
+    b = batch.gather([c["b"] for c in cells])
+    db = B.DeviceArray.from_host(b)
+    t = batch.autotune(ctx, db, B.default_opts(kind=B.BLOCKPCG, precision=B.F32), 6)
+    print(f"tuned: mode={t.sweep_mode} waves={t.waves_per_cu} graph={t.graph} every={t.check_every} "
+          f"{t.ms_per_iter:.3f} ms/iter", flush=True)
+    assert t.sweep_mode in (0, 1) and t.check_every in (4, 16) and t.ms_per_iter > 0
+    dx = B.DeviceArray(b.shape)
+    dx.zero()
+    st, its, rr = batch.solve(ctx, db, dx, B.default_opts(kind=B.BLOCKPCG, precision=B.F32), 6)
+    assert st.converged == st.pairs
+    with pytest.raises(B.BcmError):
+        batch.solve(ctx, db, dx, B.default_opts(kind=B.BLOCKPCG), 2)
