@@ -85,6 +85,14 @@ class BatchTune(C.Structure):
     ]
 
 
+class GuessReport(C.Structure):
+    _fields_ = [
+        ("snapshots", C.c_int), ("modes", C.c_int), ("steps", C.c_int), ("loss_first", C.c_double),
+        ("loss_final", C.c_double), ("pod_energy", C.c_double), ("seconds_pod", C.c_double),
+        ("seconds_train", C.c_double),
+    ]
+
+
 _P = C.c_void_p
 _IP = C.POINTER(C.c_int)
 _DP = C.POINTER(C.c_double)
@@ -144,6 +152,21 @@ SIGNATURES = {
     "bcm_batch_autotune": (C.c_int, [_P, _P, C.POINTER(SolveOpts), C.c_int, C.c_void_p, C.POINTER(BatchTune)]),
     "bcm_batch_get_tune": (C.c_int, [_P, C.POINTER(BatchTune)]),
     "bcm_batch_set_tune": (C.c_int, [_P, C.POINTER(BatchTune)]),
+    "bcm_mlp_create": (C.c_int, [C.c_int, C.c_int, C.c_int, C.c_ulonglong, C.POINTER(_P)]),
+    "bcm_mlp_destroy": (C.c_int, [_P]),
+    "bcm_mlp_nparam": (C.c_int, [_P, C.POINTER(C.c_longlong)]),
+    "bcm_mlp_set_params": (C.c_int, [_P, C.c_void_p]),
+    "bcm_mlp_get_params": (C.c_int, [_P, C.c_void_p]),
+    "bcm_mlp_get_grads": (C.c_int, [_P, C.c_void_p]),
+    "bcm_mlp_forward": (C.c_int, [_P, _P, C.c_int, C.c_void_p, C.c_void_p]),
+    "bcm_mlp_backward": (C.c_int, [_P, _P, C.c_int, C.c_void_p, C.c_void_p, _DP]),
+    "bcm_mlp_adam_step": (C.c_int, [_P, _P, C.c_float]),
+    "bcm_guess_create": (C.c_int, [C.c_longlong, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_ulonglong,
+                                   C.POINTER(_P)]),
+    "bcm_guess_destroy": (C.c_int, [_P]),
+    "bcm_guess_add": (C.c_int, [_P, _P, C.c_int, _DP, C.c_void_p]),
+    "bcm_guess_fit": (C.c_int, [_P, _P, C.c_int, C.c_double, C.POINTER(GuessReport)]),
+    "bcm_guess_predict": (C.c_int, [_P, _P, C.c_int, _DP, C.c_void_p]),
     "bcm_abstract_symm_solver": (C.c_int, [C.c_void_p, C.POINTER(Params), _CB_NSUP, _CB_SPRNDS, _CB_IAJA,
                                            _CB_BLOCK, _CB_BLOCK, _CB_BLOCK, _CB_BLOCK]),
     "bcm_abstract_solver": (C.c_int, [C.c_void_p, C.POINTER(Params), _CB_NSUP, _CB_SPRNDS, _CB_IAJA,
@@ -512,4 +535,86 @@ class Batch:
     def __del__(self):
         if getattr(self, "h", None) and _lib is not None:
             _lib.bcm_batch_destroy(self.h)
+            self.h = None
+
+
+class Mlp:
+    """MLP nin -> nh (tanh) -> nh (tanh) -> nout trained on the GPU (BF16 MFMA)."""
+
+    def __init__(self, nin: int, nh: int, nout: int, seed: int = 1):
+        out = C.c_void_p()
+        _check(lib().bcm_mlp_create(nin, nh, nout, seed, C.byref(out)))
+        self.h = out.value
+        self.nin, self.nh, self.nout = nin, nh, nout
+        n = C.c_longlong()
+        _check(lib().bcm_mlp_nparam(self.h, C.byref(n)))
+        self.nparam = n.value
+
+    def set_params(self, p) -> None:
+        p = np.ascontiguousarray(p, dtype=np.float32)
+        assert p.size == self.nparam
+        _check(lib().bcm_mlp_set_params(self.h, p.ctypes.data))
+
+    def params(self) -> np.ndarray:
+        out = np.empty(self.nparam, np.float32)
+        _check(lib().bcm_mlp_get_params(self.h, out.ctypes.data))
+        return out
+
+    def grads(self) -> np.ndarray:
+        out = np.empty(self.nparam, np.float32)
+        _check(lib().bcm_mlp_get_grads(self.h, out.ctypes.data))
+        return out
+
+    def forward(self, ctx: "Context", x) -> np.ndarray:
+        x = np.ascontiguousarray(x, dtype=np.float32)
+        m = x.shape[0]
+        dx = DeviceArray.from_host(x.ravel())
+        dy = DeviceArray(m * self.nout, np.float32)
+        _check(lib().bcm_mlp_forward(ctx.h, self.h, m, ptr(dx), ptr(dy)))
+        ctx.sync()
+        return dy.host().reshape(m, self.nout)
+
+    def backward(self, ctx: "Context", x, t) -> float:
+        x = np.ascontiguousarray(x, dtype=np.float32)
+        t = np.ascontiguousarray(t, dtype=np.float32)
+        dx, dt = DeviceArray.from_host(x.ravel()), DeviceArray.from_host(t.ravel())
+        loss = C.c_double()
+        _check(lib().bcm_mlp_backward(ctx.h, self.h, x.shape[0], ptr(dx), ptr(dt), C.byref(loss)))
+        return loss.value
+
+    def adam_step(self, ctx: "Context", lr: float) -> None:
+        _check(lib().bcm_mlp_adam_step(ctx.h, self.h, lr))
+
+    def __del__(self):
+        if getattr(self, "h", None) and _lib is not None:
+            _lib.bcm_mlp_destroy(self.h)
+            self.h = None
+
+
+class Guess:
+    """Learned initial guess (POD bases + MLP) for a family of cells of one layout."""
+
+    def __init__(self, n: int, nrhs: int, nparam: int, nmodes: int = 64, nhidden: int = 256,
+                 capacity: int = 1024, seed: int = 1):
+        out = C.c_void_p()
+        _check(lib().bcm_guess_create(n, nrhs, nparam, nmodes, nhidden, capacity, seed, C.byref(out)))
+        self.h = out.value
+        self.n, self.nrhs, self.nparam = n, nrhs, nparam
+
+    def add(self, ctx: "Context", params, x) -> None:
+        params = np.ascontiguousarray(params, dtype=np.float64).reshape(-1, self.nparam)
+        _check(lib().bcm_guess_add(ctx.h, self.h, params.shape[0], _dp(params), ptr(x)))
+
+    def fit(self, ctx: "Context", steps: int = 3000, lr: float = 2e-3) -> GuessReport:
+        rep = GuessReport()
+        _check(lib().bcm_guess_fit(ctx.h, self.h, steps, lr, C.byref(rep)))
+        return rep
+
+    def predict(self, ctx: "Context", params, x0) -> None:
+        params = np.ascontiguousarray(params, dtype=np.float64).reshape(-1, self.nparam)
+        _check(lib().bcm_guess_predict(ctx.h, self.h, params.shape[0], _dp(params), ptr(x0)))
+
+    def __del__(self):
+        if getattr(self, "h", None) and _lib is not None:
+            _lib.bcm_guess_destroy(self.h)
             self.h = None

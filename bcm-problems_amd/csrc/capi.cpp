@@ -12,6 +12,7 @@
 #include <new>
 #include <string>
 
+#include "device/learn.hpp"
 #include "device/solver.hpp"
 #include "host/bcm_host.hpp"
 #include "internal.hpp"
@@ -27,6 +28,12 @@ struct bcm_context_s {
 };
 struct bcm_batch_s {
   bcm::Batch b;
+};
+struct bcm_mlp_s {
+  std::unique_ptr<bcm::Mlp> m;
+};
+struct bcm_guess_s {
+  std::unique_ptr<bcm::Guess> g;
 };
 
 namespace {
@@ -687,6 +694,123 @@ int bcm_batch_set_tune(bcm_batch b, const bcm_batch_tune* t) {
     b->b.tune.graph = t->graph ? 1 : 0;
     b->b.tune.check_every = t->check_every;
     b->b.drop_graph();
+  });
+}
+
+int bcm_mlp_create(int nin, int nhidden, int nout, unsigned long long seed, bcm_mlp* out) {
+  return guarded([&] {
+    need(out != nullptr, "mlp_create: bad argument");
+    need_device();
+    std::unique_ptr<bcm_mlp_s> m(new bcm_mlp_s());
+    m->m.reset(new bcm::Mlp(nin, nhidden, nout, seed));
+    *out = m.release();
+  });
+}
+
+int bcm_mlp_destroy(bcm_mlp m) {
+  delete m;
+  return BCM_OK;
+}
+
+int bcm_mlp_nparam(bcm_mlp m, long long* nparam) {
+  return guarded([&] {
+    need(m && nparam, "mlp_nparam: bad argument");
+    *nparam = m->m->nparam;
+  });
+}
+
+int bcm_mlp_set_params(bcm_mlp m, const float* host) {
+  return guarded([&] {
+    need(m && host, "mlp_set_params: bad argument");
+    m->m->set_params(host);
+  });
+}
+
+int bcm_mlp_get_params(bcm_mlp m, float* host) {
+  return guarded([&] {
+    need(m && host, "mlp_get_params: bad argument");
+    BCM_HIP(hipDeviceSynchronize());
+    BCM_HIP(hipMemcpy(host, m->m->params.p, m->m->nparam * 4, hipMemcpyDeviceToHost));
+  });
+}
+
+int bcm_mlp_get_grads(bcm_mlp m, float* host) {
+  return guarded([&] {
+    need(m && host, "mlp_get_grads: bad argument");
+    BCM_HIP(hipDeviceSynchronize());
+    BCM_HIP(hipMemcpy(host, m->m->grads.p, m->m->nparam * 4, hipMemcpyDeviceToHost));
+  });
+}
+
+int bcm_mlp_forward(bcm_context c, bcm_mlp m, int batch, const float* x, float* y) {
+  return guarded([&] {
+    need(c && m && x && y && batch > 0, "mlp_forward: bad argument");
+    m->m->forward(c->c.stream, batch, x, y);
+  });
+}
+
+int bcm_mlp_backward(bcm_context c, bcm_mlp m, int batch, const float* x, const float* t, double* loss) {
+  return guarded([&] {
+    need(c && m && x && t && batch > 0, "mlp_backward: bad argument");
+    double* dl = c->c.vec(60, 16);
+    m->m->backward(c->c.stream, batch, x, t, dl);
+    BCM_HIP(hipStreamSynchronize(c->c.stream));
+    if (loss) BCM_HIP(hipMemcpy(loss, dl, 8, hipMemcpyDeviceToHost));
+  });
+}
+
+int bcm_mlp_adam_step(bcm_context c, bcm_mlp m, float lr) {
+  return guarded([&] {
+    need(c && m, "mlp_adam_step: bad argument");
+    m->m->adam(c->c.stream, lr);
+    BCM_HIP(hipStreamSynchronize(c->c.stream));
+  });
+}
+
+int bcm_guess_create(long long n, int nrhs, int nparam, int nmodes, int nhidden, int capacity,
+                     unsigned long long seed, bcm_guess* out) {
+  return guarded([&] {
+    need(out != nullptr, "guess_create: bad argument");
+    need_device();
+    std::unique_ptr<bcm_guess_s> g(new bcm_guess_s());
+    g->g.reset(new bcm::Guess(n, nrhs, nparam, nmodes, nhidden, capacity, seed));
+    *out = g.release();
+  });
+}
+
+int bcm_guess_destroy(bcm_guess g) {
+  delete g;
+  return BCM_OK;
+}
+
+int bcm_guess_add(bcm_context c, bcm_guess g, int count, const double* params, const double* x) {
+  return guarded([&] {
+    need(c && g && params && x, "guess_add: bad argument");
+    g->g->add(c->c.stream, count, params, x);
+  });
+}
+
+int bcm_guess_fit(bcm_context c, bcm_guess g, int steps, double lr, bcm_guess_report* rep) {
+  return guarded([&] {
+    need(c && g && steps >= 0 && lr > 0, "guess_fit: bad argument");
+    bcm::GuessReport r = g->g->fit(c->c.stream, steps, lr);
+    if (rep) {
+      rep->snapshots = r.snapshots;
+      rep->modes = r.modes;
+      rep->steps = r.steps;
+      rep->loss_first = r.loss_first;
+      rep->loss_final = r.loss_final;
+      rep->pod_energy = r.pod_energy;
+      rep->seconds_pod = r.seconds_pod;
+      rep->seconds_train = r.seconds_train;
+    }
+  });
+}
+
+int bcm_guess_predict(bcm_context c, bcm_guess g, int count, const double* params, double* x0) {
+  return guarded([&] {
+    need(c && g && params && x0, "guess_predict: bad argument");
+    g->g->predict(c->c.stream, count, params, x0);
   });
 }
 

@@ -11,6 +11,8 @@
 #include <cmath>
 #include <cstring>
 #include <map>
+#include <memory>
+#include <mutex>
 #include <random>
 
 namespace bcm {
@@ -21,9 +23,12 @@ struct Normal {
   std::mt19937_64 g;
   bool has = false;
   double spare = 0.0;
+  std::vector<double>* record = nullptr;     // appends every draw when set
+  const std::vector<double>* tape = nullptr;  // replays recorded draws when set
+  size_t pos = 0;
   explicit Normal(uint64_t seed) : g(seed) {}
   double uniform() { return ((double)(g() >> 11) + 0.5) * 0x1.0p-53; }
-  double operator()() {
+  double draw() {
     if (has) { has = false; return spare; }
     const double u1 = uniform(), u2 = uniform();
     const double r = std::sqrt(-2.0 * std::log(u1));
@@ -32,7 +37,22 @@ struct Normal {
     has = true;
     return r * std::cos(t);
   }
+  double operator()() {
+    if (tape) return (*tape)[pos++];
+    const double v = draw();
+    if (record) record->push_back(v);
+    return v;
+  }
 };
+
+// The random template of an elasticity cell depends on (m, seed) only, so
+// its normal draws are recorded once and replayed for every cell of the
+// same template (families of cells differ only in their physics
+// parameters). Bounded to kTapeBytes of host memory in total.
+constexpr size_t kTapeBytes = size_t(2) << 30;
+std::mutex g_tape_mu;
+std::map<std::pair<int, uint64_t>, std::shared_ptr<const std::vector<double>>> g_tapes;
+size_t g_tape_bytes = 0;
 
 double harmonic_mean(double a, double b) { return 2.0 * a * b / (a + b); }
 
@@ -173,6 +193,15 @@ void gen_elastic_cell(int m, const CellParams& cp, uint64_t template_seed, Block
   const int bla = 24, nterm = 8, bb = bla * bla;
   const int nsup = m * m * m;
   Normal rnd(template_seed ^ (0x9E3779B97F4A7C15ull * (uint64_t)m));
+  std::shared_ptr<const std::vector<double>> tape;
+  std::vector<double> recorded;
+  {
+    std::lock_guard<std::mutex> lk(g_tape_mu);
+    auto it = g_tapes.find({m, template_seed});
+    if (it != g_tapes.end()) tape = it->second;
+  }
+  if (tape) rnd.tape = tape.get();
+  else rnd.record = &recorded;
   // Stiffness field: smoothed sphere of radius cp.radius around (0.5 + shift, 0.5, 0.5).
   std::vector<double> e(nsup);
   const double w = 0.5 / m;
@@ -215,6 +244,14 @@ void gen_elastic_cell(int m, const CellParams& cp, uint64_t template_seed, Block
       }
   as.add_diagonals(rnd, e, e);
   upper = as.build(true);
+  if (!tape) {
+    std::lock_guard<std::mutex> lk(g_tape_mu);
+    const size_t bytes = recorded.size() * sizeof(double);
+    if (g_tape_bytes + bytes <= kTapeBytes && !g_tapes.count({m, template_seed})) {
+      g_tapes[{m, template_seed}] = std::make_shared<const std::vector<double>>(std::move(recorded));
+      g_tape_bytes += bytes;
+    }
+  }
   // Macroscopic strains (Voigt order 11, 22, 33, 23, 13, 12): the load is the
   // jump of the stiffness across each cell times the strain's traction.
   static const double eps[6][3][3] = {

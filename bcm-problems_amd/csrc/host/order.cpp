@@ -289,6 +289,9 @@ BlockCsr symm_mtr(const BlockCsr& up) {
   s.nsup = n;
   s.sprnd = up.sprnd;
   s.ia.assign(n + 1, 0);
+  s.ja.reserve(up.ja.size() + t.ja.size());
+  s.bsa.reserve(up.ja.size() + t.ja.size());
+  s.a.reserve(up.a.size() + t.a.size());
   int64_t nz = 0;
   for (int i = 0; i < n; ++i) {
     int p = up.ia[i], pe = up.ia[i + 1], q = t.ia[i], qe = t.ia[i + 1];
@@ -316,6 +319,9 @@ BlockCsr upper_part(const BlockCsr& a) {
   u.nsup = a.nsup;
   u.sprnd = a.sprnd;
   u.ia.assign(a.nsup + 1, 0);
+  u.ja.reserve(a.ja.size());
+  u.bsa.reserve(a.ja.size());
+  u.a.reserve(a.a.size());
   int64_t nz = 0;
   for (int i = 0; i < a.nsup; ++i) {
     for (int j = a.ia[i]; j < a.ia[i + 1]; ++j) {

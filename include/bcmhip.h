@@ -202,6 +202,42 @@ int bcm_batch_autotune(bcm_context c, bcm_batch b, const bcm_solve_opts* o, int 
 int bcm_batch_get_tune(bcm_batch b, bcm_batch_tune* t);
 int bcm_batch_set_tune(bcm_batch b, const bcm_batch_tune* t);
 
+/* ---- learned initial guess (no counterpart in the reference) ----
+ * MLP nin -> nh (tanh) -> nh (tanh) -> nout with BF16 weights and FP32
+ * accumulation on the matrix cores. Parameters are one FP32 array
+ * [W1 b1 W2 b2 W3 b3], W_l row-major fan_in x fan_out. Activations are
+ * row-major batch x features, device memory. */
+typedef struct bcm_mlp_s* bcm_mlp;
+int bcm_mlp_create(int nin, int nhidden, int nout, unsigned long long seed, bcm_mlp* out);
+int bcm_mlp_destroy(bcm_mlp m);
+int bcm_mlp_nparam(bcm_mlp m, long long* nparam);
+int bcm_mlp_set_params(bcm_mlp m, const float* host);  /* also resets the Adam state */
+int bcm_mlp_get_params(bcm_mlp m, float* host);
+int bcm_mlp_get_grads(bcm_mlp m, float* host);
+int bcm_mlp_forward(bcm_context c, bcm_mlp m, int batch, const float* x, float* y);
+/* gradients of mean((MLP(x) - t)^2) (kept in the model); *loss (host) may be NULL */
+int bcm_mlp_backward(bcm_context c, bcm_mlp m, int batch, const float* x, const float* t, double* loss);
+int bcm_mlp_adam_step(bcm_context c, bcm_mlp m, float lr);
+
+/* Initial-guess model for a family of cells with n unknowns and nrhs
+ * right-hand sides: solved cells are stored as snapshots, fit() builds
+ * nmodes POD modes per right-hand side (randomized range finder on the
+ * device) and trains the MLP (nparam -> nhidden -> nhidden -> nrhs*nmodes)
+ * on the snapshots' coefficients; predict() writes x0 for new cells. Cell
+ * vectors use the batch layout: column q holds the cells' q-th vectors one
+ * after another (n each). */
+typedef struct bcm_guess_s* bcm_guess;
+typedef struct bcm_guess_report {
+  int snapshots, modes, steps;
+  double loss_first, loss_final, pod_energy, seconds_pod, seconds_train;
+} bcm_guess_report;
+int bcm_guess_create(long long n, int nrhs, int nparam, int nmodes, int nhidden, int capacity,
+                     unsigned long long seed, bcm_guess* out);
+int bcm_guess_destroy(bcm_guess g);
+int bcm_guess_add(bcm_context c, bcm_guess g, int count, const double* params, const double* x);
+int bcm_guess_fit(bcm_context c, bcm_guess g, int steps, double lr, bcm_guess_report* rep);
+int bcm_guess_predict(bcm_context c, bcm_guess g, int count, const double* params, double* x0);
+
 /* ---- drop-in entry points (callback ABI of smatrix.h:15-21; C++ references
  *      are passed as pointers, which is the same machine-level convention) ---- */
 typedef void (*bcm_cb_nsup)(void* obj, int* nsup);

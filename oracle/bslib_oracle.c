@@ -364,11 +364,12 @@ done:
 }
 
 /* Flexible GMRES(m) (Saad 1993) with modified Gram-Schmidt and Givens
- * rotations, right preconditioning through the factor, stopping on
+ * rotations, right preconditioning through the factor (symm: A is
+ * upper-stored symmetric and applied as MvmASymm), stopping on
  * ||b - A x|| <= eps ||b|| (the Arnoldi residual estimate, confirmed by the
  * true residual at every restart). */
 orc_result orc_fgmres(const orc_bsr* a, const orc_bsr* l, const orc_bsr* u, const double* x0, const double* b,
-                      int m, int niter, double eps, double* sol_out) {
+                      int m, int niter, double eps, int symm, double* sol_out) {
   const int n = a->sprnd[a->nsup];
   const size_t vb = sizeof(double) * (size_t)n;
   double* V = malloc(vb * (m + 1));
@@ -381,7 +382,7 @@ orc_result orc_fgmres(const orc_bsr* a, const orc_bsr* l, const orc_bsr* u, cons
   const double bnorm = sqrt(orc_scprod(n, b, b));
   int total = 0;
   while (total < niter) {
-    orc_mvma(a, x, r, 1);
+    if (symm) orc_mvmasymm(a, x, r); else orc_mvma(a, x, r, 1);
     for (int i = 0; i < n; ++i) r[i] = b[i] - r[i];
     const double beta = sqrt(orc_scprod(n, r, r));
     res.resfin = beta / bnorm;
@@ -393,7 +394,7 @@ orc_result orc_fgmres(const orc_bsr* a, const orc_bsr* l, const orc_bsr* u, cons
     for (j = 0; j < m && total < niter; ++j) {
       precond(l, u, V + (size_t)j * n, Z + (size_t)j * n, tmp);
       double* w = V + (size_t)(j + 1) * n;
-      orc_mvma(a, Z + (size_t)j * n, w, 1);
+      if (symm) orc_mvmasymm(a, Z + (size_t)j * n, w); else orc_mvma(a, Z + (size_t)j * n, w, 1);
       for (int i = 0; i <= j; ++i) {
         const double h = orc_scprod(n, w, V + (size_t)i * n);
         H[(size_t)j * (m + 1) + i] = h;

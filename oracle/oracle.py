@@ -63,7 +63,7 @@ def orc() -> C.CDLL:
         l.orc_lanczos.restype = Result
         l.orc_bicgstab.argtypes = [C.POINTER(Bsr)] * 3 + [_DP, _DP, C.c_int, C.c_double, _DP]
         l.orc_bicgstab.restype = Result
-        l.orc_fgmres.argtypes = [C.POINTER(Bsr)] * 3 + [_DP, _DP, C.c_int, C.c_int, C.c_double, _DP]
+        l.orc_fgmres.argtypes = [C.POINTER(Bsr)] * 3 + [_DP, _DP, C.c_int, C.c_int, C.c_double, C.c_int, _DP]
         l.orc_fgmres.restype = Result
         _orc = l
     return _orc
@@ -180,9 +180,10 @@ def bicgstab(a: Block, l: Block, u: Block, b, x0=None, niter=1000, eps=1e-10):
     return x, r
 
 
-def fgmres(a: Block, l: Block, u: Block, b, x0=None, m=30, niter=1000, eps=1e-10):
+def fgmres(a: Block, l: Block, u: Block, b, x0=None, m=30, niter=1000, eps=1e-10, symm=False):
     b = _d(b); x0 = np.zeros_like(b) if x0 is None else _d(x0); x = np.zeros_like(b)
-    r = orc().orc_fgmres(C.byref(a.c), C.byref(l.c), C.byref(u.c), _dp(x0), _dp(b), m, niter, eps, _dp(x))
+    r = orc().orc_fgmres(C.byref(a.c), C.byref(l.c), C.byref(u.c), _dp(x0), _dp(b), m, niter, eps, int(symm),
+                         _dp(x))
     return x, r
 
 

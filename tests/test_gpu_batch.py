@@ -17,7 +17,7 @@ from conftest import rel
 
 pytestmark = pytest.mark.gpu
 
-SIZES = [3, 5, 4, 2, 6]  # uneven periodic cells (m^3 block rows of 24)
+SIZES = [3, 5, 4, 7, 6]  # uneven periodic cells (m^3 block rows of 24)
 
 
 def cell(B, O, m, seed):

@@ -67,7 +67,7 @@ def test_fgmres_short_restart_matches_oracle(B, O, ctx, sys3d):
 def test_fgmres_symmetric_ic2(B, O, ctx):
     h = heat(B, O, 14, 12, 9, seed=22)
     x, st = run(B, ctx, h["mo"], h["f"], h["b"], eps=1e-10, restart=20)
-    xo, r = O.fgmres(h["A"], h["U"], h["U"], h["b"], m=20, eps=1e-10)
+    xo, r = O.fgmres(h["A"], h["U"], h["U"], h["b"], m=20, eps=1e-10, symm=True)
     assert st.converged == 1 and st.relres_true <= 1e-10
     assert abs(st.iterations - r.nitperf) <= 1
     assert rel(x, xo) < 1e-8

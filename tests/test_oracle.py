@@ -156,3 +156,38 @@ def test_krylov_against_direct(O, hs, ns):
         assert rel(x, xd) < 1e-9
     x, r = O.lanczos(ns["A"], ns["L"], ns["U"], ns["b"], eps=1e-12, sttype=1)
     assert rel(x, xd) < 1e-8
+
+
+def test_mlp_oracle_gradient_matches_finite_differences():
+    """Pins the learned-guess network's backward pass (the GPU gradients'
+    checker) against central differences of its own float64 loss."""
+    import mlp_oracle as M
+
+    rng = np.random.default_rng(5)
+    nin, nh, nout, batch = 4, 7, 5, 6
+    _, npar = M.layout(nin, nh, nout)
+    p = rng.normal(0, 0.5, npar)
+    x = rng.normal(size=(batch, nin))
+    t = rng.normal(size=(batch, nout))
+    loss, g = M.loss_grad(p, x, t, nin, nh, nout)
+    h = 1e-6
+    num = np.empty(npar)
+    for i in range(npar):
+        e = np.zeros(npar)
+        e[i] = h
+        num[i] = (M.loss_grad(p + e, x, t, nin, nh, nout)[0] - M.loss_grad(p - e, x, t, nin, nh, nout)[0]) / (2 * h)
+    assert np.max(np.abs(num - g)) <= 1e-7 * max(1.0, np.max(np.abs(g)))
+    # the loss is the closed form of the network
+    w1, b1, w2, b2, w3, b3 = M.unpack(p, nin, nh, nout)
+    assert abs(loss - np.mean((np.tanh(np.tanh(x @ w1 + b1) @ w2 + b2) @ w3 + b3 - t) ** 2)) < 1e-14
+
+
+def test_mlp_oracle_bf16_rounding():
+    import mlp_oracle as M
+
+    v = np.array([1.0, 1.0 + 2 ** -8, 1.0 + 3 * 2 ** -8, -3.14159, 1e-30, 65504.0])
+    r = M.bf16_round(v)
+    assert r[0] == 1.0 and r[1] == 1.0  # tie to even
+    assert r[2] == 1.0 + 2 ** -6  # 1 + 3/256 rounds to 1 + 4/256
+    assert abs(r[3] + 3.140625) < 1e-12
+    assert np.all(np.abs(r - v) <= np.abs(v) * 2 ** -8)
