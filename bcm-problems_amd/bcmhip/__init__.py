@@ -80,7 +80,8 @@ class BatchStats(C.Structure):
 
 class BatchTune(C.Structure):
     _fields_ = [
-        ("sweep_mode", C.c_int), ("waves_per_cu", C.c_int), ("graph", C.c_int), ("check_every", C.c_int),
+        ("sweep_mode", C.c_int), ("waves_per_cu", C.c_int), ("level_order", C.c_int), ("graph", C.c_int),
+        ("check_every", C.c_int),
         ("ms_per_iter", C.c_double),
     ]
 

@@ -495,9 +495,10 @@ int bcm_context_sync(bcm_context c) {
 
 int bcm_context_set_sweep_mode(bcm_context c, int mode) {
   return guarded([&] {
-    need(c != nullptr && (mode == 0 || mode == 1), "context_set_sweep_mode: bad argument");
+    need(c != nullptr && mode >= 0 && mode <= 2, "context_set_sweep_mode: bad argument");
     BCM_HIP(hipStreamSynchronize(c->c.stream));
-    c->c.sweep_mode = mode;
+    c->c.sweep_mode = mode == 0 ? 0 : 1;
+    c->c.sync.level_order = mode == 1;
   });
 }
 
@@ -660,6 +661,7 @@ int bcm_batch_solve(bcm_context c, bcm_batch b, const bcm_solve_opts* o, int nrh
 static void tune_out(const bcm::BatchTune& t, bcm_batch_tune* out) {
   out->sweep_mode = t.sweep_mode;
   out->waves_per_cu = t.waves_per_cu;
+  out->level_order = t.level_order;
   out->graph = t.graph;
   out->check_every = t.check_every;
   out->ms_per_iter = t.ms_per_iter;
@@ -691,6 +693,7 @@ int bcm_batch_set_tune(bcm_batch b, const bcm_batch_tune* t) {
     need(t->check_every >= 1 && t->check_every <= 1024, "batch_set_tune: check_every in 1..1024");
     b->b.tune.sweep_mode = t->sweep_mode;
     b->b.tune.waves_per_cu = t->waves_per_cu;
+    b->b.tune.level_order = t->level_order ? 1 : 0;
     b->b.tune.graph = t->graph ? 1 : 0;
     b->b.tune.check_every = t->check_every;
     b->b.drop_graph();

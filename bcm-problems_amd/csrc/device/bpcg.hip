@@ -372,7 +372,7 @@ void iteration(Batch& bt, Context& ctx, const BWork& w, const SolveOptions& opt,
 struct BatchGraph {
   hipGraph_t g = nullptr;
   hipGraphExec_t exec = nullptr;
-  int nrhs = 0, prec = -1, iters = 0, sweep_mode = -1, waves = 0, flex = 0, niter = 0, ittype = -1;
+  int nrhs = 0, prec = -1, iters = 0, sweep_mode = -1, waves = 0, order = -1, flex = 0, niter = 0, ittype = -1;
   double eps = 0.0;
   std::vector<const void*> key;  // device buffers the captured launches use
   ~BatchGraph() {
@@ -456,11 +456,14 @@ void solve_nr(Batch& bt, Context& ctx, const SolveOptions& opt, const double* b,
   const BatchTune& tu = bt.tune;
   const int saved_mode = ctx.sweep_mode;
   const int saved_waves = ctx.sync.waves_per_cu;
+  const bool saved_order = ctx.sync.level_order;
   ctx.sweep_mode = tu.sweep_mode;
   ctx.sync.waves_per_cu = tu.waves_per_cu;
+  ctx.sync.level_order = tu.level_order != 0;
   auto restore = [&] {
     ctx.sweep_mode = saved_mode;
     ctx.sync.waves_per_cu = saved_waves;
+    ctx.sync.level_order = saved_order;
     ctx.sync.cooperative = true;
   };
   try {
@@ -488,7 +491,8 @@ void solve_nr(Batch& bt, Context& ctx, const SolveOptions& opt, const double* b,
       }
       BatchGraph* g = bt.graph.get();
       const bool valid = g && g->nrhs == NR && g->prec == opt.prec && g->iters == every &&
-                         g->sweep_mode == tu.sweep_mode && g->waves == tu.waves_per_cu && g->flex == flex &&
+                         g->sweep_mode == tu.sweep_mode && g->waves == tu.waves_per_cu &&
+                         g->order == tu.level_order && g->flex == flex &&
                          g->niter == opt.niter && g->ittype == opt.ittype && g->eps == opt.eps && g->key == key;
       if (!valid) {
         bt.graph.reset(new BatchGraph());
@@ -507,7 +511,7 @@ void solve_nr(Batch& bt, Context& ctx, const SolveOptions& opt, const double* b,
         BCM_HIP(hipGraphInstantiate(&g->exec, g->g, nullptr, nullptr, 0));
         ctx.sync.cooperative = true;
         g->nrhs = NR; g->prec = opt.prec; g->iters = every; g->sweep_mode = tu.sweep_mode;
-        g->waves = tu.waves_per_cu; g->flex = flex; g->niter = opt.niter; g->ittype = opt.ittype; g->eps = opt.eps;
+        g->waves = tu.waves_per_cu; g->order = tu.level_order; g->flex = flex; g->niter = opt.niter; g->ittype = opt.ittype; g->eps = opt.eps;
         g->key = key;
       }
     }
@@ -580,32 +584,20 @@ void batch_solve(Batch& bt, Context& ctx, const SolveOptions& opt, int nrhs, con
 }
 
 BatchTune batch_tune(Batch& bt, Context& ctx, const SolveOptions& opt, int nrhs, const double* b) {
-  // candidates: sweep schedule x launch shape x graph replay x polling interval
-  std::vector<BatchTune> cand;
-  for (int graph = 0; graph <= 1; ++graph)
-    for (int mode = 0; mode <= 1; ++mode)
-      for (int waves : {4, 8, 16}) {
-        if (mode == 0 && waves != 8) continue;
-        for (int every : {4, 16}) {
-          BatchTune t;
-          t.sweep_mode = mode;
-          t.waves_per_cu = waves;
-          t.graph = graph;
-          t.check_every = every;
-          cand.push_back(t);
-        }
-      }
+  // Two stages: the sweep schedule (level-scheduled, or sync-free with a row
+  // order and a launch shape) with eager launches, then graph replay and the
+  // polling interval for the winner. Every candidate runs the same fixed
+  // number of iterations (tolerance 0) from x0 = 0.
   const size_t vb = (size_t)bt.n * nrhs * 8;
   double* xs = ctx.vec(49, vb);
   SolveOptions o = opt;
-  o.eps = 0.0;         // never converges: every candidate runs the same iterations
+  o.eps = 0.0;
   o.max_restarts = 0;
+  o.niter = 16;  // a multiple of every polling interval tried
   const BatchTune keep = bt.tune;
-  BatchTune best = keep;
-  best.ms_per_iter = 1e300;
-  o.niter = 32;  // a multiple of every polling interval: all candidates run the same iterations
-  for (const BatchTune& c : cand) {
+  auto time_it = [&](const BatchTune& c) {
     bt.tune = c;
+    bt.drop_graph();
     BatchStats st;
     BCM_HIP(hipMemsetAsync(xs, 0, vb, ctx.stream));
     batch_solve(bt, ctx, o, nrhs, b, xs, false, st, nullptr, nullptr);  // warm-up (and graph capture)
@@ -613,15 +605,42 @@ BatchTune batch_tune(Batch& bt, Context& ctx, const SolveOptions& opt, int nrhs,
     auto t0 = std::chrono::steady_clock::now();
     batch_solve(bt, ctx, o, nrhs, b, xs, false, st, nullptr, nullptr);
     const double ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
-    const double per = ms / std::max(1, st.iters_max);
-    if (per < best.ms_per_iter) {
-      best = c;
-      best.ms_per_iter = per;
-    }
+    return ms / std::max(1, st.iters_max);
+  };
+  std::vector<BatchTune> stage1;
+  {
+    BatchTune t;
+    t.graph = 0;
+    t.check_every = 16;
+    t.sweep_mode = 0;
+    stage1.push_back(t);
+    t.sweep_mode = 1;
+    for (int order = 0; order <= 1; ++order)
+      for (int waves : {4, 8, 16}) {
+        t.level_order = order;
+        t.waves_per_cu = waves;
+        stage1.push_back(t);
+      }
   }
+  BatchTune best = keep;
+  best.ms_per_iter = 1e300;
+  for (const BatchTune& c : stage1) {
+    const double per = time_it(c);
+    if (per < best.ms_per_iter) { best = c; best.ms_per_iter = per; }
+  }
+  const BatchTune sched = best;
+  for (int graph = 0; graph <= 1; ++graph)
+    for (int every : {4, 16}) {
+      if (graph == 0 && every == 16) continue;  // measured in stage one
+      BatchTune c = sched;
+      c.graph = graph;
+      c.check_every = every;
+      const double per = time_it(c);
+      if (per < best.ms_per_iter) { best = c; best.ms_per_iter = per; }
+    }
   bt.tune = best;
   bt.tuned = true;
-  bt.graph.reset();
+  bt.drop_graph();
   return best;
 }
 

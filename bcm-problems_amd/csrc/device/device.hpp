@@ -100,6 +100,7 @@ struct SweepSync {
   int nsup = 0;
   int waves_per_cu = 8;
   bool cooperative = true;
+  bool level_order = true;  // process rows sorted by dependency level (false: natural sweep order)
   void ensure(int nsup, hipStream_t s);
   int error(hipStream_t s);
 };

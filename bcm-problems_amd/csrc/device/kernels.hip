@@ -204,10 +204,12 @@ __global__ __launch_bounds__(256) void k_sweep_level(int nrows, const int* __res
 
 // ---------------------------------------------------------------------------
 // Sync-free triangular sweep (one cooperative launch per sweep). Wave w owns
-// rows w, w+W, w+2W, ... of the sweep order and processes them in that order;
-// every wave of the grid is resident (cooperative launch), so the smallest
-// unfinished row always belongs to a wave whose dependencies are complete and
-// the sweep cannot deadlock.
+// positions w, w+W, w+2W, ... of a topological order of the rows -- the
+// natural sweep order, or the rows sorted by dependency level, which
+// interleaves independent chains (e.g. the cells of a block-diagonal union)
+// -- and processes them in that order; every wave of the grid is resident
+// (cooperative launch), so the smallest unfinished position always belongs
+// to a wave whose dependencies are complete and the sweep cannot deadlock.
 //
 // The output vector is its own completion record: before the sweep it is
 // filled with a sentinel NaN pattern that no arithmetic produces (a signaling
@@ -248,7 +250,8 @@ __global__ void k_sweep_fill(int64_t n, TX* __restrict__ y, const int* skip) {
 }
 
 template <int B, int NR, class TA, class TX>
-__global__ __launch_bounds__(64) void k_sweep_free(int nsup, int n, int forward, const int* __restrict__ ia,
+__global__ __launch_bounds__(64) void k_sweep_free(int nsup, int n, int forward, const int* __restrict__ order,
+                                                   const int* __restrict__ ia,
                                                    const int* __restrict__ ja, const TA* __restrict__ off,
                                                    const TA* __restrict__ diag, const TX* __restrict__ x, TX* y,
                                                    int* ctl, const int* skip) {
@@ -263,7 +266,7 @@ __global__ __launch_bounds__(64) void k_sweep_free(int nsup, int n, int forward,
   const int r = lane % B, g = lane / B;
   const bool act = g < G;
   for (int t = blockIdx.x; t < nsup; t += gridDim.x) {
-    const int row = forward ? t : nsup - 1 - t;
+    const int row = order ? order[t] : (forward ? t : nsup - 1 - t);
     const int jb = ia[row], je = ia[row + 1];
     const int ncol = (je - jb) * B;
     const TA* blk0 = off + (size_t)jb * B * B;
@@ -757,7 +760,8 @@ static void sweep_b(const DevTri& T, const void* x, void* y, hipStream_t s, cons
     const TA* dg = T.diag.as<TA>();
     const TX* xx = static_cast<const TX*>(x);
     int* ct = sync->ctl.as<int>();
-    void* args[] = {&nsup, &n, &fwd, &ia, &ja, &off, &dg, &xx, &yy, &ct, (void*)&skip};
+    const int* ord = sync->level_order ? T.level_rows.as<int>() : nullptr;
+    void* args[] = {&nsup, &n, &fwd, &ord, &ia, &ja, &off, &dg, &xx, &yy, &ct, (void*)&skip};
     if (sync->cooperative)
       BCM_HIP(hipLaunchCooperativeKernel(reinterpret_cast<const void*>(kern), dim3(waves), dim3(64), args, 0, s));
     else

@@ -157,6 +157,7 @@ void solve_fgmres(Matrix& A, Factor& F, Context& ctx, const SolveOptions& opt, c
 struct BatchTune {
   int sweep_mode = 1;     // 0: level-scheduled sweeps, 1: sync-free sweeps
   int waves_per_cu = 8;   // sync-free sweeps: resident waves per CU
+  int level_order = 1;    // sync-free sweeps: rows in dependency-level order (0: natural order)
   int graph = 0;          // 1: replay check_every iterations as one captured hipGraph
   int check_every = 8;    // iterations between host polls of the done flag
   double ms_per_iter = 0.0;

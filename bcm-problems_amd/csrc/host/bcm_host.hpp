@@ -148,10 +148,12 @@ void gen_heat2d(int nx, int ny, int bla, uint64_t seed, BlockCsr& upper, std::ve
 // Config-1 family: 3D nonsymmetric (skew advection), n^3 cells, 7-point, full storage.
 void gen_seepage3d(int nx, int ny, int nz, int bla, double skew, uint64_t seed, BlockCsr& a,
                    std::vector<double>& rhs);
-// Config-3/4 family: periodic elasticity cell with a spherical inclusion,
-// block size 24 (3 components x 8 terms), 7-point, block-SPD, upper storage,
-// 6 macroscopic-strain right-hand sides. The random template depends on
-// (m, template_seed) only; the physics parameters enter smoothly.
+// Homogenization-cell family: periodic elasticity cell with a spherical
+// inclusion, block size 24 (3 components x 8 terms), 7-point, block-SPD
+// (a sum of positive semidefinite face elements plus a small multiple of the
+// local stiffness on the diagonal), upper storage, 6 macroscopic-strain
+// right-hand sides. The random template depends on (m, template_seed) only;
+// the physics parameters enter smoothly.
 void gen_elastic_cell(int m, const CellParams& cp, uint64_t template_seed, BlockCsr& upper,
                       std::vector<double>& rhs6);
 

@@ -217,8 +217,17 @@ void gen_elastic_cell(int m, const CellParams& cp, uint64_t template_seed, Block
       }
   const double nu = cp.poisson;
   const double cpl = 0.5 * nu / (1.0 - nu);
+  // Stiffness assembled from one positive semidefinite element matrix per
+  // cell face: K = h * (W_d o T T^T / bla) couples the two cells as
+  // [K -K; -K K] (h: harmonic mean of the two stiffnesses, W_d: Poisson
+  // coupling weights of the components for face direction d, T: template
+  // block). The sum is a block graph Laplacian -- not diagonally dominant,
+  // singular only along the cell-wise constant fields -- plus kMargin times
+  // the local stiffness on the diagonal, so the system is SPD with the
+  // conditioning of a discrete elliptic operator.
+  constexpr double kMargin = 1e-3;
   Assembler as(nsup, bla);
-  std::vector<double> t(bb);
+  std::vector<double> t(bb), k(bb);
   const double sc = 1.0 / std::sqrt((double)bla);
   for (int z = 0; z < m; ++z)
     for (int y = 0; y < m; ++y)
@@ -230,19 +239,30 @@ void gen_elastic_cell(int m, const CellParams& cp, uint64_t template_seed, Block
           const int o = nbr[d];
           for (auto& v : t) v = rnd() * sc;
           const double h = harmonic_mean(e[c], e[o]);
-          auto& up = as.block(c, o);
           for (int col = 0; col < bla; ++col)
-            for (int r = 0; r < bla; ++r) {
+            for (int r = col; r < bla; ++r) {
+              double s2 = 0.0;
+              for (int j = 0; j < bla; ++j) s2 += t[(size_t)j * bla + r] * t[(size_t)j * bla + col];
               const int ar = r / nterm, ac = col / nterm;
               const double wgt = ar == ac ? (ar == d ? 1.0 + 2.0 * cpl : 1.0) : cpl;
-              up[(size_t)col * bla + r] += h * wgt * t[(size_t)col * bla + r];
+              k[(size_t)col * bla + r] = k[(size_t)r * bla + col] = h * wgt * s2 / bla;
             }
-          auto& lo = as.block(o, c);
-          for (int col = 0; col < bla; ++col)
-            for (int r = 0; r < bla; ++r) lo[(size_t)col * bla + r] = up[(size_t)r * bla + col];
+          auto& cc = as.block(c, c);
+          auto& oo = as.block(o, o);
+          auto& co = as.block(c, o);
+          auto& oc = as.block(o, c);
+          for (int i = 0; i < bb; ++i) {
+            cc[i] += k[i];
+            oo[i] += k[i];
+            co[i] -= k[i];
+            oc[i] -= k[i];
+          }
         }
       }
-  as.add_diagonals(rnd, e, e);
+  for (int c = 0; c < nsup; ++c) {
+    auto& d = as.block(c, c);
+    for (int r = 0; r < bla; ++r) d[(size_t)r * bla + r] += kMargin * e[c];
+  }
   upper = as.build(true);
   if (!tape) {
     std::lock_guard<std::mutex> lk(g_tape_mu);

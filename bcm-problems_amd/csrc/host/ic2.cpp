@@ -98,6 +98,11 @@ BlockCsr ich2(const BlockCsr& A, const SlvParam& prm, FactorStats* st) {
   int ibegfr = 0;
   int64_t nzjrl = 0, nzrtot = 0, nzju = 0, nzjr = 0;
   int64_t compact_mark = std::max<int64_t>(A.nza(), 1 << 20);
+  if (cap_vals < 0) {  // storage stays below twice the compaction mark plus one row: reserve it once
+    S.g.reserve((size_t)(2 * compact_mark) + (size_t)blamx * blamx * 64);
+    S.jg.reserve((size_t)2 * A.nzja() + 64);
+    S.addrg.reserve((size_t)2 * A.nzja() + 64);
+  }
 
   // Removes second-order blocks whose column lies before `irow`: they can no
   // longer take part in any update (Ich2FreeR). Values are untouched.

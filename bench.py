@@ -1,19 +1,31 @@
 #!/usr/bin/env python3
 """Benchmark of the GPU solve phase on one MI355X.
 
-A step is one complete solve, from x0 = 0 to FP64 true relative residual
-||b - A x|| / ||b|| <= 1e-10, of the plain FP64 GPU drop-in workload: the 3D
-nonsymmetric block system on a 32^3 cell grid, block size 24, 7-point block
-stencil (N = 786,432), RCM ordering, ILU2 (tau1 = 1e-2, tau2 = 1e-4) and
-right-preconditioned BiCGStab. The matrix, its factors and the right-hand
-side are resident in HBM before the timed region; host factorization time is
-reported separately.
+Default workload -- learned initial guess over a stream of homogenization
+cells: periodic 16^3 elasticity cells (block 24, 7-point, N = 98,304, one
+spherical inclusion with radius U[0.1, 0.45], stiffness contrast
+10^U[0, 3], Poisson ratio U[0.2, 0.45], centre shift U[-0.15, 0.15]), 6
+macroscopic-strain right-hand sides each, IC2 factors (tau1 = 0.1,
+tau2 = 0.01) computed on the host. The first 1024 cells are solved from
+x0 = 0 (batched block PCG, FP32 factors, FP64 outer recurrence) and train
+the initial-guess model (64 POD modes per right-hand side, MLP 4 -> 256 ->
+256 -> 384 trained on the GPU with BF16 MFMA kernels). Evaluation cells
+are then solved in groups of 64 cells x 6 right-hand sides: MLP inference,
+POD reconstruction and one batched block-PCG launch sequence per group
+(fallback to x0 = 0 for any pair whose ||r0|| >= ||b||), every pair driven
+to FP64 true relative residual <= 1e-10.
 
-Prints one JSON line (rank 0) with the solves/sec, the HBM roofline of the
-dominant kernel (HIP-event times of every SpMV / triangular-sweep launch in
-the timed region against their algorithmic bytes) and a bounded CPU baseline
-(the oracle's BiCGStab on the box's host, one core, scaled by the iteration
-count of the GPU solve).
+A step is one evaluation group; the timed region covers the guess and the
+solve of `--steps` groups with their matrices and factors already resident
+in HBM (uploads happen between steps, outside the timed region). `value`
+is (cell, right-hand side) solves per second. The plain FP64 GPU drop-in
+(one right-hand side at a time, FP64 factors, x0 = 0) is timed on a sample
+of the same cells for the comparison, and the reference's own serial Cg
+(compiled from its sources into oracle/_ref) on one host core gives the
+CPU baseline.
+
+`--workload drop-in` runs the plain FP64 nonsymmetric drop-in instead (32^3
+cells, block 24, ILU2 + BiCGStab).
 """
 
 import argparse
@@ -21,7 +33,9 @@ import json
 import os
 import platform
 import sys
+import threading
 import time
+from concurrent.futures import ThreadPoolExecutor
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.join(ROOT, "bcm-problems_amd"))
@@ -31,6 +45,11 @@ import numpy as np  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0
 EPS = 1e-10
+METRIC = "solves/sec at rel-res 1e-10: mixed+batched+learned vs plain FP64 GPU; HBM GB/s"
+CELL_M = 16
+GROUP = 64
+TEMPLATE_SEED = 20261206
+PARAM_SEED = 20261206
 
 
 def log(msg):
@@ -47,134 +66,373 @@ def cpu_model():
     return platform.processor()
 
 
-def build_system(B, nx, bla, seed):
+def host_threads():
+    try:
+        return max(1, min(16, len(os.sched_getaffinity(0))))
+    except AttributeError:
+        return max(1, min(16, os.cpu_count() or 1))
+
+
+def kernel_stats(ms, launches, nbytes):
+    avg_ms = ms / max(launches, 1)
+    per_launch = nbytes / max(launches, 1)
+    gbs = per_launch / (avg_ms * 1e-3) / 1e9 if avg_ms > 0 else 0.0
+    return dict(total_ms=ms, launches=int(launches), avg_ms=avg_ms, bytes_per_launch=per_launch, gbs=gbs)
+
+
+def roofline_from(prof):
+    spmv = kernel_stats(prof.spmv_ms, prof.spmv_launches, prof.spmv_bytes)
+    sweep = kernel_stats(prof.sweep_ms, prof.sweep_launches, prof.sweep_bytes)
+    name, dom = ("sweep", sweep) if sweep["total_ms"] >= spmv["total_ms"] else ("spmv", spmv)
+    roof = dict(bound="hbm", achieved=dom["gbs"], peak=HBM_PEAK_GBS, unit="GB/s", frac=dom["gbs"] / HBM_PEAK_GBS,
+                traffic=None, kernel=name, bytes_per_launch=dom["bytes_per_launch"], avg_launch_ms=dom["avg_ms"])
+    return roof, {"spmv": spmv, "sweep": sweep}
+
+
+# ---------------------------------------------------------------------------
+# learned-guess workload
+
+
+class CellFactory:
+    """Generates, orders and factorizes cells on host threads, in order, at
+    most `window` cells ahead of the consumer (bounded host memory)."""
+
+    def __init__(self, B, params, threads, window):
+        self.B = B
+        self.params = params
+        mat0, _ = B.gen_elastic_cell(CELL_M, 0.3, 10.0, 0.3, 0.0, TEMPLATE_SEED)
+        self.order = mat0.rcm_order()  # every cell shares the template's pattern, hence its RCM ordering
+        self.fparams = B.default_params(tau1=0.1, tau2=0.01)
+        self.pool = ThreadPoolExecutor(threads)
+        self.window = window
+        self.futs = [None] * len(params)
+        self.submitted = 0
+        self.lock = threading.Lock()
+        self._fill(0)
+
+    def _fill(self, i):
+        with self.lock:
+            while self.submitted < min(len(self.params), i + self.window):
+                self.futs[self.submitted] = self.pool.submit(self._build, self.submitted)
+                self.submitted += 1
+
+    def _build(self, i):
+        r, lc, nu, sh = self.params[i]
+        mat, rhs = self.B.gen_elastic_cell(CELL_M, r, 10 ** lc, nu, sh, TEMPLATE_SEED)
+        mo = mat.reorder(self.order)
+        f = self.B.Factor.ic2(mo, self.fparams)
+        return mo, f, mo.reorder_vector(self.order, rhs)
+
+    def get(self, i):
+        self._fill(i)
+        return self.futs[i].result()
+
+    def release(self, i):
+        self.futs[i] = None
+
+    def close(self):
+        self.pool.shutdown(wait=False, cancel_futures=True)
+
+
+def draw_params(k, seed):
+    rng = np.random.default_rng(seed)
+    return np.stack([rng.uniform(0.1, 0.45, k), rng.uniform(0.0, 3.0, k), rng.uniform(0.2, 0.45, k),
+                     rng.uniform(-0.15, 0.15, k)], axis=1)
+
+
+def make_group(B, fac, idx, prec_mask):
+    cells = [fac.get(i) for i in idx]
+    bt = B.Batch([(mo, f) for mo, f, _ in cells], prec_mask=prec_mask)
+    b = bt.gather([c[2] for c in cells])
+    return bt, B.DeviceArray.from_host(b), cells
+
+
+def run_learned(args, B):
+    ntrain = args.train_cells
+    neval = (args.warmup + args.steps) * GROUP
+    params = draw_params(ntrain + neval, PARAM_SEED)
+    threads = host_threads()
+    t_setup0 = time.perf_counter()
+    fac = CellFactory(B, params, threads, window=2 * GROUP + threads)
+    ctx = B.Context()
+    n = 24 * CELL_M ** 3
+    opts = B.default_opts(kind=B.BLOCKPCG, eps=EPS, precision=B.F32, niter=5000)
+    guess = B.Guess(n, 6, 4, nmodes=64, nhidden=256, capacity=ntrain, seed=11)
+    tune = None
+    train_pairs, train_s, train_iters = 0, 0.0, []
+    # prefetch the next group's device images while the current one solves
+    next_group = [None]
+
+    def prefetch(idx, mask):
+        try:
+            next_group[0] = make_group(B, fac, idx, mask)
+        except BaseException as e:  # re-raised by the consumer
+            next_group[0] = e
+
+    def take():
+        th.join()
+        g = next_group[0]
+        next_group[0] = None
+        if isinstance(g, BaseException) or g is None:
+            raise RuntimeError(f"cell preparation failed: {g!r}")
+        return g
+
+    groups = [list(range(k, min(ntrain, k + GROUP))) for k in range(0, ntrain, GROUP)]
+    evalg = [list(range(ntrain + k * GROUP, ntrain + (k + 1) * GROUP)) for k in range(args.warmup + args.steps)]
+    th = threading.Thread(target=prefetch, args=(groups[0], 0b010))
+    th.start()
+    for gi, idx in enumerate(groups):
+        bt, db, cells = take()
+        nxt = groups[gi + 1] if gi + 1 < len(groups) else evalg[0]
+        th = threading.Thread(target=prefetch, args=(nxt, 0b010))
+        th.start()
+        if tune is None:
+            t = bt.autotune(ctx, db, opts, 6)
+            tune = dict(sweep_mode=t.sweep_mode, waves_per_cu=t.waves_per_cu, level_order=t.level_order,
+                        graph=t.graph, check_every=t.check_every)
+            log(f"autotuned schedule {tune} ({t.ms_per_iter:.2f} ms/iteration)")
+        bt.set_tune(**tune)
+        dx = B.DeviceArray(db.shape)
+        dx.zero()
+        t0 = time.perf_counter()
+        st, its, rr = bt.solve(ctx, db, dx, opts, 6)
+        train_s += time.perf_counter() - t0
+        assert st.converged == st.pairs, f"training group {gi}: {st.converged}/{st.pairs} converged"
+        train_pairs += st.pairs
+        train_iters.append(float(its.mean()))
+        guess.add(ctx, params[idx], dx)
+        for i in idx:
+            fac.release(i)
+        if gi % 4 == 0 or gi == len(groups) - 1:
+            log(f"training group {gi + 1}/{len(groups)}: {st.seconds * 1e3:.0f} ms, mean iterations {its.mean():.1f}, "
+                f"host cells ready {time.perf_counter() - t_setup0:.0f}s")
+        del bt, db, dx, cells
+    rep = guess.fit(ctx, steps=args.fit_steps, lr=2e-3)
+    log(f"guess fit: POD energy kept {rep.pod_energy:.6f}, loss {rep.loss_first:.3e} -> {rep.loss_final:.3e}, "
+        f"POD {rep.seconds_pod:.2f}s, MLP {rep.seconds_train:.2f}s")
+
+    # evaluation groups
+    timed_s, timed_pairs, its_learned, its_zero, relres_max, ok = 0.0, 0, [], [], 0.0, True
+    prof = None
+    plain_cells = []
+    for gi, idx in enumerate(evalg):
+        bt, db, cells = take()
+        if gi + 1 < len(evalg):
+            th = threading.Thread(target=prefetch, args=(evalg[gi + 1], 0b010))
+            th.start()
+        bt.set_tune(**tune)
+        dx = B.DeviceArray(db.shape)
+        ctx.sync()
+        t0 = time.perf_counter()
+        guess.predict(ctx, params[idx], dx)
+        st, its, rr = bt.solve(ctx, db, dx, opts, 6, True)
+        dt = time.perf_counter() - t0
+        timed = gi >= args.warmup
+        if timed:
+            timed_s += dt
+            timed_pairs += st.pairs
+            its_learned.append(float(its.mean()))
+            relres_max = max(relres_max, float(rr.max()))
+            ok = ok and st.converged == st.pairs and float(rr.max()) <= EPS
+        if gi == args.warmup:
+            # same group from x0 = 0 (iteration comparison) and one profiled pass (kernel roofline)
+            dz = B.DeviceArray(db.shape)
+            dz.zero()
+            stz, itz, _ = bt.solve(ctx, db, dz, opts, 6)
+            its_zero.append(float(itz.mean()))
+            ctx.profile(True)
+            ctx.profile_read(reset=True)
+            bt.set_tune(**dict(tune, graph=0))
+            guess.predict(ctx, params[idx], dz)
+            bt.solve(ctx, db, dz, opts, 6, True)
+            ctx.sync()
+            prof = ctx.profile_read()
+            ctx.profile(False)
+            plain_cells = cells[:args.plain_cells]
+            log(f"group {gi}: zero guess {itz.mean():.1f} iterations ({stz.seconds * 1e3:.0f} ms), learned "
+                f"{its.mean():.1f} iterations ({st.seconds * 1e3:.0f} ms)")
+        log(f"eval group {gi + 1}/{len(evalg)}{'' if timed else ' (warm-up)'}: {dt * 1e3:.0f} ms, mean iterations "
+            f"{its.mean():.1f}, max relres {rr.max():.2e}")
+        for i in idx[args.plain_cells:] if gi == args.warmup else idx:
+            fac.release(i)
+        del bt, db, dx
+    fac.close()
+
+    # plain FP64 drop-in on a sample of the evaluation cells: one right-hand side at a time
+    popts = B.default_opts(kind=B.PCG, eps=EPS, ittype=3, check_every=8, niter=5000)
+    pctx = B.Context()
+    for mo, f, b in plain_cells:  # device images are uploaded outside the timed loop
+        d1 = B.DeviceArray.from_host(np.ascontiguousarray(b[:, 0]))
+        x1 = B.DeviceArray(mo.n)
+        x1.zero()
+        pctx.solve(mo, f, d1, x1, popts)
+    nplain, plain_its = 0, []
     t0 = time.perf_counter()
-    m, rhs = B.gen_seepage3d(nx, nx, nx, bla, 0.3, seed)
+    for mo, f, b in plain_cells:
+        for q in range(6):
+            d1 = B.DeviceArray.from_host(np.ascontiguousarray(b[:, q]))
+            x1 = B.DeviceArray(mo.n)
+            x1.zero()
+            stp = pctx.solve(mo, f, d1, x1, popts)
+            plain_its.append(stp.iterations)
+            nplain += 1
+            ok = ok and stp.converged == 1
+    plain_s = time.perf_counter() - t0
+    plain_rate = nplain / plain_s
+
+    roof, kernels = roofline_from(prof)
+    value = timed_pairs / timed_s
+    cpu = None
+    if not args.no_cpu:
+        cpu = cpu_baseline_ref(B, plain_cells[0], np.mean(plain_its))
+    line = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "solves/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": timed_s / args.steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64 outer / f32 factors / bf16 MLP",
+        "data": "synthetic (seeded homogenization-cell generator)",
+        "config": {
+            "workload": f"learned initial guess: periodic {CELL_M}^3 elasticity cells (block 24, N={n}), 6 strain "
+                        f"load cases each, IC2(0.1, 0.01) factors in FP32, batched block PCG in groups of {GROUP} "
+                        f"cells to FP64 true rel-res {EPS}; {ntrain} training cells, {args.steps} timed groups of "
+                        f"{GROUP} evaluation cells",
+            "train_cells": ntrain,
+            "eval_cells_timed": args.steps * GROUP,
+            "solve": "(cell, load case) pairs",
+        },
+        "valid": bool(ok),
+        "plain_fp64_gpu": {"value": plain_rate, "unit": "solves/s", "sample": f"{nplain} solves "
+                           f"({len(plain_cells)} evaluation cells x 6 load cases, one at a time, FP64 IC2, x0 = 0)",
+                           "mean_iterations": float(np.mean(plain_its))},
+        "speedup_vs_plain": value / plain_rate,
+        "iterations": {"learned_mean": float(np.mean(its_learned)), "zero_guess_mean": float(np.mean(its_zero)),
+                       "training_mean": float(np.mean(train_iters))},
+        "relres_true_max": relres_max,
+        "batched_zero_guess_training": {"value": train_pairs / train_s, "unit": "solves/s"},
+        "guess_fit": {"pod_energy": rep.pod_energy, "loss_first": rep.loss_first, "loss_final": rep.loss_final,
+                      "seconds_pod": rep.seconds_pod, "seconds_mlp": rep.seconds_train},
+        "schedule": tune,
+        "kernels": kernels,
+        "roofline": roof,
+        "cpu_baseline": cpu,
+        "host_setup_seconds": time.perf_counter() - t_setup0,
+    }
+    print(json.dumps(line), flush=True)
+
+
+def cpu_baseline_ref(B, cell, iters_plain):
+    """The reference's serial Cg (Bslib sources compiled into oracle/_ref) on one host core."""
+    import oracle as O
+
+    mo, f, b = cell
+    A = O.Block(*mo.arrays())
+    U = O.Block(A.sprnd, *f.arrays(1, A.nsup))
+    if O.ref_available():
+        t0 = time.perf_counter()
+        _, nit, res = O.ref_cg(A, U, b[:, 0], eps=EPS, ittype=0, niter=5000)
+        dt = time.perf_counter() - t0
+        return dict(value=1.0 / dt, unit="solves/s", cores=1, kind="reference",
+                    sample=f"one load case of one evaluation cell by the reference's serial Cg (compiled from its "
+                           f"sources), {nit} iterations to rel-res {EPS} in {dt:.1f}s on one core of {cpu_model()}")
+    t0 = time.perf_counter()
+    _, r = O.cg(A, U, b[:, 0], eps=EPS, ittype=3, niter=5000)
+    dt = time.perf_counter() - t0
+    return dict(value=1.0 / dt, unit="solves/s", cores=1, kind="port",
+                sample=f"one load case of one evaluation cell by the oracle's Cg, {r.updates} iterations in "
+                       f"{dt:.1f}s on one core of {cpu_model()}")
+
+
+# ---------------------------------------------------------------------------
+# plain FP64 drop-in workload (nonsymmetric, 32^3, ILU2 + BiCGStab)
+
+
+def run_dropin(args, B):
+    import oracle as O
+
+    seed = 20261203
+    t0 = time.perf_counter()
+    m, rhs = B.gen_seepage3d(32, 32, 32, 24, 0.3, seed)
     order = m.rcm_order()
     mo = m.reorder(order)
     b = mo.reorder_vector(order, rhs)
-    t1 = time.perf_counter()
     params = B.default_params()
     f = B.Factor.ilu2(mo, params)
-    t2 = time.perf_counter()
-    log(f"system n={mo.n} nzja={mo.info()['nzja']} generated+ordered {t1 - t0:.1f}s, ILU2 {t2 - t1:.1f}s "
-        f"(fill {params.density:.3f}, second order {params.density2:.3f})")
-    return m, mo, f, b, t2 - t1
-
-
-def cpu_baseline(B, O, mo, f, b, iters_gpu, sample_iters):
-    """Times `sample_iters` BiCGStab iterations of the oracle on one host core."""
-    A = O.Block(*mo.arrays())
-    nsup = A.nsup
-    L = O.Block(A.sprnd, *f.arrays(0, nsup))
-    U = O.Block(A.sprnd, *f.arrays(1, nsup))
-    t0 = time.perf_counter()
-    _, r = O.bicgstab(A, L, U, b, niter=sample_iters, eps=1e-300)
-    dt = time.perf_counter() - t0
-    per_iter = dt / max(r.nitperf, 1)
-    solve_s = per_iter * iters_gpu
-    return dict(value=1.0 / solve_s, unit="solves/s", cores=1, kind="port",
-                sample=f"{r.nitperf} BiCGStab iterations of the same system by the oracle (C restatement, "
-                       f"one core of {cpu_model()}) in {dt:.1f}s, scaled to the {iters_gpu} iterations of one "
-                       f"GPU solve")
-
-
-def main():
-    ap = argparse.ArgumentParser()
-    ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=5)
-    ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--grid", type=int, default=32, help="cells per edge (32 is the benchmarked workload)")
-    ap.add_argument("--cpu-sample-iters", type=int, default=8)
-    ap.add_argument("--no-cpu", action="store_true")
-    args = ap.parse_args()
-    if args.gpus != 1:
-        print(json.dumps({"error": "this build runs on one GPU; multi-GPU runs are not part of it",
-                          "n_gpus": args.gpus}))
-        sys.exit(2)
-
-    import bcmhip as B
-
-    if B.device_count() < 1:
-        raise SystemExit("no GPU visible: the solve phase runs on the GPU only")
-    seed = 20261202 + 1
-    m, mo, f, b, fct_s = build_system(B, args.grid, 24, seed)
-    n = mo.n
+    fct_s = time.perf_counter() - t0
+    log(f"drop-in system n={mo.n}, ILU2 {fct_s:.1f}s")
     ctx = B.Context()
-    db = B.DeviceArray.from_host(b)
-    dx = B.DeviceArray(n)
-    # check_every=1: the host tests convergence after every test kernel, so no launch is
-    # queued past convergence and the per-kernel HIP-event times cover real work only
+    db, dx = B.DeviceArray.from_host(b), B.DeviceArray(mo.n)
     opts = B.default_opts(kind=B.BICGSTAB, eps=EPS, ittype=3, check_every=1, niter=2000)
-    log("warmup")
     for _ in range(args.warmup):
         dx.zero()
-        st = ctx.solve(mo, f, db, dx, opts)
-        log(f"warmup solve: {st.iterations} iterations, true relres {st.relres_true:.3e}, {st.seconds * 1e3:.1f} ms")
+        ctx.solve(mo, f, db, dx, opts)
     ctx.sync()
     ctx.profile(True)
     ctx.profile_read(reset=True)
     iters, relres, ok = [], [], True
     t0 = time.perf_counter()
-    for k in range(args.steps):
+    for _ in range(args.steps):
         dx.zero()
         st = ctx.solve(mo, f, db, dx, opts)
         iters.append(st.iterations)
         relres.append(st.relres_true)
         ok = ok and st.converged == 1 and st.relres_true <= EPS
     ctx.sync()
-    t1 = time.perf_counter()
+    dt = time.perf_counter() - t0
     prof = ctx.profile_read()
     ctx.profile(False)
-    dt = t1 - t0
-    log(f"timed {args.steps} solves in {dt:.3f}s: iterations {iters}, max relres {max(relres):.3e}")
-
-    def kernel(ms, launches, nbytes):
-        avg_ms = ms / max(launches, 1)
-        per_launch = nbytes / max(launches, 1)
-        gbs = per_launch / (avg_ms * 1e-3) / 1e9 if avg_ms > 0 else 0.0
-        return dict(total_ms=ms, launches=int(launches), avg_ms=avg_ms, bytes_per_launch=per_launch, gbs=gbs)
-
-    spmv = kernel(prof.spmv_ms, prof.spmv_launches, prof.spmv_bytes)
-    sweep = kernel(prof.sweep_ms, prof.sweep_launches, prof.sweep_bytes)
-    dom_name, dom = ("sweep", sweep) if sweep["total_ms"] >= spmv["total_ms"] else ("spmv", spmv)
-    roofline = dict(bound="hbm", achieved=dom["gbs"], peak=HBM_PEAK_GBS, unit="GB/s",
-                    frac=dom["gbs"] / HBM_PEAK_GBS, traffic=None, kernel=dom_name,
-                    bytes_per_launch=dom["bytes_per_launch"], avg_launch_ms=dom["avg_ms"])
+    roof, kernels = roofline_from(prof)
     cpu = None
     if not args.no_cpu:
-        import oracle as O
+        A = O.Block(*mo.arrays())
+        L = O.Block(A.sprnd, *f.arrays(0, A.nsup))
+        U = O.Block(A.sprnd, *f.arrays(1, A.nsup))
+        t1 = time.perf_counter()
+        _, r = O.bicgstab(A, L, U, b, niter=8, eps=1e-300)
+        per = (time.perf_counter() - t1) / max(r.nitperf, 1)
+        cpu = dict(value=1.0 / (per * np.median(iters)), unit="solves/s", cores=1, kind="port",
+                   sample=f"{r.nitperf} BiCGStab iterations by the oracle on one core of {cpu_model()}, scaled to "
+                          f"the {int(np.median(iters))} iterations of one GPU solve")
+    print(json.dumps({
+        "metric": METRIC, "value": args.steps / dt, "unit": "solves/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": dt / args.steps * 1e3, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f64", "data": "synthetic (seeded generator)",
+        "config": {"workload": f"plain FP64 GPU drop-in: 3D nonsymmetric 32^3 cells, block 24, 7-point, N={mo.n}, "
+                               f"RCM + ILU2(1e-2, 1e-4) + BiCGStab to true rel-res {EPS}", "seed": seed},
+        "valid": bool(ok), "iterations": iters, "relres_true_max": max(relres), "factor_seconds_host": fct_s,
+        "kernels": kernels, "roofline": roof, "cpu_baseline": cpu,
+    }), flush=True)
 
-        log("cpu baseline (oracle, one core)")
-        cpu = cpu_baseline(B, O, mo, f, b, int(np.median(iters)), args.cpu_sample_iters)
-    value = args.steps / dt
-    line = {
-        "metric": "solves/sec at rel-res 1e-10: mixed+batched+learned vs plain FP64 GPU; HBM GB/s",
-        "value": value,
-        "unit": "solves/s",
-        "n_gpus": 1,
-        "steps": args.steps,
-        "warmup": args.warmup,
-        "ms_per_step": dt / args.steps * 1e3,
-        "higher_is_better": True,
-        "scaling": "weak",
-        "vs_baseline": None,
-        "dtype": "f64",
-        "data": "synthetic (seeded generator, config-1 shape)",
-        "config": {
-            "workload": f"plain FP64 GPU drop-in: 3D nonsymmetric {args.grid}^3 cells, block 24, 7-point, "
-                        f"N={n}, RCM + ILU2(tau1=1e-2, tau2=1e-4) + BiCGStab to true rel-res 1e-10",
-            "seed": seed,
-        },
-        "valid": bool(ok),
-        "iterations": iters,
-        "relres_true_max": max(relres),
-        "factor_seconds_host": fct_s,
-        "kernels": {"spmv": spmv, "sweep": sweep},
-        "roofline": roofline,
-        "cpu_baseline": cpu,
-    }
-    print(json.dumps(line), flush=True)
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=8, help="timed evaluation groups (learned) or solves (drop-in)")
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--workload", choices=["learned", "drop-in"], default="learned")
+    ap.add_argument("--train-cells", type=int, default=1024)
+    ap.add_argument("--fit-steps", type=int, default=4000)
+    ap.add_argument("--plain-cells", type=int, default=4)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU; multi-GPU runs are not part of it",
+                          "n_gpus": args.gpus}))
+        sys.exit(2)
+    import bcmhip as B
+
+    if B.device_count() < 1:
+        raise SystemExit("no GPU visible: the solve phase runs on the GPU only")
+    if args.workload == "learned":
+        run_learned(args, B)
+    else:
+        run_dropin(args, B)
 
 
 if __name__ == "__main__":

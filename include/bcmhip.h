@@ -128,9 +128,11 @@ int bcm_dev_zero(void* p, unsigned long long bytes);
 int bcm_context_create(bcm_context* out);
 int bcm_context_destroy(bcm_context c);
 int bcm_context_sync(bcm_context c);
-/* Triangular sweep schedule: 0 = one launch per dependency level,
- * 1 = one sync-free cooperative launch per sweep (default: rows owned
- * round-robin by resident waves, bounded waits on per-row completion epochs). */
+/* Triangular sweep schedule: 0 = one launch per dependency level;
+ * 1 = one sync-free cooperative launch per sweep (default): resident waves
+ * own positions of the rows sorted by dependency level round-robin and wait
+ * (bounded) on the values of the rows they depend on; 2 = as 1 over the
+ * natural sweep order. All three give bit-identical results. */
 int bcm_context_set_sweep_mode(bcm_context c, int mode);
 
 /* Per-kernel-class timing with HIP events on the context's stream (the
@@ -178,6 +180,7 @@ typedef struct bcm_batch_stats {
 typedef struct bcm_batch_tune {
   int sweep_mode;    /* 0 level-scheduled sweeps, 1 sync-free sweeps */
   int waves_per_cu;  /* sync-free sweeps: resident waves per CU */
+  int level_order;   /* sync-free sweeps: 1 rows in dependency-level order, 0 natural order */
   int graph;         /* 1: replay check_every iterations as one captured hipGraph */
   int check_every;   /* iterations between host polls */
   double ms_per_iter;

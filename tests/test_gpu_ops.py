@@ -14,7 +14,7 @@ from helpers import heat, seepage
 pytestmark = pytest.mark.gpu
 
 
-@pytest.fixture(scope="module", params=[0, 1], ids=["levels", "syncfree"])
+@pytest.fixture(scope="module", params=[0, 1, 2], ids=["levels", "syncfree", "syncfree-natural"])
 def ctx(B, request):
     c = B.Context()
     c.set_sweep_mode(request.param)
