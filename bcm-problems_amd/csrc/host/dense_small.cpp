@@ -87,9 +87,8 @@ void tridiagonalize(int n, double* z, double* d, double* e) {
 }
 
 // Implicit QL iteration on the tridiagonal (d, e); rotations accumulated into
-// the columns of z (row-major view).
-void tridiagonal_ql(int n, double* d, double* e, double* z) {
-  auto Z = [&](int i, int j) -> double& { return z[i * n + j]; };
+// the rows of zt (zt = Z^T, so every rotation updates two contiguous rows).
+void tridiagonal_ql(int n, double* d, double* e, double* zt) {
   for (int i = 1; i < n; ++i) e[i - 1] = e[i];
   e[n - 1] = 0.0;
   const double eps = 2.220446049250313e-16;
@@ -127,10 +126,12 @@ void tridiagonal_ql(int n, double* d, double* e, double* z) {
           p = s * r;
           d[i + 1] = g + p;
           g = c * r - b;
+          double* zi = zt + (size_t)i * n;
+          double* zi1 = zt + (size_t)(i + 1) * n;
           for (int k = 0; k < n; ++k) {
-            double t = Z(k, i + 1);
-            Z(k, i + 1) = s * Z(k, i) + c * t;
-            Z(k, i) = c * Z(k, i) - s * t;
+            const double t = zi1[k];
+            zi1[k] = s * zi[k] + c * t;
+            zi[k] = c * zi[k] - s * t;
           }
         }
         if (deflated) continue;
@@ -151,17 +152,19 @@ void sym_eig(int n, double* a, double* w) {
     return;
   }
   // a is column-major symmetric: the row-major view is its transpose, i.e. the same matrix.
-  std::vector<double> z(a, a + n * n), e(n);
+  std::vector<double> z(a, a + n * n), zt((size_t)n * n), e(n);
   tridiagonalize(n, z.data(), w, e.data());
-  tridiagonal_ql(n, w, e.data(), z.data());
-  // Sort ascending; eigenvector i is column i of the row-major z.
+  for (int i = 0; i < n; ++i)
+    for (int k = 0; k < n; ++k) zt[(size_t)i * n + k] = z[(size_t)k * n + i];
+  tridiagonal_ql(n, w, e.data(), zt.data());
+  // Sort ascending; eigenvector i is row i of zt.
   std::vector<int> idx(n);
   for (int i = 0; i < n; ++i) idx[i] = i;
   std::stable_sort(idx.begin(), idx.end(), [&](int x, int y) { return w[x] < w[y]; });
   std::vector<double> ws(w, w + n);
   for (int j = 0; j < n; ++j) {
     w[j] = ws[idx[j]];
-    for (int k = 0; k < n; ++k) a[j * n + k] = z[k * n + idx[j]];
+    std::memcpy(a + (size_t)j * n, zt.data() + (size_t)idx[j] * n, sizeof(double) * n);
   }
 }
 

@@ -339,8 +339,64 @@ BlockCsr upper_part(const BlockCsr& a) {
   return u;
 }
 
+// Same result as upper_part(ord_mtr(symm_mtr(up), order)) -- the
+// reference's OrdMtrSymm -- in one pass: stored block (i, j) moves to
+// (order[i], order[j]) when that stays in the upper triangle, otherwise its
+// transpose moves to (order[j], order[i]).
 BlockCsr ord_mtr_symm(const BlockCsr& up, const std::vector<int>& order) {
-  return upper_part(ord_mtr(symm_mtr(up), order));
+  const int n = up.nsup;
+  BlockCsr r;
+  r.nsup = n;
+  r.sprnd.assign(n + 1, 0);
+  for (int i = 0; i < n; ++i) r.sprnd[order[i] + 1] = up.bla(i);
+  for (int i = 0; i < n; ++i) r.sprnd[i + 1] += r.sprnd[i];
+  struct Ent {
+    int col;
+    int src;
+    bool tr;
+  };
+  std::vector<int> cnt(n + 1, 0);
+  for (int i = 0; i < n; ++i)
+    for (int j = up.ia[i]; j < up.ia[i + 1]; ++j) cnt[std::min(order[i], order[up.ja[j]]) + 1]++;
+  r.ia.assign(n + 1, 0);
+  for (int i = 0; i < n; ++i) r.ia[i + 1] = r.ia[i] + cnt[i + 1];
+  std::vector<Ent> ent(r.ia[n]);
+  std::vector<int> pos(r.ia.begin(), r.ia.end() - 1);
+  for (int i = 0; i < n; ++i)
+    for (int j = up.ia[i]; j < up.ia[i + 1]; ++j) {
+      const int p = order[i], q = order[up.ja[j]];
+      if (p <= q) ent[pos[p]++] = Ent{q, j, false};
+      else ent[pos[q]++] = Ent{p, j, true};
+    }
+  r.ja.resize(ent.size());
+  r.bsa.resize(ent.size());
+  int64_t nz = 0;
+  for (int i = 0; i < n; ++i) {
+    std::sort(ent.begin() + r.ia[i], ent.begin() + r.ia[i + 1],
+              [](const Ent& a, const Ent& b) { return a.col < b.col; });
+    for (int k = r.ia[i]; k < r.ia[i + 1]; ++k) {
+      r.ja[k] = ent[k].col;
+      r.bsa[k] = nz;
+      nz += (int64_t)r.bla(i) * r.bla(ent[k].col);
+    }
+  }
+  r.a.resize(nz);
+  for (int i = 0; i < n; ++i)
+    for (int k = r.ia[i]; k < r.ia[i + 1]; ++k) {
+      const Ent& e = ent[k];
+      const int bi = r.bla(i), bj = r.bla(e.col);
+      const double* src = &up.a[up.bsa[e.src]];
+      double* dst = &r.a[r.bsa[k]];
+      if (!e.tr) {
+        std::memcpy(dst, src, sizeof(double) * bi * bj);
+      } else {
+        // source block: bj x bi (column-major); dst = its transpose, bi x bj
+        for (int rr = 0; rr < bj; ++rr)
+          for (int cc = 0; cc < bi; ++cc) dst[rr * bi + cc] = src[cc * bj + rr];
+      }
+    }
+  r.set_blamx();
+  return r;
 }
 
 BlockCsr extend_sparsity(const BlockCsr& a, const BlockCsr& b) {

@@ -413,7 +413,7 @@ def run_dropin(args, B):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=8, help="timed evaluation groups (learned) or solves (drop-in)")
+    ap.add_argument("--steps", type=int, default=6, help="timed evaluation groups (learned) or solves (drop-in)")
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", choices=["learned", "drop-in"], default="learned")
     ap.add_argument("--train-cells", type=int, default=1024)
