@@ -150,7 +150,8 @@ SIGNATURES = {
     "bcm_batch_layout": (C.c_int, [_P, _IP, C.POINTER(C.c_longlong), C.POINTER(C.c_longlong)]),
     "bcm_batch_solve": (C.c_int, [_P, _P, C.POINTER(SolveOpts), C.c_int, C.c_void_p, C.c_void_p, C.c_int,
                                   C.POINTER(BatchStats), _IP, _DP]),
-    "bcm_batch_autotune": (C.c_int, [_P, _P, C.POINTER(SolveOpts), C.c_int, C.c_void_p, C.POINTER(BatchTune)]),
+    "bcm_batch_autotune": (C.c_int, [_P, _P, C.POINTER(SolveOpts), C.c_int, C.c_void_p, C.c_int,
+                                     C.POINTER(BatchTune)]),
     "bcm_batch_get_tune": (C.c_int, [_P, C.POINTER(BatchTune)]),
     "bcm_batch_set_tune": (C.c_int, [_P, C.POINTER(BatchTune)]),
     "bcm_mlp_create": (C.c_int, [C.c_int, C.c_int, C.c_int, C.c_ulonglong, C.POINTER(_P)]),
@@ -517,9 +518,9 @@ class Batch:
                                      _ip(iters), _dp(relres)))
         return st, iters.reshape(self.ncell, nrhs), relres.reshape(self.ncell, nrhs)
 
-    def autotune(self, ctx: "Context", b, opts: SolveOpts, nrhs: int) -> BatchTune:
+    def autotune(self, ctx: "Context", b, opts: SolveOpts, nrhs: int, allow_graph: bool = True) -> BatchTune:
         t = BatchTune()
-        _check(lib().bcm_batch_autotune(ctx.h, self.h, C.byref(opts), nrhs, ptr(b), C.byref(t)))
+        _check(lib().bcm_batch_autotune(ctx.h, self.h, C.byref(opts), nrhs, ptr(b), int(allow_graph), C.byref(t)))
         return t
 
     def get_tune(self) -> BatchTune:

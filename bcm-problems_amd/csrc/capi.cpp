@@ -668,12 +668,12 @@ static void tune_out(const bcm::BatchTune& t, bcm_batch_tune* out) {
 }
 
 int bcm_batch_autotune(bcm_context c, bcm_batch b, const bcm_solve_opts* o, int nrhs, const double* rhs,
-                       bcm_batch_tune* out) {
+                       int allow_graph, bcm_batch_tune* out) {
   return guarded([&] {
     need(c && b && rhs, "batch_autotune: bad argument");
     bcm::SolveOptions opt = to_opts(o);
     opt.kind = bcm::kBlockPcg;
-    bcm::BatchTune t = bcm::batch_tune(b->b, c->c, opt, nrhs, rhs);
+    bcm::BatchTune t = bcm::batch_tune(b->b, c->c, opt, nrhs, rhs, allow_graph != 0);
     if (out) tune_out(t, out);
   });
 }

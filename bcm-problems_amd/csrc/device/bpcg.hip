@@ -478,7 +478,8 @@ void solve_nr(Batch& bt, Context& ctx, const SolveOptions& opt, const double* b,
     if (fallback)
       hipLaunchKernelGGL(k_bp_zero<NR>, dim3(bt.nchunks), dim3(256), 0, s, (int64_t)bt.n, bt.chunk_cell.as<int>(),
                          bt.chunk_lo.as<int>(), bt.chunk_hi.as<int>(), w.s, w.b, w.x, w.r);
-    const int every = std::max(1, tu.check_every);
+    // profiling counts launches: poll after every iteration so that none is queued past the end
+    const int every = ctx.prof ? 1 : std::max(1, tu.check_every);
     const bool use_graph = tu.graph && !ctx.prof;
     if (use_graph) {
       // every buffer the iteration touches exists before capture (no allocation inside it)
@@ -583,7 +584,7 @@ void batch_solve(Batch& bt, Context& ctx, const SolveOptions& opt, int nrhs, con
   st.seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
 }
 
-BatchTune batch_tune(Batch& bt, Context& ctx, const SolveOptions& opt, int nrhs, const double* b) {
+BatchTune batch_tune(Batch& bt, Context& ctx, const SolveOptions& opt, int nrhs, const double* b, bool allow_graph) {
   // Two stages: the sweep schedule (level-scheduled, or sync-free with a row
   // order and a launch shape) with eager launches, then graph replay and the
   // polling interval for the winner. Every candidate runs the same fixed
@@ -629,7 +630,7 @@ BatchTune batch_tune(Batch& bt, Context& ctx, const SolveOptions& opt, int nrhs,
     if (per < best.ms_per_iter) { best = c; best.ms_per_iter = per; }
   }
   const BatchTune sched = best;
-  for (int graph = 0; graph <= 1; ++graph)
+  for (int graph = 0; graph <= (allow_graph ? 1 : 0); ++graph)
     for (int every : {4, 16}) {
       if (graph == 0 && every == 16) continue;  // measured in stage one
       BatchTune c = sched;

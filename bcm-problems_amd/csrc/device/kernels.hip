@@ -138,12 +138,103 @@ __global__ __launch_bounds__(256) void k_spmv(int nsup, int n, const int* __rest
   }
 }
 
+// SpMV for several right-hand sides. The x values of a block row's columns
+// are staged through LDS first (one coalesced read of B values per block
+// column and right-hand side), so the inner loop issues a single vector load
+// of V matrix values per column and reads the NR x values of that column
+// from LDS. Lanes, column groups and the per-lane column sequence are those
+// of k_spmv; the same LDS area then holds the G partial sums of the row.
+template <int B, int V, int NR, class TA>
+__global__ __launch_bounds__(256) void k_spmv_mr(int nsup, int n, const int* __restrict__ ia,
+                                                 const int* __restrict__ ja, const TA* __restrict__ a,
+                                                 const double* __restrict__ x, double* __restrict__ y,
+                                                 const double* __restrict__ bsub, const double* __restrict__ wdot,
+                                                 double* partials, int rows_per_wave, const int* skip) {
+  if (skip && *skip) return;
+  constexpr int L = B / V;
+  constexpr int G = 64 / L;
+  constexpr int NRP = NR % 2 ? NR + 1 : NR;                          // LDS stride per column (16-byte rows)
+  constexpr int KB = (G * NR * B + B * NRP - 1) / (B * NRP);         // block columns per staged chunk
+  constexpr int WL = KB * B * NRP;                                   // doubles per wave (>= G * NR * B)
+  __shared__ __align__(16) double lds[4][WL];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int lr = lane % L, g = lane / L;
+  const bool act = g < G;
+  double* xs = lds[wave];
+  const int tile = xcd_tile(blockIdx.x, gridDim.x);
+  const int wave_global = __builtin_amdgcn_readfirstlane(tile * (blockDim.x >> 6) + wave);
+  double dotv = 0.0;
+  for (int q0 = 0; q0 < rows_per_wave; ++q0) {
+    const int row = wave_global * rows_per_wave + q0;
+    if (row >= nsup) break;
+    double acc[NR][V];
+#pragma unroll
+    for (int q = 0; q < NR; ++q)
+#pragma unroll
+      for (int k = 0; k < V; ++k) acc[q][k] = 0.0;
+    const int jb = ia[row], nblk = ia[row + 1] - jb;
+    const TA* ap = a + (size_t)jb * B * B + lr * V;
+    int t = g;  // this lane group's next column of the row's stream
+    for (int kb0 = 0; kb0 < nblk; kb0 += KB) {
+      const int nb = min(KB, nblk - kb0);
+      wave_lds_sync();  // the previous chunk (or row) is no longer read
+      for (int e = lane; e < nb * NR * B; e += 64) {
+        const int c = e % B, rest = e / B;
+        const int q = rest % NR, kb = rest / NR;
+        const int col = ja[jb + kb0 + kb];
+        xs[(kb * B + c) * NRP + q] = x[(size_t)q * n + (size_t)col * B + c];
+      }
+      wave_lds_sync();
+      if (act) {
+        const int t0 = kb0 * B, t1 = (kb0 + nb) * B;
+#pragma unroll 2
+        for (; t < t1; t += G) {
+          double av[V];
+          load_vec<V>(ap + (size_t)t * B, av);
+          const double* xp = xs + (t - t0) * NRP;
+          double xv[NRP];
+#pragma unroll
+          for (int q = 0; q < NRP; ++q) xv[q] = xp[q];
+#pragma unroll
+          for (int q = 0; q < NR; ++q)
+#pragma unroll
+            for (int k = 0; k < V; ++k) acc[q][k] += av[k] * xv[q];
+        }
+      }
+    }
+    wave_lds_sync();
+    if (act) {
+#pragma unroll
+      for (int q = 0; q < NR; ++q)
+#pragma unroll
+        for (int k = 0; k < V; ++k) xs[(g * NR + q) * B + lr * V + k] = acc[q][k];
+    }
+    wave_lds_sync();
+    if (lane < B) {
+      const size_t e = (size_t)row * B + lane;
+#pragma unroll
+      for (int q = 0; q < NR; ++q) {
+        double s = 0.0;
+#pragma unroll
+        for (int h = 0; h < G; ++h) s += xs[(h * NR + q) * B + lane];
+        const double v = bsub ? bsub[(size_t)q * n + e] - s : s;
+        y[(size_t)q * n + e] = v;
+        if (q == 0 && wdot) dotv += wdot[e] * v;
+      }
+    }
+  }
+  if (partials) {
+    const double vv[1] = {dotv};
+    block_partials<1>(vv, partials);
+  }
+}
+
 // ---------------------------------------------------------------------------
 // Level-scheduled triangular sweep: one launch per dependency level; every
 // row of a level depends only on rows of earlier levels, which earlier
 // launches on the same stream have completed, so no wave ever waits on
 // another. One wavefront per block row.
-template <int B, int NR, class TA, class TX>
+template <int B, int V, int NR, class TA, class TX>
 __global__ __launch_bounds__(256) void k_sweep_level(int nrows, const int* __restrict__ rows, int n,
                                                      const int* __restrict__ ia, const int* __restrict__ ja,
                                                      const TA* __restrict__ off, const TA* __restrict__ diag,
@@ -151,54 +242,99 @@ __global__ __launch_bounds__(256) void k_sweep_level(int nrows, const int* __res
                                                      const int* skip) {
   if (skip && *skip) return;
   using Acc = typename AccOf<TX>::type;
-  constexpr int G = 64 / B;
-  __shared__ Acc vs[4][NR][B];
+  static_assert(sizeof(Acc) == sizeof(TX), "staged values and partial sums share the LDS area");
+  constexpr int L = B / V;
+  constexpr int G = 64 / L;
+  constexpr int NRP = NR % 2 ? NR + 1 : NR;
+  constexpr int KB = (G * NR * B + B * NRP - 1) / (B * NRP);
+  constexpr int WL = KB * B * NRP;
+  __shared__ __align__(16) Acc lds[4][WL + NR * B];
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  const int idx = blockIdx.x * 4 + wave;
-  const int r = lane % B, g = lane / B;
-  const int row = idx < nrows ? rows[idx] : -1;
-  Acc acc[NR];
+  const int lr = lane % L, g = lane / L;
+  const bool act = g < G;
+  Acc* xs = lds[wave];
+  Acc* vs = lds[wave] + WL;
+  const int idx = __builtin_amdgcn_readfirstlane(blockIdx.x * 4 + wave);
+  if (idx >= nrows) return;  // wave-uniform; no block-level synchronisation below
+  const int row = rows[idx];
+  // v = x_row - sum_k F_rk y_k, with the y values of each chunk of block columns staged in LDS
+  Acc acc[NR][V];
 #pragma unroll
-  for (int q = 0; q < NR; ++q) acc[q] = Acc(0);
-  if (row >= 0 && g < G) {
-    const int jb = ia[row], je = ia[row + 1];
-    const TA* blk0 = off + (size_t)jb * B * B;
-    const int ncol = (je - jb) * B;
-#pragma unroll 4
-    for (int tt = g; tt < ncol; tt += G) {
-      const int bk = tt / B, c = tt - bk * B;
-      const int k = ja[jb + bk];
-      const Acc av = (Acc)ldnt(blk0 + (size_t)tt * B + r);
+  for (int q = 0; q < NR; ++q)
 #pragma unroll
-      for (int q = 0; q < NR; ++q) acc[q] += av * (Acc)y[(size_t)q * n + (size_t)k * B + c];
+    for (int k = 0; k < V; ++k) acc[q][k] = Acc(0);
+  const int jb = ia[row], nblk = ia[row + 1] - jb;
+  const TA* ap = off + (size_t)jb * B * B + lr * V;
+  int t = g;
+  for (int kb0 = 0; kb0 < nblk; kb0 += KB) {
+    const int nb = min(KB, nblk - kb0);
+    wave_lds_sync();
+    for (int e = lane; e < nb * NR * B; e += 64) {
+      const int c = e % B, rest = e / B;
+      const int q = rest % NR, kb = rest / NR;
+      xs[(kb * B + c) * NRP + q] = (Acc)y[(size_t)q * n + (size_t)ja[jb + kb0 + kb] * B + c];
+    }
+    wave_lds_sync();
+    if (act) {
+      const int t0 = kb0 * B, t1 = (kb0 + nb) * B;
+      for (; t < t1; t += G) {
+        double av[V];
+        load_vec<V>(ap + (size_t)t * B, av);
+        const Acc* xp = xs + (t - t0) * NRP;
+#pragma unroll
+        for (int q = 0; q < NR; ++q)
+#pragma unroll
+          for (int k = 0; k < V; ++k) acc[q][k] += (Acc)av[k] * xp[q];
+      }
     }
   }
+  wave_lds_sync();
+  if (act) {
 #pragma unroll
-  for (int q = 0; q < NR; ++q) {
-    Acc s = acc[q];
+    for (int q = 0; q < NR; ++q)
 #pragma unroll
-    for (int h = 1; h < G; ++h) s += __shfl(acc[q], lane + h * B);
-    if (row >= 0 && lane < B) vs[wave][q][lane] = (Acc)x[(size_t)q * n + (size_t)row * B + lane] - s;
+      for (int k = 0; k < V; ++k) xs[(g * NR + q) * B + lr * V + k] = acc[q][k];
   }
-  __syncthreads();
-  Acc z[NR];
+  wave_lds_sync();
+  if (lane < B) {
 #pragma unroll
-  for (int q = 0; q < NR; ++q) z[q] = Acc(0);
-  if (row >= 0 && g < G) {
-    const TA* d = diag + (size_t)row * B * B;
+    for (int q = 0; q < NR; ++q) {
+      Acc sm = Acc(0);
 #pragma unroll
+      for (int h = 0; h < G; ++h) sm += xs[(h * NR + q) * B + lane];
+      vs[q * B + lane] = (Acc)x[(size_t)q * n + (size_t)row * B + lane] - sm;
+    }
+  }
+  wave_lds_sync();
+  // y_row = D v (inverted diagonal block, column-major as stored for this sweep)
+#pragma unroll
+  for (int q = 0; q < NR; ++q)
+#pragma unroll
+    for (int k = 0; k < V; ++k) acc[q][k] = Acc(0);
+  if (act) {
+    const TA* dp = diag + (size_t)row * B * B + lr * V;
     for (int c = g; c < B; c += G) {
-      const Acc dv = (Acc)ldnt(d + c * B + r);
+      double dv[V];
+      load_vec<V>(dp + (size_t)c * B, dv);
 #pragma unroll
-      for (int q = 0; q < NR; ++q) z[q] += dv * vs[wave][q][c];
+      for (int q = 0; q < NR; ++q)
+#pragma unroll
+        for (int k = 0; k < V; ++k) acc[q][k] += (Acc)dv[k] * vs[q * B + c];
     }
+#pragma unroll
+    for (int q = 0; q < NR; ++q)
+#pragma unroll
+      for (int k = 0; k < V; ++k) xs[(g * NR + q) * B + lr * V + k] = acc[q][k];
   }
+  wave_lds_sync();
+  if (lane < B) {
 #pragma unroll
-  for (int q = 0; q < NR; ++q) {
-    Acc s = z[q];
+    for (int q = 0; q < NR; ++q) {
+      Acc sm = Acc(0);
 #pragma unroll
-    for (int h = 1; h < G; ++h) s += __shfl(z[q], lane + h * B);
-    if (row >= 0 && lane < B) y[(size_t)q * n + (size_t)row * B + lane] = (TX)s;
+      for (int h = 0; h < G; ++h) sm += xs[(h * NR + q) * B + lane];
+      y[(size_t)q * n + (size_t)row * B + lane] = (TX)sm;
+    }
   }
 }
 
@@ -249,6 +385,52 @@ __global__ void k_sweep_fill(int64_t n, TX* __restrict__ y, const int* skip) {
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) yb[i] = Sentinel<TX>::bits;
 }
 
+// acc[q] += sum_i w[i] * y(column t0 + i*G of the row's block stream, rhs q)
+// for the CNT columns this lane owns from t0 on; waits (bounded) for values
+// still holding the sentinel. Columns are added in ascending order.
+template <int CNT, int NR, int B, int G, class Acc, class TX>
+__device__ __forceinline__ void gather_acc(int t0, int ncol, int jb, const int* __restrict__ ja, TX* y, int n,
+                                           const Acc (&w)[CNT], Acc (&acc)[NR], int* ctl, int& spins) {
+  int src[CNT];  // element offsets in y (-1: column beyond the row)
+  TX yv[CNT][NR];
+#pragma unroll
+  for (int i = 0; i < CNT; ++i) {
+    const int tt = t0 + i * G;
+    const int bk = tt / B, c = tt - bk * B;
+    src[i] = tt < ncol ? ja[jb + bk] * B + c : -1;
+  }
+#pragma unroll
+  for (int i = 0; i < CNT; ++i)
+#pragma unroll
+    for (int q = 0; q < NR; ++q) yv[i][q] = src[i] >= 0 ? ld_coherent(y + src[i] + (size_t)q * n) : TX(0);
+  bool miss = false;
+#pragma unroll
+  for (int i = 0; i < CNT; ++i)
+#pragma unroll
+    for (int q = 0; q < NR; ++q) miss |= is_sentinel(yv[i][q]);
+  while (miss) {  // dependencies not all published yet
+    __builtin_amdgcn_s_sleep(1);
+    miss = false;
+#pragma unroll
+    for (int i = 0; i < CNT; ++i)
+#pragma unroll
+      for (int q = 0; q < NR; ++q)
+        if (is_sentinel(yv[i][q])) {
+          yv[i][q] = ld_coherent(y + src[i] + (size_t)q * n);
+          miss |= is_sentinel(yv[i][q]);
+        }
+    if (++spins > kSpinLimit ||
+        ((spins & 1023) == 0 && __hip_atomic_load(&ctl[2], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))) {
+      __hip_atomic_store(&ctl[2], 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      break;
+    }
+  }
+#pragma unroll
+  for (int i = 0; i < CNT; ++i)
+#pragma unroll
+    for (int q = 0; q < NR; ++q) acc[q] += w[i] * (Acc)yv[i][q];
+}
+
 template <int B, int NR, class TA, class TX>
 __global__ __launch_bounds__(64) void k_sweep_free(int nsup, int n, int forward, const int* __restrict__ order,
                                                    const int* __restrict__ ia,
@@ -261,6 +443,7 @@ __global__ __launch_bounds__(64) void k_sweep_free(int nsup, int n, int forward,
   // off-diagonal values prefetched per lane: three blocks (one per extra right-hand side set)
   constexpr int PF = ((NR == 1 ? 3 : 1) * B + G - 1) / G;
   constexpr int PD = (B + G - 1) / G;       // diagonal-block columns per lane
+  constexpr int CH = NR == 1 ? 8 : 24 / NR;  // columns per lane and chunk after the prefetched ones
   __shared__ Acc v[NR][B];
   const int lane = threadIdx.x;
   const int r = lane % B, g = lane / B;
@@ -284,64 +467,19 @@ __global__ __launch_bounds__(64) void k_sweep_free(int nsup, int n, int forward,
     for (int q = 0; q < NR; ++q) acc[q] = Acc(0);
     if (act) {
       int spins = 0;
-      // issue every dependency load at once, then re-poll only the values still
-      // holding the sentinel (one round trip when the dependencies are done)
-      int src[PF];  // element offsets in y (-1: column beyond the row)
-      TX yv[PF][NR];
+      // the prefetched columns, then the rest in chunks of CH columns per lane:
+      // every load of a chunk is issued at once and only the values still
+      // holding the sentinel are polled again (one round trip per chunk when
+      // the dependencies are done)
+      gather_acc<PF, NR, B, G>(g, ncol, jb, ja, y, n, pf, acc, ctl, spins);
+      for (int t0 = g + PF * G; t0 < ncol; t0 += CH * G) {
+        Acc av[CH];
 #pragma unroll
-      for (int i = 0; i < PF; ++i) {
-        const int tt = g + i * G;
-        const int bk = tt / B, c = tt - bk * B;
-        src[i] = tt < ncol ? ja[jb + bk] * B + c : -1;
-      }
-#pragma unroll
-      for (int i = 0; i < PF; ++i)
-#pragma unroll
-        for (int q = 0; q < NR; ++q) yv[i][q] = src[i] >= 0 ? ld_coherent(y + src[i] + (size_t)q * n) : TX(0);
-      bool miss = false;
-#pragma unroll
-      for (int i = 0; i < PF; ++i)
-#pragma unroll
-        for (int q = 0; q < NR; ++q) miss |= is_sentinel(yv[i][q]);
-      while (miss) {  // dependencies not all published yet
-        __builtin_amdgcn_s_sleep(1);
-        miss = false;
-#pragma unroll
-        for (int i = 0; i < PF; ++i)
-#pragma unroll
-          for (int q = 0; q < NR; ++q)
-            if (is_sentinel(yv[i][q])) {
-              yv[i][q] = ld_coherent(y + src[i] + (size_t)q * n);
-              miss |= is_sentinel(yv[i][q]);
-            }
-        if (++spins > kSpinLimit ||
-            ((spins & 1023) == 0 && __hip_atomic_load(&ctl[2], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))) {
-          __hip_atomic_store(&ctl[2], 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-          break;
+        for (int i = 0; i < CH; ++i) {
+          const int tt = t0 + i * G;
+          av[i] = tt < ncol ? (Acc)ldnt(blk0 + (size_t)tt * B + r) : Acc(0);
         }
-      }
-#pragma unroll
-      for (int i = 0; i < PF; ++i)
-#pragma unroll
-        for (int q = 0; q < NR; ++q) acc[q] += pf[i] * (Acc)yv[i][q];
-      for (int tt = g + PF * G; tt < ncol; tt += G) {  // dependencies beyond the prefetched ones
-        const int bk = tt / B, c = tt - bk * B;
-        const int k = ja[jb + bk];
-        const Acc av = (Acc)ldnt(blk0 + (size_t)tt * B + r);
-        for (int q = 0; q < NR; ++q) {
-          const TX* src = &y[(size_t)q * n + (size_t)k * B + c];
-          TX yv = ld_coherent(src);
-          while (is_sentinel(yv)) {
-            __builtin_amdgcn_s_sleep(1);
-            yv = ld_coherent(src);
-            if (++spins > kSpinLimit || ((spins & 1023) == 0 &&
-                __hip_atomic_load(&ctl[2], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))) {
-              __hip_atomic_store(&ctl[2], 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-              break;
-            }
-          }
-          acc[q] += av * (Acc)yv;
-        }
+        gather_acc<CH, NR, B, G>(t0, ncol, jb, ja, y, n, av, acc, ctl, spins);
       }
     }
 #pragma unroll
@@ -671,11 +809,10 @@ void launch_finalize(RedWork& rw, double* out, hipStream_t s, const int* skip) {
 // ---------------------------------------------------------------------------
 // Launch wrappers
 
-// Vector width of the SpMV loads: four values where the block size allows
-// (two for six right-hand sides, to bound the LDS of the reduction).
-template <int B, int NR>
+// Vector width of the SpMV loads: four values where the block size allows.
+template <int B>
 constexpr int spmv_width() {
-  return (B % 4 == 0 && NR <= 3) ? 4 : (B % 2 == 0 ? 2 : 1);
+  return B % 4 == 0 ? 4 : (B % 2 == 0 ? 2 : 1);
 }
 
 template <int B, int NR, class TA>
@@ -693,8 +830,14 @@ static void spmv_b(const DevBsr& A, const double* x, double* y, const double* bs
     if (grid > kRedBlocks * kMaxDots) throw Error(kErrInternal, "spmv: too many blocks for the fused dot");
     partials = rw.partials.as<double>();
   }
-  hipLaunchKernelGGL((k_spmv<B, spmv_width<B, NR>(), NR, TA>), dim3(grid), dim3(64 * kWavesPerBlock), 0, s, A.nsup, A.n,
-                     A.ia.as<int>(), A.ja.as<int>(), A.a.as<TA>(), x, y, bsub, wdot, partials, rows_per_wave, skip);
+  if constexpr (NR == 1)
+    hipLaunchKernelGGL((k_spmv<B, spmv_width<B>(), 1, TA>), dim3(grid), dim3(64 * kWavesPerBlock), 0, s, A.nsup,
+                       A.n, A.ia.as<int>(), A.ja.as<int>(), A.a.as<TA>(), x, y, bsub, wdot, partials, rows_per_wave,
+                       skip);
+  else
+    hipLaunchKernelGGL((k_spmv_mr<B, spmv_width<B>(), NR, TA>), dim3(grid), dim3(64 * kWavesPerBlock), 0, s, A.nsup,
+                       A.n, A.ia.as<int>(), A.ja.as<int>(), A.a.as<TA>(), x, y, bsub, wdot, partials, rows_per_wave,
+                       skip);
   BCM_HIP(hipGetLastError());
   if (dot_out) {
     rw.pending_blocks = grid;
@@ -771,7 +914,8 @@ static void sweep_b(const DevTri& T, const void* x, void* y, hipStream_t s, cons
   const int* rows = T.level_rows.as<int>();
   for (int l = 0; l < T.levels; ++l) {
     const int r0 = T.level_ptr[l], nr = T.level_ptr[l + 1] - r0;
-    hipLaunchKernelGGL((k_sweep_level<B, NR, TA, TX>), dim3((nr + 3) / 4), dim3(256), 0, s, nr, rows + r0, T.n,
+    hipLaunchKernelGGL((k_sweep_level<B, spmv_width<B>(), NR, TA, TX>), dim3((nr + 3) / 4), dim3(256), 0, s, nr,
+                       rows + r0, T.n,
                        T.ia.as<int>(), T.ja.as<int>(), T.off.as<TA>(), T.diag.as<TA>(), static_cast<const TX*>(x),
                        static_cast<TX*>(y), skip);
   }

@@ -201,7 +201,8 @@ void batch_solve(Batch& bt, Context& ctx, const SolveOptions& opt, int nrhs, con
                  bool fallback, BatchStats& st, int* iters, double* relres);
 // Times the candidate schedules on this batch (a few iterations each, on
 // scratch copies of b) and keeps the fastest in bt.tune.
-BatchTune batch_tune(Batch& bt, Context& ctx, const SolveOptions& opt, int nrhs, const double* b);
+BatchTune batch_tune(Batch& bt, Context& ctx, const SolveOptions& opt, int nrhs, const double* b,
+                     bool allow_graph = true);
 
 // Two-sided Lanczos driver (lanczos.hip).
 void solve_lanczos(Matrix& A, Factor& F, Context& ctx, const SolveOptions& opt, const double* b, double* x,

@@ -187,7 +187,7 @@ def run_learned(args, B):
         th = threading.Thread(target=prefetch, args=(nxt, 0b010))
         th.start()
         if tune is None:
-            t = bt.autotune(ctx, db, opts, 6)
+            t = bt.autotune(ctx, db, opts, 6, allow_graph=not args.no_graph)
             tune = dict(sweep_mode=t.sweep_mode, waves_per_cu=t.waves_per_cu, level_order=t.level_order,
                         graph=t.graph, check_every=t.check_every)
             log(f"autotuned schedule {tune} ({t.ms_per_iter:.2f} ms/iteration)")
@@ -420,6 +420,7 @@ def main():
     ap.add_argument("--fit-steps", type=int, default=4000)
     ap.add_argument("--plain-cells", type=int, default=4)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-graph", action="store_true", help="leave hipGraph replay out of the batch schedules")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; multi-GPU runs are not part of it",

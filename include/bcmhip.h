@@ -199,9 +199,10 @@ int bcm_batch_layout(bcm_batch b, int* ncell, long long* n, long long* elem0);
  * (cell-major) or NULL. nrhs is 1, 3 or 6. */
 int bcm_batch_solve(bcm_context c, bcm_batch b, const bcm_solve_opts* o, int nrhs, const double* rhs, double* x,
                     int fallback, bcm_batch_stats* st, int* iters, double* relres);
-/* Times the candidate schedules on this batch and keeps the fastest. */
+/* Times the candidate schedules on this batch and keeps the fastest;
+ * allow_graph = 0 leaves hipGraph replay out of the candidates. */
 int bcm_batch_autotune(bcm_context c, bcm_batch b, const bcm_solve_opts* o, int nrhs, const double* rhs,
-                       bcm_batch_tune* out);
+                       int allow_graph, bcm_batch_tune* out);
 int bcm_batch_get_tune(bcm_batch b, bcm_batch_tune* t);
 int bcm_batch_set_tune(bcm_batch b, const bcm_batch_tune* t);
 

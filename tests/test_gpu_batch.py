@@ -5,9 +5,8 @@ the oracle's Cg on that cell alone (cg.cpp:31-349 restated in the oracle):
 iteration count within one (the batched dots add chunk partials in a
 different order) and relative solution difference <= 1e-8; FP32/BF16 factor
 sweeps (flexible CG, FP64 outer recurrence) reach FP64 true relative
-residual <= 1e-10 and stay within 1e-8 of the FP64 oracle solution. All
-launch schedules (level-scheduled or sync-free sweeps, eager or hipGraph
-replay) give bit-identical results.
+residual <= 1e-10 and stay within 1e-8 of the FP64 oracle solution. Eager
+and hipGraph-replayed launches of a schedule give bit-identical results.
 """
 
 import numpy as np
@@ -94,21 +93,27 @@ def test_batch_mixed_precision(B, ctx, batch, cells, oracle_sol, prec):
         assert np.all(its[c] <= ito + (3 if prec == "F32" else ito + 10)), (c, its[c], ito)
 
 
-def test_batch_schedules_are_bit_identical(B, ctx, batch, cells):
-    ref = None
-    for mode, graph, every in [(1, 0, 8), (0, 0, 8), (1, 1, 4), (0, 1, 16), (1, 0, 1)]:
+def test_batch_schedules_agree(B, ctx, batch, cells):
+    """Graph replay and eager launches of one sweep schedule are bit-identical;
+    the level-scheduled and sync-free sweeps add the same terms in different
+    orders, so across them the solutions agree to rounding."""
+    ref, first = {}, {}
+    for mode, graph, every in [(1, 0, 8), (1, 1, 4), (1, 0, 1), (0, 0, 8), (0, 1, 16), (0, 0, 3)]:
         print(f"schedule mode={mode} graph={graph} every={every}", flush=True)
         batch.set_tune(sweep_mode=mode, graph=graph, check_every=every, waves_per_cu=8)
         for prec in (B.F64, B.F32):
             xs, st, its, rr = run(B, ctx, batch, cells, eps=1e-10, precision=prec)
-            assert st.converged == st.pairs
-            got = (np.concatenate([x.ravel() for x in xs]), its.copy())
-            if ref is None:
-                ref = {}
-            if prec in ref:
-                assert np.array_equal(got[0], ref[prec][0]) and np.array_equal(got[1], ref[prec][1]), (mode, graph)
+            assert st.converged == st.pairs and rr.max() <= 1e-10
+            x = np.concatenate([v.ravel() for v in xs])
+            key = (mode, prec)
+            if key in ref:
+                assert np.array_equal(x, ref[key][0]) and np.array_equal(its, ref[key][1]), (mode, graph, every)
             else:
-                ref[prec] = got
+                ref[key] = (x, its.copy())
+            if prec in first:
+                assert rel(x, first[prec][0]) < 1e-8 and np.all(np.abs(its - first[prec][1]) <= 1)
+            else:
+                first[prec] = (x, its.copy())
     batch.set_tune(sweep_mode=1, graph=0, check_every=8, waves_per_cu=8)
 
 
