@@ -138,6 +138,40 @@ __global__ __launch_bounds__(256) void k_spmv(int nsup, int n, const int* __rest
   }
 }
 
+// Stages the vector values of nb block columns (cols[0..nb)) for NR
+// right-hand sides into this wave's LDS area: xs[(kb*B + c)*NRP + q]. Each
+// lane issues U loads before their LDS stores (a chunk of up to 64*U values
+// costs one memory round trip; a small U keeps short-lived waves light).
+// Brackets itself with wave-level LDS ordering.
+template <int B, int NR, int NRP, int KB, int U, class TV, class TS>
+__device__ __forceinline__ void stage_chunk(int lane, int nb, const int* __restrict__ cols, const TV* __restrict__ v,
+                                            int n, TS* xs) {
+  const int cnt = nb * NR * B;
+  wave_lds_sync();  // earlier reads of the area are done
+  for (int base = 0; base < cnt; base += 64 * U) {
+    TV val[U];
+#pragma unroll
+    for (int i = 0; i < U; ++i) {
+      const int e = base + lane + 64 * i;
+      if (e < cnt) {
+        const int c = e % B, rest = e / B;
+        const int q = rest % NR, kb = rest / NR;
+        val[i] = v[(size_t)q * n + (size_t)cols[kb] * B + c];
+      }
+    }
+#pragma unroll
+    for (int i = 0; i < U; ++i) {
+      const int e = base + lane + 64 * i;
+      if (e < cnt) {
+        const int c = e % B, rest = e / B;
+        const int q = rest % NR, kb = rest / NR;
+        xs[(kb * B + c) * NRP + q] = (TS)val[i];
+      }
+    }
+  }
+  wave_lds_sync();
+}
+
 // SpMV for several right-hand sides. The x values of a block row's columns
 // are staged through LDS first (one coalesced read of B values per block
 // column and right-hand side), so the inner loop issues a single vector load
@@ -177,14 +211,7 @@ __global__ __launch_bounds__(256) void k_spmv_mr(int nsup, int n, const int* __r
     int t = g;  // this lane group's next column of the row's stream
     for (int kb0 = 0; kb0 < nblk; kb0 += KB) {
       const int nb = min(KB, nblk - kb0);
-      wave_lds_sync();  // the previous chunk (or row) is no longer read
-      for (int e = lane; e < nb * NR * B; e += 64) {
-        const int c = e % B, rest = e / B;
-        const int q = rest % NR, kb = rest / NR;
-        const int col = ja[jb + kb0 + kb];
-        xs[(kb * B + c) * NRP + q] = x[(size_t)q * n + (size_t)col * B + c];
-      }
-      wave_lds_sync();
+      stage_chunk<B, NR, NRP, KB, (KB * NR * B + 63) / 64>(lane, nb, ja + jb + kb0, x, n, xs);
       if (act) {
         const int t0 = kb0 * B, t1 = (kb0 + nb) * B;
 #pragma unroll 2
@@ -268,13 +295,7 @@ __global__ __launch_bounds__(256) void k_sweep_level(int nrows, const int* __res
   int t = g;
   for (int kb0 = 0; kb0 < nblk; kb0 += KB) {
     const int nb = min(KB, nblk - kb0);
-    wave_lds_sync();
-    for (int e = lane; e < nb * NR * B; e += 64) {
-      const int c = e % B, rest = e / B;
-      const int q = rest % NR, kb = rest / NR;
-      xs[(kb * B + c) * NRP + q] = (Acc)y[(size_t)q * n + (size_t)ja[jb + kb0 + kb] * B + c];
-    }
-    wave_lds_sync();
+    stage_chunk<B, NR, NRP, KB, 1>(lane, nb, ja + jb + kb0, y, n, xs);
     if (act) {
       const int t0 = kb0 * B, t1 = (kb0 + nb) * B;
       for (; t < t1; t += G) {
