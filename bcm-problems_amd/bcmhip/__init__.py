@@ -26,7 +26,7 @@ LIB_PATH = os.path.join(os.path.dirname(_HERE), "lib", "libbcmhip.so")
 
 OK = 0
 E_ARG, E_NOT_SPD, E_WORKSPACE, E_BREAKDOWN, E_HIP, E_NOCONV, E_NODEVICE, E_INTERNAL = range(1, 9)
-PCG, BICGSTAB, FGMRES, LANCZOS, BLOCKPCG = 0, 1, 2, 3, 5
+PCG, BICGSTAB, FGMRES, LANCZOS, CGPAR, BLOCKPCG = 0, 1, 2, 3, 4, 5
 F64, F32, BF16 = 0, 1, 2
 
 

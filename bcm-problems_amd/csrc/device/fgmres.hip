@@ -374,8 +374,4 @@ void solve_fgmres(Matrix& A, Factor& F, Context& ctx, const SolveOptions& opt, c
   st.converged = std::sqrt(rr) <= thr ? 1 : 0;
 }
 
-void solve_lanczos(Matrix&, Factor&, Context&, const SolveOptions&, const double*, double*, SolveStats&) {
-  throw Error(kErrArg, "Lanczos driver not built yet");
-}
-
 }  // namespace bcm

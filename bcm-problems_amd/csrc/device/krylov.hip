@@ -28,8 +28,9 @@ enum ScalarOp : int {
   kOpBicgSCheck = 12,
   kOpBicgOmega = 13,
   kOpBicgNext = 14,
-  kOpCgParInit = 20,
-  kOpCgParStep = 21,
+  kOpCgParAlpha = 20,
+  kOpCgParCheck = 21,
+  kOpCgParBeta = 22,
 };
 
 // Scalar slots (KState::sc)
@@ -39,6 +40,8 @@ enum : int {
   // BiCGStab
   kRho = 0, kRhoOld = 1, kBAlpha = 2, kOmega = 3, kRv = 4, kSs = 5, kTs = 6, kTt = 7, kRr = 8,
   kRhoNew = 10,
+  // parallel-form CG
+  kCpUau = 11, kCpRu = 12, kCpAcoef = 13, kCpBcoef = 14, kCpRr = 15, kCpVau = 16, kCpThr2 = 17,
   // shared
   kBnorm2 = 20, kR0norm2 = 21, kTrue0 = 22, kTrue1 = 23,
 };
@@ -121,6 +124,21 @@ __global__ __launch_bounds__(256) void k_scalar(int op, KState* ks, double eps2,
       if (sc[kRho] == 0.0 || !std::isfinite(sc[kRho])) { st[0] = 1; st[2] = 1; return; }
       break;
     }
+    case kOpCgParAlpha: {  // slots kCpUau, kCpRu hold (Au, u), (r, u)
+      if (sc[kCpUau] < 0.0) { st[0] = 1; st[2] = 1; return; }  // "Matrix is not positive definite"
+      sc[kCpAcoef] = (sc[kCpRu] * -1.0) / sc[kCpUau];
+      ++st[1];
+      break;
+    }
+    case kOpCgParCheck: {  // slot kCpRr holds (r, r) after the residual update
+      if (!(sc[kCpRr] > sc[kCpThr2])) { st[0] = 1; st[4] = 1; st[6] = 1; return; }  // converged: last x step pending
+      if (st[1] + 1 >= niter) { st[0] = 1; return; }
+      break;
+    }
+    case kOpCgParBeta: {  // slot kCpVau holds (v, Au)
+      sc[kCpBcoef] = (sc[kCpVau] * -1.0) / sc[kCpUau];
+      break;
+    }
     default: break;
   }
 }
@@ -196,6 +214,57 @@ __global__ __launch_bounds__(256) void k_bicg_xr(int64_t n, const KState* ks, co
     acc[1] += rhat[i] * ri;
   }
   block_partials<2>(acc, partials);  // r.r and rhat.r, added by the next scalar step
+}
+
+// ---- parallel-form CG vector kernels (cg.cpp:355-771; r = A x - b) ----
+__global__ void k_cp_sub(int64_t n, double* __restrict__ r, const double* __restrict__ b) {
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) r[i] = r[i] - b[i];
+}
+
+__global__ __launch_bounds__(256) void k_cp_r(int64_t n, const KState* ks, const double* __restrict__ au,
+                                              double* __restrict__ r, double* partials) {
+  if (ks->st[0]) return;
+  const double a = ks->sc[kCpAcoef];
+  double acc[1] = {0.0};
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+    const double ri = r[i] + a * au[i];
+    r[i] = ri;
+    acc[0] += ri * ri;
+  }
+  block_partials<1>(acc, partials);
+}
+
+__global__ void k_cp_su(int64_t n, const KState* ks, const double* __restrict__ v, double* __restrict__ x,
+                        double* __restrict__ u) {
+  if (ks->st[0]) return;
+  const double a = ks->sc[kCpAcoef], bc = ks->sc[kCpBcoef];
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+    const double ui = u[i];
+    x[i] = x[i] + a * ui;
+    u[i] = v[i] + bc * ui;
+  }
+}
+
+// The step x += acoef u that the reference skips when it leaves at the test.
+__global__ void k_cp_last(int64_t n, const KState* ks, const double* __restrict__ u, double* __restrict__ x) {
+  if (!ks->st[6]) return;
+  const double a = ks->sc[kCpAcoef];
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) x[i] = x[i] + a * u[i];
+}
+
+__global__ void k_move2(KState* ks, int from, int to) { ks->sc[to] = ks->sc[from]; }
+
+__global__ void k_cp_init(KState* ks, double eps, int sttype) {
+  double* sc = ks->sc;
+  const double r0 = sc[kCpRr];
+  sc[kR0norm2] = r0;
+  if (sttype == 1) sc[kCpThr2] = eps * eps * r0;
+  else if (sttype == 3) sc[kCpThr2] = eps * eps * sc[kBnorm2];
+  else sc[kCpThr2] = eps * eps;
 }
 
 int ew_grid(int64_t n) {
@@ -447,6 +516,79 @@ void pcg(Matrix& A, Factor& F, Context& ctx, const SolveOptions& opt, const doub
   st.iterations = total;
 }
 
+// Parallel-form CG (CSVector::Cg with operator plug-ins, cg.cpp:355-771) on one
+// device: the (Au,u), (r,u) pair and (v,Au) are the reference's reductions
+// (its 2-element and 1-element allreduces). Stopping: ittype 0 -> ||r|| <=
+// eps ||r0|| (sttype 1), 1 -> ||r|| <= eps (sttype 0), 3 -> ||r|| <= eps ||b||.
+// At a converged exit the reference has not yet added the last acoef u to x
+// (cg.cpp:607-614 against :660-661); this driver adds it, so the returned x
+// is the iterate whose residual met the test.
+void cg_par(Matrix& A, Factor& F, Context& ctx, const SolveOptions& opt, const double* b, double* x,
+            SolveStats& st) {
+  const int64_t n = A.d.n;
+  const size_t vb = (size_t)n * 8;
+  double* r = ctx.vec(0, vb);
+  double* u = ctx.vec(1, vb);
+  double* au = ctx.vec(2, vb);
+  double* v = ctx.vec(3, vb);
+  KState* ks = ctx.dstate();
+  const int* done = &ks->st[0];
+  hipStream_t s = ctx.stream;
+  const int ge = ew_grid(n);
+  const int sttype = opt.ittype == 0 ? 1 : opt.ittype == 3 ? 3 : 0;
+  hipLaunchKernelGGL(k_state_reset, dim3(1), dim3(64), 0, s, ks, 0);
+  launch_spmv(A.d, x, r, nullptr, 1, nullptr, nullptr, ctx.rw, s);
+  hipLaunchKernelGGL(k_cp_sub, dim3(ge), dim3(256), 0, s, n, r, b);
+  {
+    const double* xs[2] = {r, b};
+    launch_multidot((int)n, 2, xs, xs, &ks->sc[kCpRr], ctx.rw, s);  // kCpRr: r.r, kCpVau(16): b.b
+    hipLaunchKernelGGL(k_move2, dim3(1), dim3(1), 0, s, ks, kCpVau, kBnorm2);
+    hipLaunchKernelGGL(k_cp_init, dim3(1), dim3(1), 0, s, ks, opt.eps, sttype);
+  }
+  precond_apply(F, ctx, r, u, opt.prec, false, 1);
+  const bool exact = opt.check_every == 1;
+  while (true) {
+    for (int c = 0; c < std::max(1, opt.check_every); ++c) {
+      {
+        ProfScope ps(ctx, kProfSpmv, spmv_bytes(A.d, 1));
+        launch_spmv(A.d, u, au, nullptr, 1, nullptr, nullptr, ctx.rw, s, done);
+      }
+      {
+        const double* xs[2] = {au, r};
+        const double* ys[2] = {u, u};
+        launch_multidot((int)n, 2, xs, ys, kDeferred, ctx.rw, s, done);
+      }
+      scalar_step(ctx, kOpCgParAlpha, 0.0, opt.ittype, opt.niter, kCpUau);
+      hipLaunchKernelGGL(k_cp_r, dim3(ge), dim3(256), 0, s, n, ks, au, r, ctx.rw.partials.as<double>());
+      set_pending(ctx, ge, 1);
+      scalar_step(ctx, kOpCgParCheck, 0.0, opt.ittype, opt.niter, kCpRr);
+      precond_apply(F, ctx, r, v, opt.prec, false, 1, done);
+      {
+        const double* xs[1] = {v};
+        const double* ys[1] = {au};
+        launch_multidot((int)n, 1, xs, ys, kDeferred, ctx.rw, s, done);
+      }
+      scalar_step(ctx, kOpCgParBeta, 0.0, opt.ittype, opt.niter, kCpVau);
+      hipLaunchKernelGGL(k_cp_su, dim3(ge), dim3(256), 0, s, n, ks, v, x, u);
+      if (exact) {
+        ctx.pull_state();
+        if (ctx.host_state->st[0]) break;
+      }
+    }
+    ctx.pull_state();
+    if (ctx.host_state->st[0]) break;
+  }
+  hipLaunchKernelGGL(k_cp_last, dim3(ge), dim3(256), 0, s, n, ks, u, x);
+  ctx.pull_state();
+  const KState hs = *ctx.host_state;
+  st.iterations = hs.st[1];
+  st.breakdown = hs.st[2];
+  st.relres_rec = hs.sc[kBnorm2] > 0 ? std::sqrt(hs.sc[kCpRr] / hs.sc[kBnorm2]) : std::sqrt(hs.sc[kCpRr]);
+  TrueResidual tr = true_residual(A, ctx, b, x, r);
+  st.relres_true = tr.relres;
+  st.converged = hs.st[4] ? 1 : 0;
+}
+
 void bicgstab(Matrix& A, Factor& F, Context& ctx, const SolveOptions& opt, const double* b, double* x,
               SolveStats& st) {
   const int n = A.d.n;
@@ -545,6 +687,7 @@ void solve(Matrix& A, Factor& F, Context& ctx, const SolveOptions& opt, const do
   switch (opt.kind) {
     case kPcg: pcg(A, F, ctx, opt, b, x, st); break;
     case kBicgstab: bicgstab(A, F, ctx, opt, b, x, st); break;
+    case kCgPar: cg_par(A, F, ctx, opt, b, x, st); break;
     case kFgmres: solve_fgmres(A, F, ctx, opt, b, x, st); break;
     case kLanczos: solve_lanczos(A, F, ctx, opt, b, x, st); break;
     default: throw Error(kErrArg, "solver kind not available");

@@ -184,7 +184,9 @@ int run(bool symmetric, bcm_params* p, const Callbacks& cb) {
     opt.eps = prm.eps;
     // Cg's stopping rules (cg.cpp:238-241): 0 relative, 1 absolute, 2 max-norm
     // (absolute, norm vector of ones); the GPU drivers test the 2-norm for 2 as well.
-    opt.ittype = symmetric ? (prm.ittype == 0 ? 0 : prm.ittype == 3 ? 3 : 1) : (prm.sttype == 1 ? 0 : 1);
+    // Lanczos sttype (lanczos.cpp:263): 0 absolute, 1 relative to ||r0||, 2 max-norm.
+    opt.ittype = symmetric ? (prm.ittype == 0 ? 0 : prm.ittype == 3 ? 3 : 1)
+                           : (prm.sttype == 1 ? 0 : (prm.sttype == 2 && opt.kind == bcm::kLanczos ? 2 : 1));
     opt.max_restarts = 0;
     bcm::SolveStats st;
     bcm::solve(mat, f, ctx, opt, db.p, dx.p, st);

@@ -30,6 +30,7 @@ extern "C" {
 #define BCM_BICGSTAB 1   /* no reference counterpart (nonsymmetric drop-in) */
 #define BCM_FGMRES 2     /* flexible GMRES(m); reference block GMRES is gmres.cpp:337 */
 #define BCM_LANCZOS 3    /* CSVector::Lanczos, lanczos.cpp:34-371 */
+#define BCM_CGPAR 4      /* CSVector::Cg parallel form with operator plug-ins, cg.cpp:355-771 */
 #define BCM_BLOCKPCG 5   /* batched PCG over many cells and right-hand sides (bcm_batch_solve) */
 
 #define BCM_F64 0
@@ -60,7 +61,7 @@ typedef struct bcm_params {
 } bcm_params;
 
 typedef struct bcm_solve_opts {
-  int kind;          /* BCM_PCG, BCM_BICGSTAB, BCM_FGMRES, BCM_LANCZOS */
+  int kind;          /* BCM_PCG, BCM_BICGSTAB, BCM_FGMRES, BCM_LANCZOS, BCM_CGPAR */
   int niter;
   double eps;
   int ittype;        /* as bcm_params.ittype; 3 = ||b - A x|| <= eps ||b|| */

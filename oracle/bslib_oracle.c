@@ -232,6 +232,65 @@ orc_result orc_cg(const orc_bsr* a, const orc_bsr* u, const double* x0, const do
   return res;
 }
 
+/* Parallel-form CG, CSVector::Cg(tree, CMVM, CSOLVELU) (cg.cpp:355-771) on
+ * one process: r = A x - b, u = M^-1 r; per iteration uAu = (Au, u), ru =
+ * (r, u), acoef = -ru / uAu, r += acoef Au, the stopping test on ||r||
+ * (sttype 0: ||r|| <= eps, 1: ||r|| <= eps ||r0||), then v = M^-1 r,
+ * bcoef = -(v, Au) / uAu, x += acoef u, u = v + bcoef u. The reference leaves
+ * the loop at the stopping test before the solution takes the last acoef u
+ * step (cg.cpp:607-614 against :660-661); apply_last = 1 applies that step at
+ * exit. nitperf = k - 1 at exit; A is upper-stored (MvmASymm), M = U^T U. */
+orc_result orc_cg_par(const orc_bsr* a, const orc_bsr* u, const double* x0, const double* b, int niter, double eps,
+                      int sttype, int apply_last, double* sol_out) {
+  const int n = a->sprnd[a->nsup];
+  const size_t vb = sizeof(double) * (size_t)n;
+  double *sol = malloc(vb), *r = malloc(vb), *uu = malloc(vb), *au = malloc(vb), *v = malloc(vb), *t = malloc(vb);
+  orc_result res = {0, 0, 0.0, 0};
+  memcpy(sol, x0, vb);
+  orc_mvmasymm(a, sol, r);
+  for (int i = 0; i < n; ++i) r[i] = r[i] - b[i];
+  double aux = 0.0;
+  for (int i = 0; i < n; ++i) aux += r[i] * r[i];
+  const double resi0 = sqrt(aux);
+  orc_solvel(u, r, t, 1);
+  orc_solveu(u, t, uu, 1);
+  for (int k = 1; k < niter; ++k) {
+    const int it = k - 1;
+    orc_mvmasymm(a, uu, au);
+    const double uau = orc_scprod(n, au, uu);
+    const double ru = orc_scprod(n, r, uu);
+    if (uau < 0.0) { res.breakdown = 1; break; }
+    const double acoef = (ru * -1.0) / uau;
+    for (int i = 0; i < n; ++i) r[i] = r[i] + acoef * au[i];
+    aux = 0.0;
+    for (int i = 0; i < n; ++i) aux += r[i] * r[i];
+    const double resi = sqrt(aux);
+    res.updates = k;
+    if ((sttype == 0 && resi <= eps) || (sttype == 1 && resi <= eps * resi0)) {
+      res.nitperf = it;
+      if (apply_last)
+        for (int i = 0; i < n; ++i) sol[i] = sol[i] + acoef * uu[i];
+      break;
+    }
+    orc_solvel(u, r, t, 1);
+    orc_solveu(u, t, v, 1);
+    const double vau = orc_scprod(n, v, au);
+    const double bcoef = (vau * -1.0) / uau;
+    for (int i = 0; i < n; ++i) {
+      sol[i] = sol[i] + acoef * uu[i];
+      uu[i] = v[i] + bcoef * uu[i];
+    }
+    res.nitperf = it;
+  }
+  orc_mvmasymm(a, sol, r);
+  aux = 0.0;
+  for (int i = 0; i < n; ++i) aux += (r[i] - b[i]) * (r[i] - b[i]);
+  res.resfin = sqrt(aux);
+  memcpy(sol_out, sol, vb);
+  free(sol); free(r); free(uu); free(au); free(v); free(t);
+  return res;
+}
+
 /* Two-sided Lanczos (BiCG-type), CSVector::Lanczos serial form
  * (lanczos.cpp:34-371): preconditioner L.SolveL then U.SolveU, transposed
  * preconditioner U.SolveL then L.SolveU, A and A^T products; stops on

@@ -59,6 +59,8 @@ def orc() -> C.CDLL:
         l.orc_scprod.restype = C.c_double
         l.orc_cg.argtypes = [C.POINTER(Bsr), C.POINTER(Bsr), _DP, _DP, C.c_int, C.c_double, C.c_int, _DP]
         l.orc_cg.restype = Result
+        l.orc_cg_par.argtypes = [C.POINTER(Bsr), C.POINTER(Bsr), _DP, _DP, C.c_int, C.c_double, C.c_int, C.c_int, _DP]
+        l.orc_cg_par.restype = Result
         l.orc_lanczos.argtypes = [C.POINTER(Bsr)] * 3 + [_DP, _DP, C.c_int, C.c_double, C.c_int, _DP]
         l.orc_lanczos.restype = Result
         l.orc_bicgstab.argtypes = [C.POINTER(Bsr)] * 3 + [_DP, _DP, C.c_int, C.c_double, _DP]
@@ -165,6 +167,12 @@ def solveu(f: Block, x, nrhs=1):
 def cg(a: Block, u: Block, b, x0=None, niter=1000, eps=1e-10, ittype=3):
     b = _d(b); x0 = np.zeros_like(b) if x0 is None else _d(x0); x = np.zeros_like(b)
     r = orc().orc_cg(C.byref(a.c), C.byref(u.c), _dp(x0), _dp(b), niter, eps, ittype, _dp(x))
+    return x, r
+
+
+def cg_par(a: Block, u: Block, b, x0=None, niter=1000, eps=1e-10, sttype=1, apply_last=True):
+    b = _d(b); x0 = np.zeros_like(b) if x0 is None else _d(x0); x = np.zeros_like(b)
+    r = orc().orc_cg_par(C.byref(a.c), C.byref(u.c), _dp(x0), _dp(b), niter, eps, sttype, int(apply_last), _dp(x))
     return x, r
 
 

@@ -123,18 +123,45 @@ def _callbacks(B, m, rhs, store):
             sig[8](fstore)]
 
 
-@pytest.mark.parametrize("symmetric", [True, False])
-def test_dropin_abstract_solvers(B, O, symmetric):
+@pytest.mark.parametrize("symmetric,ittype", [(True, 3), (False, 2), (False, 1)],
+                         ids=["symm-pcg", "bicgstab", "lanczos"])
+def test_dropin_abstract_solvers(B, O, symmetric, ittype):
     if symmetric:
         m, rhs = B.gen_heat2d(9, 7, 9, 31)
     else:
         m, rhs = B.gen_seepage3d(4, 4, 3, 6, 0.3, 32)
     x = np.zeros(m.n)
     cbs = _callbacks(B, m, rhs, x)
-    p = B.default_params(ittype=3 if symmetric else 2, sttype=1, eps=1e-10, niter=500)
+    p = B.default_params(ittype=ittype, sttype=1, eps=1e-10, niter=500)
     fn = B.lib().bcm_abstract_symm_solver if symmetric else B.lib().bcm_abstract_solver
     code = fn(None, C.byref(p), *cbs)
     assert code == 0, B.lib().bcm_last_error()
     xd = np.linalg.solve(m.dense(), rhs)
     assert rel(x, xd) < 1e-8
     assert p.nitperf > 0 and p.timefct > 0 and p.density >= 1.0
+
+
+@pytest.mark.parametrize("ittype,sttype,eps", [(0, 1, 1e-10), (2, 2, 1e-9), (1, 0, 1e-8)])
+def test_lanczos_matches_oracle(B, O, ctx, ittype, sttype, eps):
+    """Two-sided Lanczos (lanczos.cpp:34-371): same iterations as the oracle's
+    restatement (+-1: the dots are parallel sums) and the same good iterate."""
+    s = seepage(B, O, 5, 4, 4, 24, seed=31)
+    print(f"lanczos ittype={ittype} n={s['A'].n}", flush=True)
+    x, st = solve(B, ctx, s["mo"], s["f"], s["b"], kind=B.LANCZOS, eps=eps, ittype=ittype, check_every=2)
+    xo, r = O.lanczos(s["A"], s["L"], s["U"], s["b"], eps=eps, sttype=sttype)
+    assert st.converged == 1 and st.breakdown == 0
+    assert abs(st.iterations - r.nitperf) <= 1, (st.iterations, r.nitperf)
+    assert rel(x, xo) < 1e-6 * max(1.0, eps / 1e-10)
+
+
+@pytest.mark.parametrize("ittype,sttype", [(0, 1), (1, 0)])
+def test_cg_par_matches_oracle(B, O, ctx, ittype, sttype):
+    """Parallel-form CG (cg.cpp:355-771) against the oracle's restatement (with
+    the last solution step applied at a converged exit)."""
+    h = heat(B, O, 16, 14, 9, seed=43)
+    eps = 1e-10 if sttype == 1 else 1e-8
+    x, st = solve(B, ctx, h["mo"], h["f"], h["b"], kind=B.CGPAR, eps=eps, ittype=ittype, check_every=4)
+    xo, r = O.cg_par(h["A"], h["U"], h["b"], eps=eps, sttype=sttype, apply_last=True)
+    assert st.converged == 1 and st.breakdown == 0
+    assert abs(st.iterations - r.updates) <= 1, (st.iterations, r.updates)
+    assert rel(x, xo) < 1e-8 * max(1.0, eps / 1e-10)

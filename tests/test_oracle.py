@@ -191,3 +191,23 @@ def test_mlp_oracle_bf16_rounding():
     assert r[2] == 1.0 + 2 ** -6  # 1 + 3/256 rounds to 1 + 4/256
     assert abs(r[3] + 3.140625) < 1e-12
     assert np.all(np.abs(r - v) <= np.abs(v) * 2 ** -8)
+
+
+def test_cg_par_oracle_pins(B, O):
+    """Parallel-form CG (cg.cpp:355-771): with an exact factor it stops after
+    one step, and the reference's exit before the last solution step leaves
+    x0 untouched (apply_last=False) while the applied step gives the solution;
+    on a general system it follows the serial Cg restatement (same iterates in
+    exact arithmetic)."""
+    h = heat(B, O, 6, 5, 9, seed=41, tau1=0.0, tau2=0.0, theta=0.0)  # exact block Cholesky
+    x1, r1 = O.cg_par(h["A"], h["U"], h["b"], eps=1e-12, sttype=1, apply_last=True)
+    x0, r0 = O.cg_par(h["A"], h["U"], h["b"], eps=1e-12, sttype=1, apply_last=False)
+    xd = np.linalg.solve(h["mo"].dense(), h["b"])
+    assert r1.nitperf == 0 and r1.updates == 1
+    assert np.linalg.norm(x1 - xd) <= 1e-10 * np.linalg.norm(xd)
+    assert np.all(x0 == 0.0)
+    g = heat(B, O, 12, 10, 9, seed=42)
+    xp, rp = O.cg_par(g["A"], g["U"], g["b"], eps=1e-11, sttype=1, apply_last=True)
+    xs, rs = O.cg(g["A"], g["U"], g["b"], eps=1e-11, ittype=0)
+    assert abs(rp.updates - rs.updates) <= 1
+    assert np.linalg.norm(xp - xs) <= 1e-8 * np.linalg.norm(xs)
