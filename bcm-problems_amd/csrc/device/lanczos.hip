@@ -105,14 +105,14 @@ __global__ __launch_bounds__(256) void k_lz_check(KState* ks, const double* part
   sc[kLzErr] = errloc;
   sc[kLzRes2] = resi;
   const bool brk = fabs(sc[kLzDelDel]) < 1.e-100 && fabs(sc[kLzDelDel1]) < 1.e-100;
-  bool stop = k > niter || brk;
-  if (sttype == 2) stop = stop || sc[kLzErrMin] <= eps;
-  else if (sttype == 0) stop = stop || resi <= eps;
-  else stop = stop || resi <= eps * sc[kLzResIni];
-  if (k >= niter) stop = true;
-  if (stop) {
+  bool met;
+  if (sttype == 2) met = sc[kLzErrMin] <= eps;
+  else if (sttype == 0) met = resi <= eps;
+  else met = resi <= eps * sc[kLzResIni];
+  if (met || brk || k >= niter) {
     st[kLzDone] = 1;
     st[kLzBreak] = brk ? 1 : 0;
+    st[4] = met ? 1 : 0;
   }
 }
 
@@ -154,6 +154,10 @@ __global__ __launch_bounds__(256) void k_lz_init(KState* ks, const double* parti
   ks->sc[kLzRo] = 1.0;
 }
 
+__global__ void k_lz_clear(KState* ks) {
+  if (threadIdx.x < 8) ks->st[threadIdx.x] = 0;
+}
+
 int grid_n(int64_t n, int cap) { return (int)std::max<int64_t>(1, std::min<int64_t>((n + 255) / 256, cap)); }
 
 }  // namespace
@@ -182,7 +186,8 @@ void solve_lanczos(Matrix& A, Factor& F, Context& ctx, const SolveOptions& opt, 
   // sttype of the reference: 0 absolute, 1 relative to ||r0||, 2 max-norm; 3 (extension) relative to ||b||
   const int sttype = opt.ittype == 0 ? 1 : opt.ittype == 2 ? 2 : opt.ittype == 3 ? 3 : 0;
   double eps = opt.eps;
-  // r = b - A x, norms, goodsol = x; Delta = r; delta = M^-1 r
+  // r = b - A x, norms, goodsol = x; Delta = r; delta = M^-1 r (the flags of a previous solve are cleared first)
+  hipLaunchKernelGGL(k_lz_clear, dim3(1), dim3(64), 0, s, ks);
   launch_spmv(A.d, x, r, b, 1, nullptr, nullptr, ctx.rw, s);
   hipLaunchKernelGGL(k_lz_norms, dim3(nb), dim3(256), 0, s, n, ks, r, partials);
   hipLaunchKernelGGL(k_lz_init, dim3(1), dim3(256), 0, s, ks, partials, nb);
@@ -253,9 +258,7 @@ void solve_lanczos(Matrix& A, Factor& F, Context& ctx, const SolveOptions& opt, 
   const double rr = ctx.host_state->sc[22], bb = ctx.host_state->sc[23];
   st.relres_true = bb > 0 ? std::sqrt(rr / bb) : std::sqrt(rr);
   st.relres_rec = hs.sc[kLzRes2] / (bb > 0 ? std::sqrt(bb) : 1.0);
-  const double resfin = sttype == 2 ? hs.sc[kLzErrMin] : std::sqrt(rr);
-  const double target = sttype == 1 ? opt.eps * hs.sc[kLzResIni] : eps;
-  st.converged = resfin <= target * (1.0 + 1e-12) ? 1 : 0;
+  st.converged = hs.st[4];  // the reference's stopping test was met (the good iterate is returned)
 }
 
 }  // namespace bcm
