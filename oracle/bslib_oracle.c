@@ -15,7 +15,11 @@
  * svector.cpp, cg.cpp, lanczos.cpp ...) bit-for-bit, and against dense
  * closed forms; orc_bicgstab and orc_fgmres have no reference counterpart and
  * are pinned by closed forms (exact preconditioner -> 1 iteration, dense
- * solve agreement, monotone FGMRES residuals).
+ * solve agreement, monotone FGMRES residuals). orc_block_gmres restates
+ * gmres.cpp, whose build needs the Fortran dlarfg (lapackdm.f) and the MPI
+ * tree of CQrd, so it is not compiled from the reference; it is pinned by
+ * brute force: each step's iterate and residual norms equal the dense
+ * least-squares minimiser over the block Krylov space.
  */
 #include <math.h>
 #include <stdlib.h>
@@ -491,5 +495,215 @@ orc_result orc_fgmres(const orc_bsr* a, const orc_bsr* l, const orc_bsr* u, cons
   res.updates = total;
   memcpy(sol_out, x, vb);
   free(V); free(Z); free(H); free(cs); free(sn); free(g); free(y); free(x); free(r); free(tmp);
+  return res;
+}
+
+/* ---- block GMRES, CSVector::BlockGMRES (gmres.cpp:337-1038) ----
+ *
+ * Right-preconditioned block GMRES without restart over nrhs right-hand
+ * sides: the block Arnoldi basis is the Q factor of a Householder QR that is
+ * extended by nrhs columns per step (UpdateQrd, globals.cpp:531-590), the
+ * Krylov directions are Q applied to unit blocks (MvmQBlk,
+ * globals.cpp:778-840), the block Hessenberg columns are reduced by block
+ * Givens rotations, each the explicit Q of a 2nrhs x 2nrhs Householder QR
+ * (BlockGivens / ApplyBlockGivens, gmres.cpp:30-90), and the solution update
+ * is a back substitution (SolveTriangular, gmres.cpp:130-160). The
+ * reflectors follow LAPACK's dlarfg convention (beta = -sign(alpha) ||x||,
+ * v(1) = 1); the safe-minimum rescaling of dlarfg is left out. */
+
+/* dlarfg: H [alpha; x] = [beta; 0] with H = I - tau [1; v][1; v]^T; x is
+ * overwritten by v, alpha by beta (k = length including alpha). */
+static void orc_larfg(int k, double* alpha, double* x, int incx, double* tau) {
+  *tau = 0.0;
+  if (k <= 1) return;
+  double xn2 = 0.0;
+  for (int i = 0; i < k - 1; ++i) xn2 += x[(size_t)i * incx] * x[(size_t)i * incx];
+  if (xn2 == 0.0) return;
+  const double beta = -copysign(sqrt(*alpha * *alpha + xn2), *alpha);
+  *tau = (beta - *alpha) / beta;
+  const double sc = 1.0 / (*alpha - beta);
+  for (int i = 0; i < k - 1; ++i) x[(size_t)i * incx] *= sc;
+  *alpha = beta;
+}
+
+/* Householder QR of columns [ibeg, iend) of the m-row panel a (ld lda),
+ * columns < ibeg already reduced (UpdateQrd, globals.cpp:531-590). */
+static void orc_update_qrd(int m, int ibeg, int iend, double* a, int lda, double* tau) {
+  for (int i = ibeg; i < iend; ++i) {
+    double* ci = a + (size_t)i * lda;
+    for (int j = 0; j < i && j < m; ++j) {
+      const double* vj = a + (size_t)j * lda;
+      double s = ci[j];
+      for (int k = j + 1; k < m; ++k) s += ci[k] * vj[k];
+      s *= tau[j];
+      ci[j] -= s;
+      for (int k = j + 1; k < m; ++k) ci[k] -= s * vj[k];
+    }
+    if (i < m) orc_larfg(m - i, ci + i, ci + i + 1, 1, tau + i);
+    else tau[i] = 0.0;
+  }
+}
+
+/* qx = Q [x; 0] for the first nq reflectors of q (MvmQBlk, globals.cpp:778-840):
+ * x is nq x nrhs (ld ldx), qx is m x nrhs (ld ldqx). */
+static void orc_mvmq(int m, int nq, int nrhs, const double* q, int ldq, const double* tau, const double* x, int ldx,
+                     double* qx, int ldqx) {
+  for (int c = 0; c < nrhs; ++c) {
+    double* y = qx + (size_t)c * ldqx;
+    for (int i = 0; i < m; ++i) y[i] = i < nq ? x[(size_t)c * ldx + i] : 0.0;
+    for (int j = nq - 1; j >= 0; --j) {
+      const double* v = q + (size_t)j * ldq;
+      double s = y[j];
+      for (int k = j + 1; k < m; ++k) s += v[k] * y[k];
+      s = -s * tau[j];
+      y[j] += s;
+      for (int k = j + 1; k < m; ++k) y[k] += s * v[k];
+    }
+  }
+}
+
+/* rows [row0, row0 + 2 nb) of the nrhs columns of e (ld lde) <- G^T (those rows),
+ * G the explicit 2nb x 2nb rotation (ApplyBlockGivens, gmres.cpp:62-90). */
+static void orc_apply_giv(int nb, int row0, const double* G, int nrhs, double* e, int lde) {
+  const int nl = 2 * nb;
+  double tmp[64];
+  for (int c = 0; c < nrhs; ++c) {
+    double* col = e + (size_t)c * lde + row0;
+    for (int r = 0; r < nl; ++r) {
+      double s = 0.0;
+      for (int q = 0; q < nl; ++q) s += col[q] * G[(size_t)r * nl + q];
+      tmp[r] = s;
+    }
+    for (int r = 0; r < nl; ++r) col[r] = tmp[r];
+  }
+}
+
+static void orc_apply_a(const orc_bsr* a, int symm, int nrhs, const double* x, double* y) {
+  const int n = a->sprnd[a->nsup];
+  if (!symm) { orc_mvma(a, x, y, nrhs); return; }
+  for (int c = 0; c < nrhs; ++c) orc_mvmasymm(a, x + (size_t)c * n, y + (size_t)c * n);
+}
+
+/* sttype 0: resi_c < eps; otherwise resi_c / resi0_c < eps (gmres.cpp:742-750).
+ * x0, b, sol: n x nrhs column-major. hist (niter x nrhs, may be NULL): the
+ * recursion's residual norms after each step; fin (nrhs): ||b_c - A x_c||.
+ * Returns nitperf = number of block steps, updates = 1 if every column met
+ * the test. nrhs <= 32. */
+orc_result orc_block_gmres(const orc_bsr* a, const orc_bsr* l, const orc_bsr* u, int symm, int nrhs, const double* x0,
+                           const double* b, int niter, double eps, int sttype, double* sol_out, double* hist,
+                           double* fin) {
+  const int n = a->sprnd[a->nsup];
+  const int nb = nrhs, nb2 = nrhs * nrhs;
+  int nl = n;
+  if ((niter + 1) * nb > nl) nl = (niter + 1) * nb;
+  const int mmax = (niter + 1) * nb, ldb = (niter + 1) * nb, ldr = niter * nb;
+  double* p = calloc((size_t)nl * mmax, 8);
+  double* taup = calloc((size_t)mmax, 8);
+  double* giv = calloc((size_t)niter * nb2 * 4 + 1, 8);
+  double* bloc = calloc((size_t)ldb * nb, 8);
+  double* eloc = calloc((size_t)(mmax + nb) * nb, 8);
+  double* rm = calloc((size_t)ldr * ldr + 1, 8);
+  double *r = malloc(8 * (size_t)n * nb), *y = malloc(8 * (size_t)n * nb), *w = malloc(8 * (size_t)n * nb);
+  double *tmp = malloc(8 * (size_t)n * nb), *yp = malloc(8 * (size_t)nl * nb);
+  double resi0[32], resi[32];
+  orc_result res = {0, 0, 0.0, 0};
+  memcpy(sol_out, x0, 8 * (size_t)n * nb);
+  /* initial block residual and its QR */
+  orc_apply_a(a, symm, nb, sol_out, r);
+  for (size_t i = 0; i < (size_t)n * nb; ++i) r[i] = b[i] - r[i];
+  for (int c = 0; c < nb; ++c) memcpy(p + (size_t)c * nl, r + (size_t)c * n, 8 * (size_t)n);
+  orc_update_qrd(nl, 0, nb, p, nl, taup);
+  for (int c = 0; c < nb; ++c) {
+    double s = 0.0;
+    for (int i = 0; i <= c; ++i) {
+      bloc[(size_t)c * ldb + i] = p[(size_t)c * nl + i];
+      s += p[(size_t)c * nl + i] * p[(size_t)c * nl + i];
+    }
+    resi0[c] = resi[c] = sqrt(s);
+  }
+  int iter = 0, conv = 0;
+  for (int k = 0; k < niter; ++k) {
+    iter = k;
+    const int ncolp = (k + 1) * nb;
+    /* directions: Q e_(k nb + c), then M^-1 and A */
+    memset(eloc, 0, 8 * (size_t)ncolp * nb);
+    for (int c = 0; c < nb; ++c) eloc[(size_t)c * ncolp + k * nb + c] = 1.0;
+    orc_mvmq(nl, ncolp, nb, p, nl, taup, eloc, ncolp, yp, nl);
+    for (int c = 0; c < nb; ++c) memcpy(y + (size_t)c * n, yp + (size_t)c * nl, 8 * (size_t)n);
+    orc_solvel(l, y, tmp, nb);
+    orc_solveu(u, tmp, w, nb);
+    memcpy(y, w, 8 * (size_t)n * nb);
+    orc_apply_a(a, symm, nb, y, w);
+    /* extend the QR by the new block */
+    for (int c = 0; c < nb; ++c) {
+      double* pc = p + (size_t)(ncolp + c) * nl;
+      memcpy(pc, w + (size_t)c * n, 8 * (size_t)n);
+      memset(pc + n, 0, 8 * (size_t)(nl - n));
+    }
+    orc_update_qrd(nl, ncolp, ncolp + nb, p, nl, taup);
+    /* block column of the Hessenberg matrix: R part of the new columns */
+    const int iend = ncolp + nb;
+    memset(eloc, 0, 8 * (size_t)iend * nb);
+    for (int c = 0; c < nb; ++c)
+      for (int i = 0; i <= ncolp + c; ++i) eloc[(size_t)c * iend + i] = p[(size_t)(ncolp + c) * nl + i];
+    for (int g = 0; g < k; ++g) orc_apply_giv(nb, g * nb, giv + (size_t)g * nb2 * 4, nb, eloc, iend);
+    /* new block rotation: QR of the 2nb x nb block, explicit Q */
+    {
+      double blk[64 * 64], tb[64], ident[64 * 64];
+      const int n2 = 2 * nb;
+      memset(blk, 0, sizeof(double) * n2 * n2);
+      for (int c = 0; c < nb; ++c)
+        for (int i = 0; i < n2; ++i) blk[c * n2 + i] = eloc[(size_t)c * iend + k * nb + i];
+      orc_update_qrd(n2, 0, n2, blk, n2, tb);
+      memset(ident, 0, sizeof(double) * n2 * n2);
+      for (int i = 0; i < n2; ++i) ident[i * n2 + i] = 1.0;
+      orc_mvmq(n2, n2, n2, blk, n2, tb, ident, n2, giv + (size_t)k * nb2 * 4, n2);
+    }
+    orc_apply_giv(nb, k * nb, giv + (size_t)k * nb2 * 4, nb, eloc, iend);
+    for (int c = 0; c < nb; ++c)
+      for (int i = 0; i < ncolp; ++i) rm[(size_t)(k * nb + c) * ldr + i] = eloc[(size_t)c * iend + i];
+    orc_apply_giv(nb, k * nb, giv + (size_t)k * nb2 * 4, nb, bloc, ldb);
+    conv = 0;
+    for (int c = 0; c < nb; ++c) {
+      double s = 0.0;
+      for (int i = 0; i < nb; ++i) s += bloc[(size_t)c * ldb + (k + 1) * nb + i] * bloc[(size_t)c * ldb + (k + 1) * nb + i];
+      resi[c] = sqrt(s);
+      if (hist) hist[(size_t)k * nb + c] = resi[c];
+      if (sttype == 0 ? resi[c] < eps : resi[c] / resi0[c] < eps) ++conv;
+    }
+    if (conv == nb) break;
+  }
+  /* back substitution and the solution update */
+  const int ncolp = (iter + 1) * nb, lde = ncolp + nb;
+  memset(eloc, 0, 8 * (size_t)lde * nb);
+  for (int c = 0; c < nb; ++c) {
+    double* z = eloc + (size_t)c * lde;
+    for (int i = 0; i < ncolp; ++i) z[i] = bloc[(size_t)c * ldb + i];
+    for (int j = ncolp - 1; j >= 0; --j) {
+      const double zj = z[j] / rm[(size_t)j * ldr + j];
+      z[j] = zj;
+      for (int i = 0; i < j; ++i) z[i] -= rm[(size_t)j * ldr + i] * zj;
+    }
+  }
+  orc_mvmq(nl, lde, nb, p, nl, taup, eloc, lde, yp, nl);
+  for (int c = 0; c < nb; ++c) memcpy(y + (size_t)c * n, yp + (size_t)c * nl, 8 * (size_t)n);
+  orc_solvel(l, y, tmp, nb);
+  orc_solveu(u, tmp, w, nb);
+  for (size_t i = 0; i < (size_t)n * nb; ++i) sol_out[i] += w[i];
+  orc_apply_a(a, symm, nb, sol_out, r);
+  double rmax = 0.0;
+  for (int c = 0; c < nb; ++c) {
+    double s = 0.0;
+    for (int i = 0; i < n; ++i) {
+      const double d = b[(size_t)c * n + i] - r[(size_t)c * n + i];
+      s += d * d;
+    }
+    if (fin) fin[c] = sqrt(s);
+    if (sqrt(s) > rmax) rmax = sqrt(s);
+  }
+  res.nitperf = iter + 1;
+  res.updates = conv == nb;
+  res.resfin = rmax;
+  free(p); free(taup); free(giv); free(bloc); free(eloc); free(rm); free(r); free(y); free(w); free(tmp); free(yp);
   return res;
 }

@@ -66,6 +66,9 @@ def orc() -> C.CDLL:
         l.orc_bicgstab.argtypes = [C.POINTER(Bsr)] * 3 + [_DP, _DP, C.c_int, C.c_double, _DP]
         l.orc_bicgstab.restype = Result
         l.orc_fgmres.argtypes = [C.POINTER(Bsr)] * 3 + [_DP, _DP, C.c_int, C.c_int, C.c_double, C.c_int, _DP]
+        l.orc_block_gmres.restype = Result
+        l.orc_block_gmres.argtypes = [C.POINTER(Bsr)] * 3 + [C.c_int, C.c_int, _DP, _DP, C.c_int, C.c_double,
+                                                            C.c_int, _DP, _DP, _DP]
         l.orc_fgmres.restype = Result
         _orc = l
     return _orc
@@ -193,6 +196,22 @@ def fgmres(a: Block, l: Block, u: Block, b, x0=None, m=30, niter=1000, eps=1e-10
     r = orc().orc_fgmres(C.byref(a.c), C.byref(l.c), C.byref(u.c), _dp(x0), _dp(b), m, niter, eps, int(symm),
                          _dp(x))
     return x, r
+
+
+def block_gmres(a: Block, l: Block, u: Block, b, x0=None, niter=50, eps=1e-10, sttype=1, symm=False):
+    """Block GMRES (gmres.cpp:337-1038) on the n x nrhs block b; returns
+    (x, result, hist, fin): hist[k, c] the recursion's residual norm of column
+    c after step k (zero past the exit), fin[c] = ||b_c - A x_c||."""
+    b = np.asfortranarray(np.asarray(b, dtype=np.float64).reshape(a.n, -1, order="F"))
+    nrhs = b.shape[1]
+    assert 1 <= nrhs <= 32
+    x0 = np.zeros_like(b) if x0 is None else np.asfortranarray(np.asarray(x0, dtype=np.float64).reshape(b.shape, order="F"))
+    x = np.zeros_like(b)
+    hist = np.zeros((niter, nrhs))
+    fin = np.zeros(nrhs)
+    r = orc().orc_block_gmres(C.byref(a.c), C.byref(l.c), C.byref(u.c), int(symm), nrhs, _dp(x0), _dp(b), niter,
+                              eps, sttype, _dp(x), _dp(hist), _dp(fin))
+    return x, r, hist, fin
 
 
 # ---- the reference's compiled code ----

@@ -211,3 +211,67 @@ def test_cg_par_oracle_pins(B, O):
     xs, rs = O.cg(g["A"], g["U"], g["b"], eps=1e-11, ittype=0)
     assert abs(rp.updates - rs.updates) <= 1
     assert np.linalg.norm(xp - xs) <= 1e-8 * np.linalg.norm(xs)
+
+
+def _block_gmres_brute(Ad, Minv, R0, k):
+    """Minimal-residual correction over the block Krylov space
+    span{R0, (A M^-1) R0, ...} of k blocks, built by block Arnoldi with full
+    reorthogonalisation; returns (correction, per-column residual norms)."""
+    AM = Ad @ Minv
+    Q, _ = np.linalg.qr(R0)
+    basis = [Q]
+    for _ in range(k - 1):
+        W = AM @ basis[-1]
+        for _ in range(2):
+            for V in basis:
+                W = W - V @ (V.T @ W)
+        Q, _ = np.linalg.qr(W)
+        basis.append(Q)
+    Vk = np.hstack(basis)
+    Z, *_ = np.linalg.lstsq(AM @ Vk, R0, rcond=None)
+    return Minv @ (Vk @ Z), np.linalg.norm(R0 - AM @ Vk @ Z, axis=0)
+
+
+def test_block_gmres_oracle_pins(B, O):
+    """Block GMRES (gmres.cpp:337-1038), pinned by what the mathematics fixes:
+    after k steps each column is the minimal-residual correction over the
+    block Krylov space of k blocks (brute force by dense least squares), the
+    recursion's residual norms are those minimal residuals, the exit step is
+    the first one at which every column passes its test, and an exact factor
+    converges in one step."""
+    s = seepage(B, O, seed=23)
+    n = s["A"].n
+    Ad = s["mo"].dense()
+    I = np.eye(n)
+    Minv = O.solveu(s["U"], O.solvel(s["L"], I, n), n)
+    rng = np.random.default_rng(5)
+    Bm = rng.standard_normal((n, 3))
+    X0 = 0.1 * rng.standard_normal((n, 3))
+    R0 = Bm - Ad @ X0
+    for k in (1, 2, 4):
+        x, r, hist, fin = O.block_gmres(s["A"], s["L"], s["U"], Bm, X0, niter=k, eps=0.0)
+        corr, res = _block_gmres_brute(Ad, Minv, R0, k)
+        assert r.nitperf == k and r.updates == 0
+        assert np.linalg.norm(x - X0 - corr) <= 1e-10 * np.linalg.norm(corr), k
+        assert np.allclose(hist[k - 1], res, rtol=1e-8, atol=0), (k, hist[k - 1], res)
+        assert np.allclose(fin, np.linalg.norm(Bm - Ad @ x, axis=0), rtol=1e-12)
+        assert np.allclose(fin, res, rtol=1e-6)
+    # exit step: relative (sttype 1) and absolute (sttype 0) tests
+    x, r, hist, fin = O.block_gmres(s["A"], s["L"], s["U"], Bm, X0, niter=60, eps=1e-9, sttype=1)
+    r0n = np.linalg.norm(R0, axis=0)
+    first = next(k for k in range(60) if np.all(hist[k] / r0n < 1e-9))
+    assert r.updates == 1 and r.nitperf == first + 1
+    assert np.all(fin <= 1e-8 * r0n)
+    xa, ra, ha, fa = O.block_gmres(s["A"], s["L"], s["U"], Bm, X0, niter=60, eps=1e-7, sttype=0)
+    first = next(k for k in range(60) if np.all(ha[k] < 1e-7))
+    assert ra.updates == 1 and ra.nitperf == first + 1
+    # single column: the block method is plain right-preconditioned GMRES
+    x1, r1, h1, f1 = O.block_gmres(s["A"], s["L"], s["U"], Bm[:, :1], X0[:, :1], niter=3, eps=0.0)
+    c1, res1 = _block_gmres_brute(Ad, Minv, R0[:, :1], 3)
+    assert np.linalg.norm(x1 - X0[:, :1] - c1) <= 1e-10 * np.linalg.norm(c1)
+    # exact factor: one step, the direct solution
+    e = seepage(B, O, 4, 3, 3, 4, seed=5, tau1=0.0, tau2=0.0, theta=0.0)
+    Be = rng.standard_normal((e["A"].n, 4))
+    xe, re, he, fe = O.block_gmres(e["A"], e["L"], e["U"], Be, niter=10, eps=1e-10)
+    assert re.nitperf == 1 and re.updates == 1
+    assert np.linalg.norm(xe - np.linalg.solve(e["mo"].dense(), Be)) <= 1e-11 * np.linalg.norm(xe)
