@@ -26,7 +26,7 @@ LIB_PATH = os.path.join(os.path.dirname(_HERE), "lib", "libbcmhip.so")
 
 OK = 0
 E_ARG, E_NOT_SPD, E_WORKSPACE, E_BREAKDOWN, E_HIP, E_NOCONV, E_NODEVICE, E_INTERNAL = range(1, 9)
-PCG, BICGSTAB, FGMRES, LANCZOS, CGPAR, BLOCKPCG = 0, 1, 2, 3, 4, 5
+PCG, BICGSTAB, FGMRES, LANCZOS, CGPAR, BLOCKPCG, BLOCKGMRES = 0, 1, 2, 3, 4, 5, 6
 F64, F32, BF16 = 0, 1, 2
 
 
@@ -145,6 +145,8 @@ SIGNATURES = {
     "bcm_precond": (C.c_int, [_P, _P, C.c_void_p, C.c_void_p, C.c_int, C.c_int, C.c_int]),
     "bcm_dot": (C.c_int, [_P, C.c_int, C.c_void_p, C.c_void_p, _DP]),
     "bcm_solve": (C.c_int, [_P, _P, _P, C.POINTER(SolveOpts), C.c_void_p, C.c_void_p, C.POINTER(Stats)]),
+    "bcm_solve_block": (C.c_int, [_P, _P, _P, C.POINTER(SolveOpts), C.c_int, C.c_void_p, C.c_void_p,
+                                  C.POINTER(Stats), C.c_void_p, C.c_void_p]),
     "bcm_batch_create": (C.c_int, [C.c_int, C.POINTER(_P), C.POINTER(_P), C.c_int, C.POINTER(_P)]),
     "bcm_batch_destroy": (C.c_int, [_P]),
     "bcm_batch_layout": (C.c_int, [_P, _IP, C.POINTER(C.c_longlong), C.POINTER(C.c_longlong)]),
@@ -467,6 +469,22 @@ class Context:
         if code not in (OK, E_BREAKDOWN):
             _check(code)
         return st
+
+    def solve_block(self, a: Matrix, f: Factor, b, x, nrhs: int, opts: Optional[SolveOpts] = None):
+        """Block GMRES (gmres.cpp:337-1038) over nrhs columns of the device
+        arrays b, x (n x nrhs column-major, x holds x0). Returns (stats,
+        resi, hist): resi[c] = ||b_c - A x_c||, hist[k, c] the recursion's
+        residual norm of column c after step k."""
+        st = Stats()
+        o = opts or default_opts(kind=BLOCKGMRES)
+        niter = max(1, o.niter)
+        resi = np.zeros(nrhs)
+        hist = np.zeros((niter, nrhs))
+        code = lib().bcm_solve_block(self.h, a.h, f.h, C.byref(o), nrhs, ptr(b), ptr(x), C.byref(st),
+                                     resi.ctypes.data_as(C.c_void_p), hist.ctypes.data_as(C.c_void_p))
+        if code not in (OK, E_BREAKDOWN):
+            _check(code)
+        return st, resi, hist
 
 
 def device_count() -> int:

@@ -604,6 +604,24 @@ int bcm_solve(bcm_context c, bcm_matrix a, bcm_factor f, const bcm_solve_opts* o
   });
 }
 
+int bcm_solve_block(bcm_context c, bcm_matrix a, bcm_factor f, const bcm_solve_opts* o, int nrhs, const double* b,
+                    double* x, bcm_stats* st, double* resi, double* hist) {
+  return guarded([&] {
+    need(c && a && f && b && x, "solve_block: bad argument");
+    need(nrhs >= 1 && nrhs <= 8, "solve_block: 1..8 right-hand sides");
+    bcm::SolveOptions opt = to_opts(o);
+    opt.kind = bcm::kBlockGmres;
+    bcm::SolveStats s;
+    auto t0 = std::chrono::steady_clock::now();
+    bcm::solve_block_gmres(a->m, f->f, c->c, opt, nrhs, b, x, s, resi, hist);
+    BCM_HIP(hipStreamSynchronize(c->c.stream));
+    c->c.check_sweep_error();
+    s.seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    fill_stats(s, st);
+    if (s.breakdown) throw bcm::Error(bcm::kErrBreakdown, "Krylov breakdown");
+  });
+}
+
 int bcm_batch_create(int ncell, const bcm_matrix* mats, const bcm_factor* facts, int prec_mask, bcm_batch* out) {
   return guarded([&] {
     need(ncell > 0 && mats && facts && out, "batch_create: bad argument");

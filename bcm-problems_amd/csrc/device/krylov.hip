@@ -690,6 +690,7 @@ void solve(Matrix& A, Factor& F, Context& ctx, const SolveOptions& opt, const do
     case kCgPar: cg_par(A, F, ctx, opt, b, x, st); break;
     case kFgmres: solve_fgmres(A, F, ctx, opt, b, x, st); break;
     case kLanczos: solve_lanczos(A, F, ctx, opt, b, x, st); break;
+    case kBlockGmres: solve_block_gmres(A, F, ctx, opt, 1, b, x, st); break;
     default: throw Error(kErrArg, "solver kind not available");
   }
   BCM_HIP(hipStreamSynchronize(ctx.stream));

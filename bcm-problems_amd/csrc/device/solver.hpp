@@ -107,6 +107,7 @@ enum SolverKind : int {
   kLanczos = 3,      // reference two-sided Lanczos (lanczos.cpp:34-371)
   kCgPar = 4,        // reference parallel-form Cg (cg.cpp:355-771)
   kBlockPcg = 5,     // block PCG over several right-hand sides
+  kBlockGmres = 6,   // reference block GMRES (gmres.cpp:337-1038)
 };
 
 struct SolveOptions {
@@ -207,5 +208,12 @@ BatchTune batch_tune(Batch& bt, Context& ctx, const SolveOptions& opt, int nrhs,
 // Two-sided Lanczos driver (lanczos.hip).
 void solve_lanczos(Matrix& A, Factor& F, Context& ctx, const SolveOptions& opt, const double* b, double* x,
                    SolveStats& st);
+
+// Block GMRES driver (bgmres.hip) over nrhs (1..8) right-hand sides: b and x
+// are n x nrhs column-major, x holds x0; opt.niter is the number of block
+// steps (no restart). resi (host, nrhs) receives ||b_c - A x_c||; hist (host,
+// niter x nrhs, optional) the recursion's residual norms after each step.
+void solve_block_gmres(Matrix& A, Factor& F, Context& ctx, const SolveOptions& opt, int nrhs, const double* b,
+                       double* x, SolveStats& st, double* resi = nullptr, double* hist = nullptr);
 
 }  // namespace bcm

@@ -32,6 +32,7 @@ extern "C" {
 #define BCM_LANCZOS 3    /* CSVector::Lanczos, lanczos.cpp:34-371 */
 #define BCM_CGPAR 4      /* CSVector::Cg parallel form with operator plug-ins, cg.cpp:355-771 */
 #define BCM_BLOCKPCG 5   /* batched PCG over many cells and right-hand sides (bcm_batch_solve) */
+#define BCM_BLOCKGMRES 6 /* CSVector::BlockGMRES, gmres.cpp:337-1038 (bcm_solve_block; bcm_solve: 1 column) */
 
 #define BCM_F64 0
 #define BCM_F32 1
@@ -61,7 +62,7 @@ typedef struct bcm_params {
 } bcm_params;
 
 typedef struct bcm_solve_opts {
-  int kind;          /* BCM_PCG, BCM_BICGSTAB, BCM_FGMRES, BCM_LANCZOS, BCM_CGPAR */
+  int kind;          /* BCM_PCG, BCM_BICGSTAB, BCM_FGMRES, BCM_LANCZOS, BCM_CGPAR, BCM_BLOCKGMRES */
   int niter;
   double eps;
   int ittype;        /* as bcm_params.ittype; 3 = ||b - A x|| <= eps ||b|| */
@@ -166,6 +167,17 @@ int bcm_dot(bcm_context c, int n, const double* x, const double* y, double* resu
 /* ---- GPU solve: x (device) holds the initial guess on entry ---- */
 int bcm_solve(bcm_context c, bcm_matrix a, bcm_factor f, const bcm_solve_opts* o, const double* b, double* x,
               bcm_stats* st);
+/* Block GMRES over nrhs (1..8) right-hand sides, replacing
+ * CSVector::BlockGMRES (gmres.cpp:337-1038) with the operator pair of
+ * CParMvmA::MvmA / CParSolveLU::SolveLU (gsmatrix.h:758-817, 1008-1071):
+ * b and x (device) are n x nrhs column-major like CSVector's vect, x holds
+ * x0; o->niter is the Krylov block dimension (no restart); o->ittype 0 tests
+ * ||r_c|| < eps ||r0_c|| (sttype != 0), 1 tests ||r_c|| < eps (sttype 0),
+ * 3 tests ||r_c|| < eps ||b_c||. Columns with r0 = 0 are left as they are.
+ * resi (host, nrhs, may be NULL) receives ||b_c - A x_c||; hist (host,
+ * niter x nrhs, may be NULL) the recursion's residual norms after each step. */
+int bcm_solve_block(bcm_context c, bcm_matrix a, bcm_factor f, const bcm_solve_opts* o, int nrhs, const double* b,
+                    double* x, bcm_stats* st, double* resi, double* hist);
 
 /* ---- batched solves: many independent cells (block-diagonal union on the
  *      device), nrhs right-hand sides each; one launch sequence for every
