@@ -175,6 +175,8 @@ SIGNATURES = {
                                            _CB_BLOCK, _CB_BLOCK, _CB_BLOCK, _CB_BLOCK]),
     "bcm_abstract_solver": (C.c_int, [C.c_void_p, C.POINTER(Params), _CB_NSUP, _CB_SPRNDS, _CB_IAJA,
                                       _CB_BLOCK, _CB_BLOCK, _CB_BLOCK, _CB_BLOCK]),
+    "bcm_abstract_par_solver": (C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(Params), _CB_NSUP, _CB_SPRNDS,
+                                          _CB_IAJA, _CB_BLOCK, _CB_BLOCK, _CB_BLOCK, _CB_BLOCK]),
     "bcm_alloc_ints": (_IP, [C.c_int]),
 }
 

@@ -120,7 +120,32 @@ struct DevVec {
   }
 };
 
-int run(bool symmetric, bcm_params* p, const Callbacks& cb) {
+// Solver kind and stopping rule of the serial dispatchers (SolveSymm / Solver(RS),
+// solvers.cpp:4452-4610, 1568-1860) or, with par, of the parallel one (Solver(tree),
+// solvers.cpp:1865-2041: ittype 1 parallel Cg, 2 Lanczos, else BlockGMRES).
+void pick_solver(bool symmetric, bool par, const bcm_params& prm, bcm::SolveOptions& opt) {
+  if (par) {
+    if (prm.ittype == 1) {
+      opt.kind = bcm::kCgPar;  // cg.cpp:607-614: sttype 0 absolute, else relative to ||r0||
+      opt.ittype = prm.sttype == 0 ? 1 : 0;
+    } else if (prm.ittype == 2) {
+      opt.kind = bcm::kLanczos;  // lanczos.cpp:797: the parallel form's relative test is against ||b||
+      opt.ittype = prm.sttype == 0 ? 1 : prm.sttype == 2 ? 2 : 3;
+    } else {
+      opt.kind = bcm::kBlockGmres;  // gmres.cpp:742-750: sttype 0 absolute, else relative to ||r0||
+      opt.ittype = prm.sttype == 0 ? 1 : 0;
+    }
+    return;
+  }
+  opt.kind = symmetric ? bcm::kPcg : (prm.ittype == 1 ? bcm::kLanczos : bcm::kBicgstab);
+  // Cg's stopping rules (cg.cpp:238-241): 0 relative, 1 absolute, 2 max-norm
+  // (absolute, norm vector of ones); the GPU drivers test the 2-norm for 2 as well.
+  // Lanczos sttype (lanczos.cpp:263): 0 absolute, 1 relative to ||r0||, 2 max-norm.
+  opt.ittype = symmetric ? (prm.ittype == 0 ? 0 : prm.ittype == 3 ? 3 : 1)
+                         : (prm.sttype == 1 ? 0 : (prm.sttype == 2 && opt.kind == bcm::kLanczos ? 2 : 1));
+}
+
+int run(bool symmetric, bool par, bcm_params* p, const Callbacks& cb) {
   try {
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0)
@@ -179,14 +204,9 @@ int run(bool symmetric, bcm_params* p, const Callbacks& cb) {
     BCM_HIP(hipMemcpy(db.p, b.data(), n * 8, hipMemcpyHostToDevice));
     BCM_HIP(hipMemcpy(dx.p, x.data(), n * 8, hipMemcpyHostToDevice));
     bcm::SolveOptions opt;
-    opt.kind = symmetric ? bcm::kPcg : (prm.ittype == 1 ? bcm::kLanczos : bcm::kBicgstab);
     opt.niter = prm.niter;
     opt.eps = prm.eps;
-    // Cg's stopping rules (cg.cpp:238-241): 0 relative, 1 absolute, 2 max-norm
-    // (absolute, norm vector of ones); the GPU drivers test the 2-norm for 2 as well.
-    // Lanczos sttype (lanczos.cpp:263): 0 absolute, 1 relative to ||r0||, 2 max-norm.
-    opt.ittype = symmetric ? (prm.ittype == 0 ? 0 : prm.ittype == 3 ? 3 : 1)
-                           : (prm.sttype == 1 ? 0 : (prm.sttype == 2 && opt.kind == bcm::kLanczos ? 2 : 1));
+    pick_solver(symmetric, par, prm, opt);
     opt.max_restarts = 0;
     bcm::SolveStats st;
     bcm::solve(mat, f, ctx, opt, db.p, dx.p, st);
@@ -215,14 +235,22 @@ int bcm_abstract_symm_solver(void* pgener, bcm_params* p, bcm_cb_nsup fnsup, bcm
                              bcm_cb_iaja fiaja, bcm_cb_blockrow fblockrow, bcm_cb_rhs frhs,
                              bcm_cb_iniguess finiguess, bcm_cb_storesol fstoresol) {
   if (!fnsup || !fsprnds || !fiaja || !fblockrow || !frhs || !finiguess || !fstoresol) return BCM_E_ARG;
-  return run(true, p, Callbacks{pgener, fnsup, fsprnds, fiaja, fblockrow, frhs, finiguess, fstoresol});
+  return run(true, false, p, Callbacks{pgener, fnsup, fsprnds, fiaja, fblockrow, frhs, finiguess, fstoresol});
 }
 
 int bcm_abstract_solver(void* pgener, bcm_params* p, bcm_cb_nsup fnsup, bcm_cb_sprnds fsprnds, bcm_cb_iaja fiaja,
                         bcm_cb_blockrow fblockrow, bcm_cb_rhs frhs, bcm_cb_iniguess finiguess,
                         bcm_cb_storesol fstoresol) {
   if (!fnsup || !fsprnds || !fiaja || !fblockrow || !frhs || !finiguess || !fstoresol) return BCM_E_ARG;
-  return run(false, p, Callbacks{pgener, fnsup, fsprnds, fiaja, fblockrow, frhs, finiguess, fstoresol});
+  return run(false, false, p, Callbacks{pgener, fnsup, fsprnds, fiaja, fblockrow, frhs, finiguess, fstoresol});
+}
+
+int bcm_abstract_par_solver(void* pgener, void* comm, bcm_params* p, bcm_cb_nsup fnsup, bcm_cb_sprnds fsprnds,
+                            bcm_cb_iaja fiaja, bcm_cb_blockrow fblockrow, bcm_cb_rhs frhs,
+                            bcm_cb_iniguess finiguess, bcm_cb_storesol fstoresol) {
+  (void)comm;  // one rank: the whole system lives on this process's GPU
+  if (!fnsup || !fsprnds || !fiaja || !fblockrow || !frhs || !finiguess || !fstoresol) return BCM_E_ARG;
+  return run(false, true, p, Callbacks{pgener, fnsup, fsprnds, fiaja, fblockrow, frhs, finiguess, fstoresol});
 }
 
 }  // extern "C"

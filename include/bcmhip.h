@@ -276,6 +276,16 @@ int bcm_abstract_solver(void* pgener, bcm_params* p, bcm_cb_nsup fnsup, bcm_cb_s
                         bcm_cb_blockrow fblockrow, bcm_cb_rhs frhs, bcm_cb_iniguess finiguess,
                         bcm_cb_storesol fstoresol);
 
+/* AbstractParSolver (solvers.cpp:5799-6224) on one rank: assemble, RCM (the
+ * reference's nested-dissection ordering needs METIS), ILU2, then the kind of
+ * Solver(tree) (solvers.cpp:1865-2041): ittype 1 parallel-form Cg, 2
+ * two-sided Lanczos (relative test against ||b||, lanczos.cpp:797), otherwise
+ * block GMRES. comm (the reference's CMPIComm*) is not used: the system lives
+ * on this process's GPU. */
+int bcm_abstract_par_solver(void* pgener, void* comm, bcm_params* p, bcm_cb_nsup fnsup, bcm_cb_sprnds fsprnds,
+                            bcm_cb_iaja fiaja, bcm_cb_blockrow fblockrow, bcm_cb_rhs frhs,
+                            bcm_cb_iniguess finiguess, bcm_cb_storesol fstoresol);
+
 /* Callback memory for bcm_cb_iaja (operator new[] of nint ints, as new_struct in Bcmlib/utils.h:29-35). */
 int* bcm_alloc_ints(int nint);
 

@@ -165,3 +165,22 @@ def test_cg_par_matches_oracle(B, O, ctx, ittype, sttype):
     assert st.converged == 1 and st.breakdown == 0
     assert abs(st.iterations - r.updates) <= 1, (st.iterations, r.updates)
     assert rel(x, xo) < 1e-8 * max(1.0, eps / 1e-10)
+
+
+@pytest.mark.parametrize("ittype,skew", [(1, 0.0), (2, 0.3), (0, 0.3)], ids=["cg-par", "lanczos", "block-gmres"])
+def test_dropin_abstract_par_solver(B, O, ittype, skew):
+    """AbstractParSolver on one rank: the kind of Solver(tree)
+    (solvers.cpp:1865-2041) by ittype (1 parallel Cg on a symmetric system in
+    full storage, 2 Lanczos, otherwise block GMRES); the stored solution
+    solves the system."""
+    m, rhs = B.gen_seepage3d(4, 4, 3, 6, skew, 33)
+    x = np.zeros(m.n)
+    cbs = _callbacks(B, m, rhs, x)
+    p = B.default_params(ittype=ittype, sttype=1, eps=1e-11, niter=400)
+    code = B.lib().bcm_abstract_par_solver(None, None, C.byref(p), *cbs)
+    assert code == 0, B.lib().bcm_last_error()
+    Ad = m.dense()
+    xd = np.linalg.solve(Ad, rhs)
+    assert rel(x, xd) < 1e-8, ittype
+    assert np.linalg.norm(rhs - Ad @ x) <= 1e-9 * np.linalg.norm(rhs)
+    assert p.nitperf > 0 and p.timefct > 0
