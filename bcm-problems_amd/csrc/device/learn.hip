@@ -87,13 +87,38 @@ __global__ __launch_bounds__(256) void k_gemm_bf16(int M, int N, int K, const TA
   }
 }
 
-// db[j] = sum_i dz[i*ld + j] (rows in order).
-__global__ void k_colsum(int M, int N, const float* __restrict__ dz, int ld, float* __restrict__ db) {
+// Column sums db[j] = sum_i dz[i*ld + j] in two deterministic passes: block
+// (tile, chunk) adds rows [chunk*rows_per, ...) of 64 columns (4 row phases
+// per column, combined in a fixed order) into part[chunk*N + j]; k_colsum_fin
+// adds the chunks in order.
+constexpr int kColChunks = 32;
+__global__ __launch_bounds__(256) void k_colsum(int M, int N, const float* __restrict__ dz, int ld,
+                                                float* __restrict__ part) {
+  __shared__ float sm[4][64];
+  const int c = threadIdx.x & 63, ph = threadIdx.x >> 6;
+  const int j = blockIdx.x * 64 + c;
+  const int per = (M + kColChunks - 1) / kColChunks;
+  const int i0 = blockIdx.y * per, i1 = min(M, i0 + per);
+  float s = 0.f;
+  if (j < N)
+    for (int i = i0 + ph; i < i1; i += 4) s += dz[(size_t)i * ld + j];
+  sm[ph][c] = s;
+  __syncthreads();
+  if (ph == 0 && j < N) part[(size_t)blockIdx.y * N + j] = ((sm[0][c] + sm[1][c]) + sm[2][c]) + sm[3][c];
+}
+
+__global__ void k_colsum_fin(int N, const float* __restrict__ part, float* __restrict__ db) {
   const int j = blockIdx.x * blockDim.x + threadIdx.x;
   if (j >= N) return;
   float s = 0.f;
-  for (int i = 0; i < M; ++i) s += dz[(size_t)i * ld + j];
+  for (int k = 0; k < kColChunks; ++k) s += part[(size_t)k * N + j];
   db[j] = s;
+}
+
+void colsum(hipStream_t s, int M, int N, const float* dz, int ld, float* db, float* part) {
+  hipLaunchKernelGGL(k_colsum, dim3((N + 63) / 64, kColChunks), dim3(256), 0, s, M, N, dz, ld, part);
+  hipLaunchKernelGGL(k_colsum_fin, dim3((N + 255) / 256), dim3(256), 0, s, N, part, db);
+  BCM_HIP(hipGetLastError());
 }
 
 // dy = 2 (y - t) / (M N); per-block partials of sum (y - t)^2.
@@ -296,6 +321,7 @@ void Mlp::ensure_batch(int M) {
   dz2.alloc((size_t)M * nh * 4);
   dy.alloc((size_t)M * nout * 4);
   partials.alloc(4096 * 8 + 64);
+  colpart.alloc((size_t)kColChunks * std::max(nh, nout) * 4);
   cap_batch = M;
 }
 
@@ -324,22 +350,19 @@ void Mlp::backward(hipStream_t s, int M, const float* x, const float* t, double*
   // layer 3: dW3 = H2^T dY, db3 = colsum(dY), dZ2 = (dY W3^T) * (1 - H2^2)
   gemm(s, nh, nout, M, h2.as<const float>(), 1, nh, dy.as<const float>(), nout, 1, G + w_off[2], nout,
        (const float*)nullptr, 0, (const float*)nullptr, 0);
-  hipLaunchKernelGGL(k_colsum, dim3((nout + 255) / 256), dim3(256), 0, s, M, nout, dy.as<const float>(), nout,
-                     G + b_off[2]);
+  colsum(s, M, nout, dy.as<const float>(), nout, G + b_off[2], colpart.as<float>());
   gemm(s, M, nh, nout, dy.as<const float>(), nout, 1, W + wb_off[2], 1, nout, dz2.as<float>(), nh,
        (const float*)nullptr, 0, h2.as<const float>(), nh);
   // layer 2
   gemm(s, nh, nh, M, h1.as<const float>(), 1, nh, dz2.as<const float>(), nh, 1, G + w_off[1], nh,
        (const float*)nullptr, 0, (const float*)nullptr, 0);
-  hipLaunchKernelGGL(k_colsum, dim3((nh + 255) / 256), dim3(256), 0, s, M, nh, dz2.as<const float>(), nh,
-                     G + b_off[1]);
+  colsum(s, M, nh, dz2.as<const float>(), nh, G + b_off[1], colpart.as<float>());
   gemm(s, M, nh, nh, dz2.as<const float>(), nh, 1, W + wb_off[1], 1, nh, dz1.as<float>(), nh, (const float*)nullptr,
        0, h1.as<const float>(), nh);
   // layer 1
   gemm(s, nin, nh, M, x, 1, nin, dz1.as<const float>(), nh, 1, G + w_off[0], nh, (const float*)nullptr, 0,
        (const float*)nullptr, 0);
-  hipLaunchKernelGGL(k_colsum, dim3((nh + 255) / 256), dim3(256), 0, s, M, nh, dz1.as<const float>(), nh,
-                     G + b_off[0]);
+  colsum(s, M, nh, dz1.as<const float>(), nh, G + b_off[0], colpart.as<float>());
 }
 
 void Mlp::adam(hipStream_t s, float lr) {

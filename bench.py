@@ -108,6 +108,8 @@ class CellFactory:
         self.futs = [None] * len(params)
         self.submitted = 0
         self.lock = threading.Lock()
+        self.cv = threading.Condition()
+        self.running, self.active = True, 0
         self._fill(0)
 
     def _fill(self, i):
@@ -117,11 +119,31 @@ class CellFactory:
                 self.submitted += 1
 
     def _build(self, i):
-        r, lc, nu, sh = self.params[i]
-        mat, rhs = self.B.gen_elastic_cell(CELL_M, r, 10 ** lc, nu, sh, TEMPLATE_SEED)
-        mo = mat.reorder(self.order)
-        f = self.B.Factor.ic2(mo, self.fparams)
-        return mo, f, mo.reorder_vector(self.order, rhs)
+        with self.cv:
+            self.cv.wait_for(lambda: self.running)
+            self.active += 1
+        try:
+            r, lc, nu, sh = self.params[i]
+            mat, rhs = self.B.gen_elastic_cell(CELL_M, r, 10 ** lc, nu, sh, TEMPLATE_SEED)
+            mo = mat.reorder(self.order)
+            f = self.B.Factor.ic2(mo, self.fparams)
+            return mo, f, mo.reorder_vector(self.order, rhs)
+        finally:
+            with self.cv:
+                self.active -= 1
+                self.cv.notify_all()
+
+    def quiet(self):
+        """Holds new cells back and waits for the ones in progress: the host
+        cores are idle while a timed solve runs."""
+        with self.cv:
+            self.running = False
+            self.cv.wait_for(lambda: self.active == 0)
+
+    def resume(self):
+        with self.cv:
+            self.running = True
+            self.cv.notify_all()
 
     def get(self, i):
         self._fill(i)
@@ -131,6 +153,7 @@ class CellFactory:
         self.futs[i] = None
 
     def close(self):
+        self.resume()
         self.pool.shutdown(wait=False, cancel_futures=True)
 
 
@@ -159,6 +182,10 @@ def run_learned(args, B):
     opts = B.default_opts(kind=B.BLOCKPCG, eps=EPS, precision=B.F32, niter=5000)
     guess = B.Guess(n, 6, 4, nmodes=64, nhidden=256, capacity=ntrain, seed=11)
     tune = None
+    if args.tune:
+        keys = ("sweep_mode", "waves_per_cu", "level_order", "graph", "check_every")
+        tune = dict(zip(keys, (int(v) for v in args.tune.split(","))))
+        assert len(tune) == len(keys), "--tune takes five comma-separated integers"
     train_pairs, train_s, train_iters = 0, 0.0, []
     # prefetch the next group's device images while the current one solves
     next_group = [None]
@@ -217,11 +244,9 @@ def run_learned(args, B):
     plain_cells = []
     for gi, idx in enumerate(evalg):
         bt, db, cells = take()
-        if gi + 1 < len(evalg):
-            th = threading.Thread(target=prefetch, args=(evalg[gi + 1], 0b010))
-            th.start()
         bt.set_tune(**tune)
         dx = B.DeviceArray(db.shape)
+        fac.quiet()  # nothing else runs on the host or the device during the timed solve
         ctx.sync()
         t0 = time.perf_counter()
         guess.predict(ctx, params[idx], dx)
@@ -251,6 +276,10 @@ def run_learned(args, B):
             plain_cells = cells[:args.plain_cells]
             log(f"group {gi}: zero guess {itz.mean():.1f} iterations ({stz.seconds * 1e3:.0f} ms), learned "
                 f"{its.mean():.1f} iterations ({st.seconds * 1e3:.0f} ms)")
+        fac.resume()
+        if gi + 1 < len(evalg):
+            th = threading.Thread(target=prefetch, args=(evalg[gi + 1], 0b010))
+            th.start()
         log(f"eval group {gi + 1}/{len(evalg)}{'' if timed else ' (warm-up)'}: {dt * 1e3:.0f} ms, mean iterations "
             f"{its.mean():.1f}, max relres {rr.max():.2e}")
         for i in idx[args.plain_cells:] if gi == args.warmup else idx:
@@ -284,7 +313,7 @@ def run_learned(args, B):
     value = timed_pairs / timed_s
     cpu = None
     if not args.no_cpu:
-        cpu = cpu_baseline_ref(B, plain_cells[0], np.mean(plain_its))
+        cpu = cpu_baseline_ref(B, plain_cells, np.mean(plain_its))
     line = {
         "metric": METRIC,
         "value": value,
@@ -319,6 +348,7 @@ def run_learned(args, B):
         "guess_fit": {"pod_energy": rep.pod_energy, "loss_first": rep.loss_first, "loss_final": rep.loss_final,
                       "seconds_pod": rep.seconds_pod, "seconds_mlp": rep.seconds_train},
         "schedule": tune,
+        "schedule_from": "--tune" if args.tune else "autotuner",
         "kernels": kernels,
         "roofline": roof,
         "cpu_baseline": cpu,
@@ -327,26 +357,36 @@ def run_learned(args, B):
     print(json.dumps(line), flush=True)
 
 
-def cpu_baseline_ref(B, cell, iters_plain):
-    """The reference's serial Cg (Bslib sources compiled into oracle/_ref) on one host core."""
+def cpu_baseline_ref(B, cells, iters_plain, budget_s=10.0):
+    """The reference's serial Cg (Bslib sources compiled into oracle/_ref) on
+    one host core, over the load cases of the sample cells until about
+    budget_s of CPU work has been done."""
     import oracle as O
 
-    mo, f, b = cell
-    A = O.Block(*mo.arrays())
-    U = O.Block(A.sprnd, *f.arrays(1, A.nsup))
-    if O.ref_available():
-        t0 = time.perf_counter()
-        _, nit, res = O.ref_cg(A, U, b[:, 0], eps=EPS, ittype=0, niter=5000)
-        dt = time.perf_counter() - t0
-        return dict(value=1.0 / dt, unit="solves/s", cores=1, kind="reference",
-                    sample=f"one load case of one evaluation cell by the reference's serial Cg (compiled from its "
-                           f"sources), {nit} iterations to rel-res {EPS} in {dt:.1f}s on one core of {cpu_model()}")
-    t0 = time.perf_counter()
-    _, r = O.cg(A, U, b[:, 0], eps=EPS, ittype=3, niter=5000)
-    dt = time.perf_counter() - t0
-    return dict(value=1.0 / dt, unit="solves/s", cores=1, kind="port",
-                sample=f"one load case of one evaluation cell by the oracle's Cg, {r.updates} iterations in "
-                       f"{dt:.1f}s on one core of {cpu_model()}")
+    use_ref = O.ref_available()
+    nsolve, nit_sum, t_sum = 0, 0, 0.0
+    for mo, f, b in cells:
+        A = O.Block(*mo.arrays())
+        U = O.Block(A.sprnd, *f.arrays(1, A.nsup))
+        for q in range(b.shape[1]):
+            t0 = time.perf_counter()
+            if use_ref:
+                _, nit, _ = O.ref_cg(A, U, b[:, q], eps=EPS, ittype=0, niter=5000)
+            else:
+                _, r = O.cg(A, U, b[:, q], eps=EPS, ittype=3, niter=5000)
+                nit = r.updates
+            t_sum += time.perf_counter() - t0
+            nsolve += 1
+            nit_sum += nit
+            if t_sum >= budget_s:
+                break
+        if t_sum >= budget_s:
+            break
+    who = "the reference's serial Cg (compiled from its sources)" if use_ref else "the oracle's Cg"
+    return dict(value=nsolve / t_sum, unit="solves/s", cores=1, kind="reference" if use_ref else "port",
+                sample=f"{nsolve} load cases of the first evaluation cells, one at a time, by {who}: "
+                       f"{nit_sum / nsolve:.1f} iterations each to rel-res {EPS}, {t_sum:.1f}s on one core "
+                       f"of {cpu_model()}")
 
 
 # ---------------------------------------------------------------------------
@@ -421,6 +461,9 @@ def main():
     ap.add_argument("--plain-cells", type=int, default=4)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-graph", action="store_true", help="leave hipGraph replay out of the batch schedules")
+    ap.add_argument("--tune", default=None,
+                    help="fixed batch schedule 'sweep_mode,waves_per_cu,level_order,graph,check_every' instead of "
+                         "the autotuner (e.g. to profile the schedule an unprofiled run chose)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; multi-GPU runs are not part of it",
