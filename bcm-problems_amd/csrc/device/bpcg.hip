@@ -78,6 +78,32 @@ __global__ __launch_bounds__(256) void k_bp_dots(int64_t n, const int* __restric
   block_partials<NR * K>(acc, partials);
 }
 
+// The flexible recurrence's three dots (r.z, r.r, r_old.z) with z kept in
+// FP32 (the preconditioner's output precision; the products are formed in
+// FP64, the same values and order as with an FP64 copy of z).
+template <int NR>
+__global__ __launch_bounds__(256) void k_bp_dots_flex(int64_t n, const int* __restrict__ lo,
+                                                      const int* __restrict__ hi, const double* __restrict__ r,
+                                                      const double* __restrict__ rold, const float* __restrict__ z,
+                                                      double* partials, const int* skip) {
+  if (skip && *skip) return;
+  double acc[NR * 3];
+#pragma unroll
+  for (int j = 0; j < NR * 3; ++j) acc[j] = 0.0;
+  const int a = lo[blockIdx.x], e = hi[blockIdx.x];
+#pragma unroll
+  for (int q = 0; q < NR; ++q) {
+    const size_t off = (size_t)q * n;
+    for (int i = a + threadIdx.x; i < e; i += blockDim.x) {
+      const double ri = r[off + i], zi = (double)z[off + i];
+      acc[q * 3 + 0] += ri * zi;
+      acc[q * 3 + 1] += ri * ri;
+      acc[q * 3 + 2] += rold[off + i] * zi;
+    }
+  }
+  block_partials<NR * 3>(acc, partials);
+}
+
 // Start of a solve: partials hold (r.r, b.b) per pair. Optional fallback to
 // x0 = 0 for pairs whose initial residual is not below ||b||.
 __global__ __launch_bounds__(kPairThreads) void k_bp_start(int npair, int nrhs, const int* cell_chunk0,
@@ -169,9 +195,9 @@ __global__ __launch_bounds__(kPairThreads) void k_bp_beta(int npair, int nrhs, c
 }
 
 // p = z + beta p for active pairs.
-template <int NR>
+template <int NR, class TZ>
 __global__ __launch_bounds__(256) void k_bp_p(int64_t n, const int* __restrict__ ccell, const int* __restrict__ lo,
-                                              const int* __restrict__ hi, BPtrs s, const double* __restrict__ z,
+                                              const int* __restrict__ hi, BPtrs s, const TZ* __restrict__ z,
                                               double* __restrict__ pv) {
   if (s.state[0]) return;
   const int c = ccell[blockIdx.x], a = lo[blockIdx.x], e = hi[blockIdx.x];
@@ -181,7 +207,7 @@ __global__ __launch_bounds__(256) void k_bp_p(int64_t n, const int* __restrict__
     if (!s.active[pr]) continue;
     const double beta = s.beta[pr];
     const size_t off = (size_t)q * n;
-    for (int i = a + threadIdx.x; i < e; i += blockDim.x) pv[off + i] = pv[off + i] * beta + 1.0 * z[off + i];
+    for (int i = a + threadIdx.x; i < e; i += blockDim.x) pv[off + i] = pv[off + i] * beta + 1.0 * (double)z[off + i];
   }
 }
 
@@ -206,12 +232,14 @@ __global__ __launch_bounds__(kPairThreads) void k_bp_alpha(int npair, int nrhs, 
   }
 }
 
-// x += alpha p; (r_old = r;) r -= alpha q for active pairs.
+// x += alpha p; (r_old = r;) r -= alpha q for active pairs; with FLEX the
+// FP32 copy of r that the preconditioner reads is written as well.
 template <int NR, bool FLEX>
 __global__ __launch_bounds__(256) void k_bp_xr(int64_t n, const int* __restrict__ ccell, const int* __restrict__ lo,
                                                const int* __restrict__ hi, BPtrs s, const double* __restrict__ pv,
                                                const double* __restrict__ qv, double* __restrict__ x,
-                                               double* __restrict__ r, double* __restrict__ rold) {
+                                               double* __restrict__ r, double* __restrict__ rold,
+                                               float* __restrict__ r32) {
   if (s.state[0]) return;
   const int c = ccell[blockIdx.x], a = lo[blockIdx.x], e = hi[blockIdx.x];
 #pragma unroll
@@ -223,7 +251,9 @@ __global__ __launch_bounds__(256) void k_bp_xr(int64_t n, const int* __restrict_
     for (int i = a + threadIdx.x; i < e; i += blockDim.x) {
       const double ri = r[off + i];
       if (FLEX) rold[off + i] = ri;
-      r[off + i] = ri + (-alpha) * qv[off + i];
+      const double rn = ri + (-alpha) * qv[off + i];
+      r[off + i] = rn;
+      if (FLEX) r32[off + i] = (float)rn;
       x[off + i] = x[off + i] + alpha * pv[off + i];
     }
   }
@@ -270,14 +300,20 @@ int grid_vec(int64_t n) { return (int)std::max<int64_t>(1, std::min<int64_t>((n 
 // Work vectors of a batched solve (all n x nrhs, fp64) and the pair state.
 struct BWork {
   double *x, *r, *z, *p, *q, *rold, *b, *partials;
+  float *r32 = nullptr, *t32 = nullptr, *z32 = nullptr;  // FP32 vectors of the low-precision preconditioner
   BPtrs s;
   int* state_host;
 };
 
-BWork work(Batch& bt, Context& ctx, int nrhs) {
+BWork work(Batch& bt, Context& ctx, int nrhs, bool flex) {
   const size_t vb = (size_t)bt.n * nrhs * 8;
   const int npair = bt.ncell * nrhs;
   BWork w;
+  if (flex) {
+    w.r32 = reinterpret_cast<float*>(ctx.vec(82, vb / 2));
+    w.t32 = reinterpret_cast<float*>(ctx.vec(83, vb / 2));
+    w.z32 = reinterpret_cast<float*>(ctx.vec(84, vb / 2));
+  }
   w.x = ctx.vec(40, vb);
   w.r = ctx.vec(41, vb);
   w.z = ctx.vec(42, vb);
@@ -331,20 +367,34 @@ void iteration(Batch& bt, Context& ctx, const BWork& w, const SolveOptions& opt,
   const int* done = &w.s.state[0];
   const int npair = bt.ncell * NR;
   const double eps2 = opt.eps * opt.eps;
-  precond_sweeps(t1, t2, ctx, w.r, w.z, opt.prec, NR, done);
   if (flex) {
-    const double* xs[3] = {w.r, w.r, w.rold};
-    const double* ys[3] = {w.z, w.r, w.z};
-    dots<NR, 3>(bt, ctx, w, xs, ys, done);
+    // FP32 sweeps straight from the FP32 copy of r (written with r) into z32
+    {
+      ProfScope ps(ctx, kProfSweep, sweep_bytes(t1, NR));
+      launch_sweep(t1, w.r32, w.t32, kF32, NR, s, done, ctx.sweep_sync());
+    }
+    {
+      ProfScope ps(ctx, kProfSweep, sweep_bytes(t2, NR));
+      launch_sweep(t2, w.t32, w.z32, kF32, NR, s, done, ctx.sweep_sync());
+    }
+    hipLaunchKernelGGL(k_bp_dots_flex<NR>, dim3(bt.nchunks), dim3(256), 0, s, (int64_t)bt.n, bt.chunk_lo.as<int>(),
+                       bt.chunk_hi.as<int>(), w.r, w.rold, w.z32, w.partials, done);
   } else {
+    precond_sweeps(t1, t2, ctx, w.r, w.z, opt.prec, NR, done);
     const double* xs[2] = {w.r, w.r};
     const double* ys[2] = {w.z, w.r};
     dots<NR, 2>(bt, ctx, w, xs, ys, done);
   }
   hipLaunchKernelGGL(k_bp_beta, dim3(1), dim3(kPairThreads), 0, s, npair, NR, bt.dcell_chunk0.as<int>(), w.partials,
                      flex ? 3 : 2, w.s, eps2, opt.ittype, opt.niter);
-  hipLaunchKernelGGL(k_bp_p<NR>, dim3(bt.nchunks), dim3(256), 0, s, (int64_t)bt.n, bt.chunk_cell.as<int>(),
-                     bt.chunk_lo.as<int>(), bt.chunk_hi.as<int>(), w.s, w.z, w.p);
+  if (flex)
+    hipLaunchKernelGGL((k_bp_p<NR, float>), dim3(bt.nchunks), dim3(256), 0, s, (int64_t)bt.n,
+                       bt.chunk_cell.as<int>(), bt.chunk_lo.as<int>(), bt.chunk_hi.as<int>(), w.s,
+                       (const float*)w.z32, w.p);
+  else
+    hipLaunchKernelGGL((k_bp_p<NR, double>), dim3(bt.nchunks), dim3(256), 0, s, (int64_t)bt.n,
+                       bt.chunk_cell.as<int>(), bt.chunk_lo.as<int>(), bt.chunk_hi.as<int>(), w.s,
+                       (const double*)w.z, w.p);
   {
     ProfScope ps(ctx, kProfSpmv, spmv_bytes(bt.A, NR));
     launch_spmv(bt.A, w.p, w.q, nullptr, NR, nullptr, nullptr, ctx.rw, s, done);
@@ -358,11 +408,11 @@ void iteration(Batch& bt, Context& ctx, const BWork& w, const SolveOptions& opt,
                      w.s);
   if (flex)
     hipLaunchKernelGGL((k_bp_xr<NR, true>), dim3(bt.nchunks), dim3(256), 0, s, (int64_t)bt.n, bt.chunk_cell.as<int>(),
-                       bt.chunk_lo.as<int>(), bt.chunk_hi.as<int>(), w.s, w.p, w.q, w.x, w.r, w.rold);
+                       bt.chunk_lo.as<int>(), bt.chunk_hi.as<int>(), w.s, w.p, w.q, w.x, w.r, w.rold, w.r32);
   else
     hipLaunchKernelGGL((k_bp_xr<NR, false>), dim3(bt.nchunks), dim3(256), 0, s, (int64_t)bt.n,
                        bt.chunk_cell.as<int>(), bt.chunk_lo.as<int>(), bt.chunk_hi.as<int>(), w.s, w.p, w.q, w.x,
-                       w.r, w.rold);
+                       w.r, w.rold, (float*)nullptr);
 }
 
 }  // namespace
@@ -449,10 +499,10 @@ void solve_nr(Batch& bt, Context& ctx, const SolveOptions& opt, const double* b,
   const int npair = bt.ncell * NR;
   if (npair > 1 << 20) throw Error(kErrArg, "batch: too many (cell, right-hand side) pairs");
   hipStream_t s = ctx.stream;
-  BWork w = work(bt, ctx, NR);
+  const bool flex = opt.prec != kF64;
+  BWork w = work(bt, ctx, NR, flex);
   const int64_t nn = bt.n * NR;
   const double eps2 = opt.eps * opt.eps;
-  const bool flex = opt.prec != kF64;
   const BatchTune& tu = bt.tune;
   const int saved_mode = ctx.sweep_mode;
   const int saved_waves = ctx.sync.waves_per_cu;
@@ -478,6 +528,7 @@ void solve_nr(Batch& bt, Context& ctx, const SolveOptions& opt, const double* b,
     if (fallback)
       hipLaunchKernelGGL(k_bp_zero<NR>, dim3(bt.nchunks), dim3(256), 0, s, (int64_t)bt.n, bt.chunk_cell.as<int>(),
                          bt.chunk_lo.as<int>(), bt.chunk_hi.as<int>(), w.s, w.b, w.x, w.r);
+    if (flex) launch_convert(nn, w.r, kF64, w.r32, kF32, s);
     // profiling counts launches: poll after every iteration so that none is queued past the end
     const int every = ctx.prof ? 1 : std::max(1, tu.check_every);
     const bool use_graph = tu.graph && !ctx.prof;
@@ -487,8 +538,9 @@ void solve_nr(Batch& bt, Context& ctx, const SolveOptions& opt, const double* b,
       if (opt.prec == kF64) {
         key.push_back(ctx.vec(12, (size_t)nn * 8));
       } else {
-        key.push_back(ctx.vec(13, (size_t)nn * 4));
-        key.push_back(ctx.vec(14, (size_t)nn * 4));
+        key.push_back(w.r32);
+        key.push_back(w.t32);
+        key.push_back(w.z32);
       }
       BatchGraph* g = bt.graph.get();
       const bool valid = g && g->nrhs == NR && g->prec == opt.prec && g->iters == every &&
@@ -538,6 +590,7 @@ void solve_nr(Batch& bt, Context& ctx, const SolveOptions& opt, const double* b,
       if (w.state_host[0]) break;
       ++passes;
       hipLaunchKernelGGL(k_zero_vec, dim3(grid_vec(nn)), dim3(256), 0, s, nn, w.p);
+      if (flex) launch_convert(nn, w.r, kF64, w.r32, kF32, s);
     }
     st.restarts = passes;
     st.breakdowns = w.state_host[1];

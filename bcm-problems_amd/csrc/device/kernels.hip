@@ -177,7 +177,16 @@ __device__ __forceinline__ void stage_chunk(int lane, int nb, const int* __restr
 // column and right-hand side), so the inner loop issues a single vector load
 // of V matrix values per column and reads the NR x values of that column
 // from LDS. Lanes, column groups and the per-lane column sequence are those
-// of k_spmv; the same LDS area then holds the G partial sums of the row.
+// of k_spmv; the same LDS area then holds the G partial sums of the row,
+// folded QF right-hand sides at a time. The staged chunk is kept to about
+// 8 KB per wave (seven 24x24 block columns at six right-hand sides: one
+// 7-point stencil row in one chunk), so that LDS does not cap the number of
+// resident waves below what the registers allow.
+template <int B, int NRP>
+constexpr int spmv_mr_chunk() {
+  return (8192 / (B * NRP * 8)) > 0 ? 8192 / (B * NRP * 8) : 1;
+}
+
 template <int B, int V, int NR, class TA>
 __global__ __launch_bounds__(256) void k_spmv_mr(int nsup, int n, const int* __restrict__ ia,
                                                  const int* __restrict__ ja, const TA* __restrict__ a,
@@ -188,8 +197,11 @@ __global__ __launch_bounds__(256) void k_spmv_mr(int nsup, int n, const int* __r
   constexpr int L = B / V;
   constexpr int G = 64 / L;
   constexpr int NRP = NR % 2 ? NR + 1 : NR;                          // LDS stride per column (16-byte rows)
-  constexpr int KB = (G * NR * B + B * NRP - 1) / (B * NRP);         // block columns per staged chunk
-  constexpr int WL = KB * B * NRP;                                   // doubles per wave (>= G * NR * B)
+  constexpr int KB = spmv_mr_chunk<B, NRP>();                        // block columns per staged chunk
+  constexpr int WL0 = KB * B * NRP;
+  constexpr int WL = WL0 > G * B ? WL0 : G * B;                      // doubles per wave
+  constexpr int QF = WL / (G * B) < NR ? WL / (G * B) : NR;          // right-hand sides per fold pass
+  static_assert(QF >= 1, "the fold needs room for one right-hand side");
   __shared__ __align__(16) double lds[4][WL];
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int lr = lane % L, g = lane / L;
@@ -229,24 +241,27 @@ __global__ __launch_bounds__(256) void k_spmv_mr(int nsup, int n, const int* __r
         }
       }
     }
-    wave_lds_sync();
-    if (act) {
 #pragma unroll
-      for (int q = 0; q < NR; ++q)
+    for (int q0 = 0; q0 < NR; q0 += QF) {
+      wave_lds_sync();
+      if (act) {
 #pragma unroll
-        for (int k = 0; k < V; ++k) xs[(g * NR + q) * B + lr * V + k] = acc[q][k];
-    }
-    wave_lds_sync();
-    if (lane < B) {
-      const size_t e = (size_t)row * B + lane;
+        for (int q = q0; q < q0 + QF && q < NR; ++q)
 #pragma unroll
-      for (int q = 0; q < NR; ++q) {
-        double s = 0.0;
+          for (int k = 0; k < V; ++k) xs[(g * QF + (q - q0)) * B + lr * V + k] = acc[q][k];
+      }
+      wave_lds_sync();
+      if (lane < B) {
+        const size_t e = (size_t)row * B + lane;
 #pragma unroll
-        for (int h = 0; h < G; ++h) s += xs[(h * NR + q) * B + lane];
-        const double v = bsub ? bsub[(size_t)q * n + e] - s : s;
-        y[(size_t)q * n + e] = v;
-        if (q == 0 && wdot) dotv += wdot[e] * v;
+        for (int q = q0; q < q0 + QF && q < NR; ++q) {
+          double s = 0.0;
+#pragma unroll
+          for (int h = 0; h < G; ++h) s += xs[(h * QF + (q - q0)) * B + lane];
+          const double v = bsub ? bsub[(size_t)q * n + e] - s : s;
+          y[(size_t)q * n + e] = v;
+          if (q == 0 && wdot) dotv += wdot[e] * v;
+        }
       }
     }
   }
@@ -840,7 +855,9 @@ template <int B, int NR, class TA>
 static void spmv_b(const DevBsr& A, const double* x, double* y, const double* bsub, double* dot_out,
                    const double* wdot, RedWork& rw, hipStream_t s, const int* skip) {
   constexpr int kWavesPerBlock = 4;
-  const int nwaves_max = 2048 * kWavesPerBlock;
+  // several right-hand sides: one row per wave, so the rows an XCD has in
+  // flight stay close together and their x columns are reused from its L2
+  const int nwaves_max = (NR == 1 || dot_out) ? 2048 * kWavesPerBlock : (1 << 30);
   const int rows_per_wave = (A.nsup + nwaves_max - 1) / nwaves_max;
   const int nwaves = (A.nsup + rows_per_wave - 1) / rows_per_wave;
   int grid = (nwaves + kWavesPerBlock - 1) / kWavesPerBlock;

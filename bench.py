@@ -234,7 +234,7 @@ def run_learned(args, B):
             log(f"training group {gi + 1}/{len(groups)}: {st.seconds * 1e3:.0f} ms, mean iterations {its.mean():.1f}, "
                 f"host cells ready {time.perf_counter() - t_setup0:.0f}s")
         del bt, db, dx, cells
-    rep = guess.fit(ctx, steps=args.fit_steps, lr=2e-3)
+    rep = guess.fit(ctx, steps=args.fit_steps, lr=args.fit_lr)
     log(f"guess fit: POD energy kept {rep.pod_energy:.6f}, loss {rep.loss_first:.3e} -> {rep.loss_final:.3e}, "
         f"POD {rep.seconds_pod:.2f}s, MLP {rep.seconds_train:.2f}s")
 
@@ -457,7 +457,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", choices=["learned", "drop-in"], default="learned")
     ap.add_argument("--train-cells", type=int, default=1024)
-    ap.add_argument("--fit-steps", type=int, default=4000)
+    ap.add_argument("--fit-steps", type=int, default=16000)
+    ap.add_argument("--fit-lr", type=float, default=5e-3)
     ap.add_argument("--plain-cells", type=int, default=4)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-graph", action="store_true", help="leave hipGraph replay out of the batch schedules")
