@@ -275,7 +275,10 @@ __global__ __launch_bounds__(256) void k_spmv_mr(int nsup, int n, const int* __r
 // Level-scheduled triangular sweep: one launch per dependency level; every
 // row of a level depends only on rows of earlier levels, which earlier
 // launches on the same stream have completed, so no wave ever waits on
-// another. One wavefront per block row.
+// another. One wavefront per block row. A level's waves start together and
+// each does little work, so the row's right-hand side values and inverted
+// diagonal block are loaded as soon as the row is known, overlapping the
+// dependency staging instead of following it.
 template <int B, int V, int NR, class TA, class TX>
 __global__ __launch_bounds__(256) void k_sweep_level(int nrows, const int* __restrict__ rows, int n,
                                                      const int* __restrict__ ia, const int* __restrict__ ja,
@@ -299,6 +302,20 @@ __global__ __launch_bounds__(256) void k_sweep_level(int nrows, const int* __res
   const int idx = __builtin_amdgcn_readfirstlane(blockIdx.x * 4 + wave);
   if (idx >= nrows) return;  // wave-uniform; no block-level synchronisation below
   const int row = rows[idx];
+  // the row's right-hand side values and its inverted diagonal block are loaded up front
+  TX xv[NR];
+#pragma unroll
+  for (int q = 0; q < NR; ++q) xv[q] = lane < B ? x[(size_t)q * n + (size_t)row * B + lane] : TX(0);
+  constexpr int DC = (B + G - 1) / G;
+  double dvv[DC][V];
+  {
+    const TA* dp0 = diag + (size_t)row * B * B + lr * V;
+#pragma unroll
+    for (int i = 0; i < DC; ++i) {
+      const int c = g + i * G;
+      if (act && c < B) load_vec<V>(dp0 + (size_t)c * B, dvv[i]);
+    }
+  }
   // v = x_row - sum_k F_rk y_k, with the y values of each chunk of block columns staged in LDS
   Acc acc[NR][V];
 #pragma unroll
@@ -338,7 +355,7 @@ __global__ __launch_bounds__(256) void k_sweep_level(int nrows, const int* __res
       Acc sm = Acc(0);
 #pragma unroll
       for (int h = 0; h < G; ++h) sm += xs[(h * NR + q) * B + lane];
-      vs[q * B + lane] = (Acc)x[(size_t)q * n + (size_t)row * B + lane] - sm;
+      vs[q * B + lane] = (Acc)xv[q] - sm;
     }
   }
   wave_lds_sync();
@@ -348,14 +365,15 @@ __global__ __launch_bounds__(256) void k_sweep_level(int nrows, const int* __res
 #pragma unroll
     for (int k = 0; k < V; ++k) acc[q][k] = Acc(0);
   if (act) {
-    const TA* dp = diag + (size_t)row * B * B + lr * V;
-    for (int c = g; c < B; c += G) {
-      double dv[V];
-      load_vec<V>(dp + (size_t)c * B, dv);
 #pragma unroll
-      for (int q = 0; q < NR; ++q)
+    for (int i = 0; i < DC; ++i) {
+      const int c = g + i * G;
+      if (c < B) {
 #pragma unroll
-        for (int k = 0; k < V; ++k) acc[q][k] += (Acc)dv[k] * vs[q * B + c];
+        for (int q = 0; q < NR; ++q)
+#pragma unroll
+          for (int k = 0; k < V; ++k) acc[q][k] += (Acc)dvv[i][k] * vs[q * B + c];
+      }
     }
 #pragma unroll
     for (int q = 0; q < NR; ++q)
