@@ -514,6 +514,7 @@ int bcm_context_profile_read(bcm_context c, bcm_profile* out, int reset) {
   return guarded([&] {
     need(c && out, "context_profile_read: null argument");
     c->c.prof_flush();
+    c->c.prof_collect();
     const auto& sp = c->c.prof_acc[bcm::kProfSpmv];
     const auto& sw = c->c.prof_acc[bcm::kProfSweep];
     out->spmv_ms = sp.ms;

@@ -36,7 +36,18 @@ struct BPtrs {
   double *gamma, *alpha, *beta, *rr0, *bb, *pq, *relres;
   int *active, *iters, *fresh, *flag, *conv;
   int* state;  // [0] done (no active pair), [1] breakdowns
+  int* cact;   // per cell: some pair of the cell is active (its rows are still computed)
 };
+
+// Refreshes the per-cell activity after a one-block kernel changed the pairs'.
+__device__ __forceinline__ void cell_activity(const BPtrs& s, int npair, int nrhs) {
+  __syncthreads();
+  for (int c = threadIdx.x; c < npair / nrhs; c += blockDim.x) {
+    int a = 0;
+    for (int q = 0; q < nrhs; ++q) a |= s.active[c * nrhs + q];
+    s.cact[c] = a;
+  }
+}
 
 struct VecSet {
   const double* x[3];
@@ -132,6 +143,7 @@ __global__ __launch_bounds__(kPairThreads) void k_bp_start(int npair, int nrhs, 
     if (!act) s.conv[p] = 1;
     any |= act;
   }
+  cell_activity(s, npair, nrhs);
   any = __syncthreads_or(any);
   if (threadIdx.x == 0) {
     s.state[0] = any ? 0 : 1;
@@ -190,6 +202,7 @@ __global__ __launch_bounds__(kPairThreads) void k_bp_beta(int npair, int nrhs, c
     s.gamma[p] = rz;
     any = 1;
   }
+  cell_activity(s, npair, nrhs);
   any = __syncthreads_or(any);
   if (threadIdx.x == 0 && !any) s.state[0] = 1;
 }
@@ -230,6 +243,7 @@ __global__ __launch_bounds__(kPairThreads) void k_bp_alpha(int npair, int nrhs, 
     s.alpha[p] = alpha;
     s.iters[p] += 1;
   }
+  cell_activity(s, npair, nrhs);
 }
 
 // x += alpha p; (r_old = r;) r -= alpha q for active pairs; with FLEX the
@@ -281,6 +295,7 @@ __global__ __launch_bounds__(kPairThreads) void k_bp_final(int npair, int nrhs, 
       s.active[p] = 0;
     }
   }
+  cell_activity(s, npair, nrhs);
   any = __syncthreads_or(any);
   if (threadIdx.x == 0) s.state[0] = any ? 0 : 1;
 }
@@ -300,6 +315,9 @@ int grid_vec(int64_t n) { return (int)std::max<int64_t>(1, std::min<int64_t>((n 
 // Work vectors of a batched solve (all n x nrhs, fp64) and the pair state.
 struct BWork {
   double *x, *r, *z, *p, *q, *rold, *b, *partials;
+  // profiling: algorithmic bytes per cell of the SpMV, forward and backward
+  // sweep (3 x ncell, device), counted for the cells a gated launch computes
+  const double* pbytes = nullptr;
   float *r32 = nullptr, *t32 = nullptr, *z32 = nullptr;  // FP32 vectors of the low-precision preconditioner
   BPtrs s;
   int* state_host;
@@ -339,6 +357,7 @@ BWork work(Batch& bt, Context& ctx, int nrhs, bool flex) {
   w.s.flag = iv + 3 * pd;
   w.s.conv = iv + 4 * pd;
   w.s.state = iv + 5 * pd;
+  w.s.cact = reinterpret_cast<int*>(ctx.vec(85, (size_t)bt.ncell * 4 + 64));
   w.state_host = reinterpret_cast<int*>(ctx.host_state->st);
   return w;
 }
@@ -367,20 +386,32 @@ void iteration(Batch& bt, Context& ctx, const BWork& w, const SolveOptions& opt,
   const int* done = &w.s.state[0];
   const int npair = bt.ncell * NR;
   const double eps2 = opt.eps * opt.eps;
+  // rows of cells whose pairs have all converged are left out of the sweeps and the product
+  RowGate gate;
+  gate.row_cell = bt.row_cell.as<int>();
+  gate.active = w.s.cact;
   if (flex) {
     // FP32 sweeps straight from the FP32 copy of r (written with r) into z32
     {
-      ProfScope ps(ctx, kProfSweep, sweep_bytes(t1, NR));
-      launch_sweep(t1, w.r32, w.t32, kF32, NR, s, done, ctx.sweep_sync());
+      ProfScope ps(ctx, kProfSweep, w.s.cact, w.pbytes + bt.ncell, bt.ncell);
+      launch_sweep(t1, w.r32, w.t32, kF32, NR, s, done, ctx.sweep_sync(), gate);
     }
     {
-      ProfScope ps(ctx, kProfSweep, sweep_bytes(t2, NR));
-      launch_sweep(t2, w.t32, w.z32, kF32, NR, s, done, ctx.sweep_sync());
+      ProfScope ps(ctx, kProfSweep, w.s.cact, w.pbytes + 2 * bt.ncell, bt.ncell);
+      launch_sweep(t2, w.t32, w.z32, kF32, NR, s, done, ctx.sweep_sync(), gate);
     }
     hipLaunchKernelGGL(k_bp_dots_flex<NR>, dim3(bt.nchunks), dim3(256), 0, s, (int64_t)bt.n, bt.chunk_lo.as<int>(),
                        bt.chunk_hi.as<int>(), w.r, w.rold, w.z32, w.partials, done);
   } else {
-    precond_sweeps(t1, t2, ctx, w.r, w.z, opt.prec, NR, done);
+    double* tmp = ctx.vec(12, (size_t)bt.n * NR * 8);
+    {
+      ProfScope ps(ctx, kProfSweep, w.s.cact, w.pbytes + bt.ncell, bt.ncell);
+      launch_sweep(t1, w.r, tmp, kF64, NR, s, done, ctx.sweep_sync(), gate);
+    }
+    {
+      ProfScope ps(ctx, kProfSweep, w.s.cact, w.pbytes + 2 * bt.ncell, bt.ncell);
+      launch_sweep(t2, tmp, w.z, kF64, NR, s, done, ctx.sweep_sync(), gate);
+    }
     const double* xs[2] = {w.r, w.r};
     const double* ys[2] = {w.z, w.r};
     dots<NR, 2>(bt, ctx, w, xs, ys, done);
@@ -396,8 +427,8 @@ void iteration(Batch& bt, Context& ctx, const BWork& w, const SolveOptions& opt,
                        bt.chunk_cell.as<int>(), bt.chunk_lo.as<int>(), bt.chunk_hi.as<int>(), w.s,
                        (const double*)w.z, w.p);
   {
-    ProfScope ps(ctx, kProfSpmv, spmv_bytes(bt.A, NR));
-    launch_spmv(bt.A, w.p, w.q, nullptr, NR, nullptr, nullptr, ctx.rw, s, done);
+    ProfScope ps(ctx, kProfSpmv, w.s.cact, w.pbytes, bt.ncell);
+    launch_spmv(bt.A, w.p, w.q, nullptr, NR, nullptr, nullptr, ctx.rw, s, done, gate);
   }
   {
     const double* xs[1] = {w.p};
@@ -456,6 +487,22 @@ void batch_build(Batch& bt, const std::vector<const Matrix*>& mats, const std::v
     upper.push_back(&facts[c]->u);
   }
   dev_bsr_upload_parts(parts, kF64, bt.A);
+  auto offdiag = [](const BlockCsr& m) {
+    int64_t k = 0;
+    for (int i = 0; i < m.nsup; ++i)
+      for (int j = m.ia[i]; j < m.ia[i + 1]; ++j) k += m.ja[j] != i ? 1 : 0;
+    return k;
+  };
+  bt.cell_rows.resize(mats.size());
+  bt.cell_ablk.resize(mats.size());
+  bt.cell_fdep.resize(mats.size());
+  bt.cell_bdep.resize(mats.size());
+  for (size_t c = 0; c < mats.size(); ++c) {
+    bt.cell_rows[c] = parts[c]->nsup;
+    bt.cell_ablk[c] = parts[c]->ia[parts[c]->nsup];
+    bt.cell_fdep[c] = offdiag(*lower[c]);
+    bt.cell_bdep[c] = offdiag(*upper[c]);
+  }
   full.clear();
   bt.b = bt.A.b;
   bt.n = bt.A.n;
@@ -485,6 +532,12 @@ void batch_build(Batch& bt, const std::vector<const Matrix*>& mats, const std::v
   bt.chunk_lo.upload(clo.data(), clo.size() * 4);
   bt.chunk_hi.upload(chi.data(), chi.size() * 4);
   bt.dcell_chunk0.upload(bt.cell_chunk0.data(), bt.cell_chunk0.size() * 4);
+  {
+    std::vector<int> rc((size_t)(bt.n / bt.b));
+    for (int c = 0; c < bt.ncell; ++c)
+      for (int64_t r = bt.elem0[c] / bt.b; r < bt.elem0[c + 1] / bt.b; ++r) rc[(size_t)r] = c;
+    bt.row_cell.upload(rc.data(), rc.size() * 4);
+  }
   bt.graph.reset();
   bt.tuned = false;
 }
@@ -501,6 +554,21 @@ void solve_nr(Batch& bt, Context& ctx, const SolveOptions& opt, const double* b,
   hipStream_t s = ctx.stream;
   const bool flex = opt.prec != kF64;
   BWork w = work(bt, ctx, NR, flex);
+  if (ctx.prof) {  // algorithmic bytes per cell of the gated launches
+    const int nc = bt.ncell;
+    const double B = bt.b, sv = (double)prec_bytes(opt.prec), sx = flex ? 4.0 : 8.0;
+    std::vector<double> pb((size_t)3 * nc);
+    for (int c = 0; c < nc; ++c) {
+      const double rows = (double)bt.cell_rows[c];
+      pb[c] = (double)bt.cell_ablk[c] * (B * B * 8.0 + 4.0) + rows * (4.0 + 16.0 * B * NR);
+      pb[nc + c] = ((double)bt.cell_fdep[c] + rows) * (B * B * sv + 4.0) + 2.0 * sx * rows * B * NR;
+      pb[2 * nc + c] = ((double)bt.cell_bdep[c] + rows) * (B * B * sv + 4.0) + 2.0 * sx * rows * B * NR;
+    }
+    double* d = ctx.vec(86, pb.size() * 8);
+    BCM_HIP(hipMemcpyAsync(d, pb.data(), pb.size() * 8, hipMemcpyHostToDevice, s));
+    BCM_HIP(hipStreamSynchronize(s));
+    w.pbytes = d;
+  }
   const int64_t nn = bt.n * NR;
   const double eps2 = opt.eps * opt.eps;
   const BatchTune& tu = bt.tune;
@@ -534,7 +602,8 @@ void solve_nr(Batch& bt, Context& ctx, const SolveOptions& opt, const double* b,
     const bool use_graph = tu.graph && !ctx.prof;
     if (use_graph) {
       // every buffer the iteration touches exists before capture (no allocation inside it)
-      std::vector<const void*> key = {w.x, w.r, w.z, w.p, w.q, w.rold, w.partials, w.s.state, ctx.sync.ctl.p};
+      std::vector<const void*> key = {w.x, w.r, w.z, w.p, w.q, w.rold, w.partials, w.s.state, w.s.cact,
+                                      ctx.sync.ctl.p};
       if (opt.prec == kF64) {
         key.push_back(ctx.vec(12, (size_t)nn * 8));
       } else {

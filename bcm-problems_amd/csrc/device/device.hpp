@@ -131,14 +131,23 @@ void launch_finalize(RedWork& rw, double* out, hipStream_t s, const int* skip = 
 // in rw when dot_out == kDeferred.
 // Every launch takes an optional device flag `skip`: when *skip != 0 the
 // kernel returns at once (a finished solver keeps its queued iterations inert).
+// Rows whose part (cell) is inactive can be left out of a product or sweep:
+// row_cell[r] is the part of block row r, active[c] != 0 marks the parts
+// whose rows are computed (rows of other parts keep their old values).
+struct RowGate {
+  const int* row_cell = nullptr;
+  const int* active = nullptr;
+};
+
 void launch_spmv(const DevBsr& A, const double* x, double* y, const double* bsub, int nrhs,
-                 double* dot_out, const double* wdot, RedWork& rw, hipStream_t s, const int* skip = nullptr);
+                 double* dot_out, const double* wdot, RedWork& rw, hipStream_t s, const int* skip = nullptr,
+                 RowGate gate = RowGate());
 
 // Triangular sweep y = T^-1 x for the given precision of vectors (TX). With
 // sync == nullptr: one launch per dependency level; otherwise one sync-free
 // launch using `sync`. x and y must not alias.
 void launch_sweep(const DevTri& T, const void* x, void* y, int vec_prec, int nrhs, hipStream_t s,
-                  const int* skip = nullptr, SweepSync* sync = nullptr);
+                  const int* skip = nullptr, SweepSync* sync = nullptr, RowGate gate = RowGate());
 
 // out[k] = sum_i x_k[i] * y_k[i] for K pairs (K <= kMaxDots), one pass plus a
 // finalize launch; with out == kDeferred the partials stay pending in rw.

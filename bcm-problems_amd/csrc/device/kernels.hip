@@ -192,7 +192,8 @@ __global__ __launch_bounds__(256) void k_spmv_mr(int nsup, int n, const int* __r
                                                  const int* __restrict__ ja, const TA* __restrict__ a,
                                                  const double* __restrict__ x, double* __restrict__ y,
                                                  const double* __restrict__ bsub, const double* __restrict__ wdot,
-                                                 double* partials, int rows_per_wave, const int* skip) {
+                                                 double* partials, int rows_per_wave, const int* skip,
+                                                 const int* __restrict__ gcell, const int* __restrict__ gact) {
   if (skip && *skip) return;
   constexpr int L = B / V;
   constexpr int G = 64 / L;
@@ -213,6 +214,7 @@ __global__ __launch_bounds__(256) void k_spmv_mr(int nsup, int n, const int* __r
   for (int q0 = 0; q0 < rows_per_wave; ++q0) {
     const int row = wave_global * rows_per_wave + q0;
     if (row >= nsup) break;
+    if (gcell && !gact[gcell[row]]) continue;  // wave-uniform: the row's part is inactive
     double acc[NR][V];
 #pragma unroll
     for (int q = 0; q < NR; ++q)
@@ -284,7 +286,8 @@ __global__ __launch_bounds__(256) void k_sweep_level(int nrows, const int* __res
                                                      const int* __restrict__ ia, const int* __restrict__ ja,
                                                      const TA* __restrict__ off, const TA* __restrict__ diag,
                                                      const TX* __restrict__ x, TX* __restrict__ y,
-                                                     const int* skip) {
+                                                     const int* skip, const int* __restrict__ gcell,
+                                                     const int* __restrict__ gact) {
   if (skip && *skip) return;
   using Acc = typename AccOf<TX>::type;
   static_assert(sizeof(Acc) == sizeof(TX), "staged values and partial sums share the LDS area");
@@ -323,6 +326,7 @@ __global__ __launch_bounds__(256) void k_sweep_level(int nrows, const int* __res
 #pragma unroll
     for (int k = 0; k < V; ++k) acc[q][k] = Acc(0);
   const int jb = ia[row], nblk = ia[row + 1] - jb;
+  if (gcell && !gact[gcell[row]]) return;  // wave-uniform: the row's part is inactive
   const TA* ap = off + (size_t)jb * B * B + lr * V;
   int t = g;
   for (int kb0 = 0; kb0 < nblk; kb0 += KB) {
@@ -871,7 +875,7 @@ constexpr int spmv_width() {
 
 template <int B, int NR, class TA>
 static void spmv_b(const DevBsr& A, const double* x, double* y, const double* bsub, double* dot_out,
-                   const double* wdot, RedWork& rw, hipStream_t s, const int* skip) {
+                   const double* wdot, RedWork& rw, hipStream_t s, const int* skip, RowGate gate) {
   constexpr int kWavesPerBlock = 4;
   // several right-hand sides: one row per wave, so the rows an XCD has in
   // flight stay close together and their x columns are reused from its L2
@@ -893,7 +897,7 @@ static void spmv_b(const DevBsr& A, const double* x, double* y, const double* bs
   else
     hipLaunchKernelGGL((k_spmv_mr<B, spmv_width<B>(), NR, TA>), dim3(grid), dim3(64 * kWavesPerBlock), 0, s, A.nsup,
                        A.n, A.ia.as<int>(), A.ja.as<int>(), A.a.as<TA>(), x, y, bsub, wdot, partials, rows_per_wave,
-                       skip);
+                       skip, gate.row_cell, gate.active);
   BCM_HIP(hipGetLastError());
   if (dot_out) {
     rw.pending_blocks = grid;
@@ -904,44 +908,45 @@ static void spmv_b(const DevBsr& A, const double* x, double* y, const double* bs
 
 template <int B, int NR>
 static void spmv_prec(const DevBsr& A, const double* x, double* y, const double* bsub, double* dot_out,
-                      const double* wdot, RedWork& rw, hipStream_t s, const int* skip) {
-  if (A.prec == kF64) spmv_b<B, NR, double>(A, x, y, bsub, dot_out, wdot, rw, s, skip);
-  else if (A.prec == kF32) spmv_b<B, NR, float>(A, x, y, bsub, dot_out, wdot, rw, s, skip);
-  else spmv_b<B, NR, uint16_t>(A, x, y, bsub, dot_out, wdot, rw, s, skip);
+                      const double* wdot, RedWork& rw, hipStream_t s, const int* skip, RowGate gate) {
+  if (A.prec == kF64) spmv_b<B, NR, double>(A, x, y, bsub, dot_out, wdot, rw, s, skip, gate);
+  else if (A.prec == kF32) spmv_b<B, NR, float>(A, x, y, bsub, dot_out, wdot, rw, s, skip, gate);
+  else spmv_b<B, NR, uint16_t>(A, x, y, bsub, dot_out, wdot, rw, s, skip, gate);
 }
 
 template <int B>
 static void spmv_nr(const DevBsr& A, const double* x, double* y, const double* bsub, int nrhs, double* dot_out,
-                    const double* wdot, RedWork& rw, hipStream_t s, const int* skip) {
+                    const double* wdot, RedWork& rw, hipStream_t s, const int* skip, RowGate gate) {
   int done = 0;
   while (done < nrhs) {
     const int left = nrhs - done;
     const size_t o = (size_t)done * A.n;
     const double* bs = bsub ? bsub + o : nullptr;
     double* dd = done == 0 ? dot_out : nullptr;
-    if (left >= 6) { spmv_prec<B, 6>(A, x + o, y + o, bs, dd, wdot, rw, s, skip); done += 6; }
-    else if (left >= 3) { spmv_prec<B, 3>(A, x + o, y + o, bs, dd, wdot, rw, s, skip); done += 3; }
-    else { spmv_prec<B, 1>(A, x + o, y + o, bs, dd, wdot, rw, s, skip); done += 1; }
+    if (left >= 6) { spmv_prec<B, 6>(A, x + o, y + o, bs, dd, wdot, rw, s, skip, gate); done += 6; }
+    else if (left >= 3) { spmv_prec<B, 3>(A, x + o, y + o, bs, dd, wdot, rw, s, skip, gate); done += 3; }
+    else { spmv_prec<B, 1>(A, x + o, y + o, bs, dd, wdot, rw, s, skip, RowGate()); done += 1; }
   }
 }
 
 void launch_spmv(const DevBsr& A, const double* x, double* y, const double* bsub, int nrhs, double* dot_out,
-                 const double* wdot, RedWork& rw, hipStream_t s, const int* skip) {
+                 const double* wdot, RedWork& rw, hipStream_t s, const int* skip, RowGate gate) {
   switch (A.b) {
-    case 3: spmv_nr<3>(A, x, y, bsub, nrhs, dot_out, wdot, rw, s, skip); break;
-    case 6: spmv_nr<6>(A, x, y, bsub, nrhs, dot_out, wdot, rw, s, skip); break;
-    case 8: spmv_nr<8>(A, x, y, bsub, nrhs, dot_out, wdot, rw, s, skip); break;
-    case 9: spmv_nr<9>(A, x, y, bsub, nrhs, dot_out, wdot, rw, s, skip); break;
-    case 12: spmv_nr<12>(A, x, y, bsub, nrhs, dot_out, wdot, rw, s, skip); break;
-    case 16: spmv_nr<16>(A, x, y, bsub, nrhs, dot_out, wdot, rw, s, skip); break;
-    case 24: spmv_nr<24>(A, x, y, bsub, nrhs, dot_out, wdot, rw, s, skip); break;
-    case 32: spmv_nr<32>(A, x, y, bsub, nrhs, dot_out, wdot, rw, s, skip); break;
+    case 3: spmv_nr<3>(A, x, y, bsub, nrhs, dot_out, wdot, rw, s, skip, gate); break;
+    case 6: spmv_nr<6>(A, x, y, bsub, nrhs, dot_out, wdot, rw, s, skip, gate); break;
+    case 8: spmv_nr<8>(A, x, y, bsub, nrhs, dot_out, wdot, rw, s, skip, gate); break;
+    case 9: spmv_nr<9>(A, x, y, bsub, nrhs, dot_out, wdot, rw, s, skip, gate); break;
+    case 12: spmv_nr<12>(A, x, y, bsub, nrhs, dot_out, wdot, rw, s, skip, gate); break;
+    case 16: spmv_nr<16>(A, x, y, bsub, nrhs, dot_out, wdot, rw, s, skip, gate); break;
+    case 24: spmv_nr<24>(A, x, y, bsub, nrhs, dot_out, wdot, rw, s, skip, gate); break;
+    case 32: spmv_nr<32>(A, x, y, bsub, nrhs, dot_out, wdot, rw, s, skip, gate); break;
     default: throw Error(kErrArg, "unsupported block size for the device SpMV (use 3,6,8,9,12,16,24,32)");
   }
 }
 
 template <int B, int NR, class TA, class TX>
-static void sweep_b(const DevTri& T, const void* x, void* y, hipStream_t s, const int* skip, SweepSync* sync) {
+static void sweep_b(const DevTri& T, const void* x, void* y, hipStream_t s, const int* skip, SweepSync* sync,
+                    RowGate gate) {
   if (sync) {
     // every wave must be resident: cooperative launch of at most the occupancy limit
     auto kern = k_sweep_free<B, NR, TA, TX>;
@@ -973,48 +978,48 @@ static void sweep_b(const DevTri& T, const void* x, void* y, hipStream_t s, cons
     hipLaunchKernelGGL((k_sweep_level<B, spmv_width<B>(), NR, TA, TX>), dim3((nr + 3) / 4), dim3(256), 0, s, nr,
                        rows + r0, T.n,
                        T.ia.as<int>(), T.ja.as<int>(), T.off.as<TA>(), T.diag.as<TA>(), static_cast<const TX*>(x),
-                       static_cast<TX*>(y), skip);
+                       static_cast<TX*>(y), skip, gate.row_cell, gate.active);
   }
   BCM_HIP(hipGetLastError());
 }
 
 template <int B, int NR>
 static void sweep_prec(const DevTri& T, const void* x, void* y, int vec_prec, hipStream_t s, const int* skip,
-                       SweepSync* sync) {
-  if (T.prec == kF64 && vec_prec == kF64) sweep_b<B, NR, double, double>(T, x, y, s, skip, sync);
-  else if (T.prec == kF32 && vec_prec == kF32) sweep_b<B, NR, float, float>(T, x, y, s, skip, sync);
-  else if (T.prec == kBF16 && vec_prec == kF32) sweep_b<B, NR, uint16_t, float>(T, x, y, s, skip, sync);
+                       SweepSync* sync, RowGate gate) {
+  if (T.prec == kF64 && vec_prec == kF64) sweep_b<B, NR, double, double>(T, x, y, s, skip, sync, gate);
+  else if (T.prec == kF32 && vec_prec == kF32) sweep_b<B, NR, float, float>(T, x, y, s, skip, sync, gate);
+  else if (T.prec == kBF16 && vec_prec == kF32) sweep_b<B, NR, uint16_t, float>(T, x, y, s, skip, sync, gate);
   else throw Error(kErrArg, "unsupported factor/vector precision pair (fp64/fp64, fp32/fp32, bf16/fp32)");
 }
 
 template <int B>
 static void sweep_nr(const DevTri& T, const void* x, void* y, int vec_prec, int nrhs, hipStream_t s,
-                     const int* skip, SweepSync* sync) {
+                     const int* skip, SweepSync* sync, RowGate gate) {
   const size_t es = prec_bytes(vec_prec);
   int done = 0;
   while (done < nrhs) {
     const int left = nrhs - done;
     const char* xs = static_cast<const char*>(x) + (size_t)done * T.n * es;
     char* ys = static_cast<char*>(y) + (size_t)done * T.n * es;
-    if (left >= 6) { sweep_prec<B, 6>(T, xs, ys, vec_prec, s, skip, sync); done += 6; }
-    else if (left >= 3) { sweep_prec<B, 3>(T, xs, ys, vec_prec, s, skip, sync); done += 3; }
-    else { sweep_prec<B, 1>(T, xs, ys, vec_prec, s, skip, sync); done += 1; }
+    if (left >= 6) { sweep_prec<B, 6>(T, xs, ys, vec_prec, s, skip, sync, gate); done += 6; }
+    else if (left >= 3) { sweep_prec<B, 3>(T, xs, ys, vec_prec, s, skip, sync, gate); done += 3; }
+    else { sweep_prec<B, 1>(T, xs, ys, vec_prec, s, skip, sync, gate); done += 1; }
   }
 }
 
 void launch_sweep(const DevTri& T, const void* x, void* y, int vec_prec, int nrhs, hipStream_t s,
-                  const int* skip, SweepSync* sync) {
+                  const int* skip, SweepSync* sync, RowGate gate) {
   if (x == y) throw Error(kErrArg, "triangular sweep: input and output must not alias");
   if (sync) sync->ensure(T.nsup, s);
   switch (T.b) {
-    case 3: sweep_nr<3>(T, x, y, vec_prec, nrhs, s, skip, sync); break;
-    case 6: sweep_nr<6>(T, x, y, vec_prec, nrhs, s, skip, sync); break;
-    case 8: sweep_nr<8>(T, x, y, vec_prec, nrhs, s, skip, sync); break;
-    case 9: sweep_nr<9>(T, x, y, vec_prec, nrhs, s, skip, sync); break;
-    case 12: sweep_nr<12>(T, x, y, vec_prec, nrhs, s, skip, sync); break;
-    case 16: sweep_nr<16>(T, x, y, vec_prec, nrhs, s, skip, sync); break;
-    case 24: sweep_nr<24>(T, x, y, vec_prec, nrhs, s, skip, sync); break;
-    case 32: sweep_nr<32>(T, x, y, vec_prec, nrhs, s, skip, sync); break;
+    case 3: sweep_nr<3>(T, x, y, vec_prec, nrhs, s, skip, sync, gate); break;
+    case 6: sweep_nr<6>(T, x, y, vec_prec, nrhs, s, skip, sync, gate); break;
+    case 8: sweep_nr<8>(T, x, y, vec_prec, nrhs, s, skip, sync, gate); break;
+    case 9: sweep_nr<9>(T, x, y, vec_prec, nrhs, s, skip, sync, gate); break;
+    case 12: sweep_nr<12>(T, x, y, vec_prec, nrhs, s, skip, sync, gate); break;
+    case 16: sweep_nr<16>(T, x, y, vec_prec, nrhs, s, skip, sync, gate); break;
+    case 24: sweep_nr<24>(T, x, y, vec_prec, nrhs, s, skip, sync, gate); break;
+    case 32: sweep_nr<32>(T, x, y, vec_prec, nrhs, s, skip, sync, gate); break;
     default: throw Error(kErrArg, "unsupported block size for the device sweep (use 3,6,8,9,12,16,24,32)");
   }
 }

@@ -383,6 +383,50 @@ void Context::prof_end(int kind, hipEvent_t e0, double bytes) {
   if (pending.size() > 4096) prof_flush();
 }
 
+namespace {
+// acc += sum_c active[c] * part_bytes[c] (one block, fixed order).
+__global__ void k_gate_bytes(int np, const int* __restrict__ active, const double* __restrict__ part_bytes,
+                             double* acc) {
+  __shared__ double sm[256];
+  double t = 0.0;
+  for (int c = threadIdx.x; c < np; c += blockDim.x) t += active[c] ? part_bytes[c] : 0.0;
+  sm[threadIdx.x] = t;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double s = 0.0;
+    for (int i = 0; i < (int)blockDim.x; ++i) s += sm[i];
+    *acc += s;
+  }
+}
+}  // namespace
+
+ProfScope::~ProfScope() {
+  if (!(c.prof && e0)) return;
+  c.prof_end(kind, e0, bytes);
+  if (gate_active) {  // after the closing event: not part of the timed span
+    hipLaunchKernelGGL(k_gate_bytes, dim3(1), dim3(256), 0, c.stream, nparts, gate_active, part_bytes,
+                       c.prof_dev_bytes() + kind);
+    (void)hipGetLastError();
+  }
+}
+
+double* Context::prof_dev_bytes() {
+  if (!prof_dev.p) {
+    prof_dev.alloc(kProfKinds * sizeof(double));
+    BCM_HIP(hipMemsetAsync(prof_dev.p, 0, kProfKinds * sizeof(double), stream));
+  }
+  return prof_dev.as<double>();
+}
+
+void Context::prof_collect() {
+  if (!prof_dev.p) return;
+  double h[kProfKinds];
+  BCM_HIP(hipMemcpyAsync(h, prof_dev.p, sizeof(h), hipMemcpyDeviceToHost, stream));
+  BCM_HIP(hipStreamSynchronize(stream));
+  for (int k = 0; k < kProfKinds; ++k) prof_acc[k].bytes += h[k];
+  BCM_HIP(hipMemsetAsync(prof_dev.p, 0, kProfKinds * sizeof(double), stream));
+}
+
 void Context::prof_flush() {
   if (pending.empty()) return;
   BCM_HIP(hipEventSynchronize(pending.back().e1));
@@ -399,27 +443,27 @@ void Context::prof_flush() {
 }
 
 void precond_sweeps(const DevTri& t1, const DevTri& t2, Context& ctx, const double* r, double* z, int prec, int nrhs,
-                    const int* skip) {
+                    const int* skip, RowGate gate) {
   const int64_t n = t1.n;
   if (prec == kF64) {
     double* tmp = ctx.vec(12, (size_t)n * nrhs * 8);
     {
       ProfScope ps(ctx, kProfSweep, sweep_bytes(t1, nrhs));
-      launch_sweep(t1, r, tmp, kF64, nrhs, ctx.stream, skip, ctx.sweep_sync());
+      launch_sweep(t1, r, tmp, kF64, nrhs, ctx.stream, skip, ctx.sweep_sync(), gate);
     }
     ProfScope ps(ctx, kProfSweep, sweep_bytes(t2, nrhs));
-    launch_sweep(t2, tmp, z, kF64, nrhs, ctx.stream, skip, ctx.sweep_sync());
+    launch_sweep(t2, tmp, z, kF64, nrhs, ctx.stream, skip, ctx.sweep_sync(), gate);
   } else {
     float* r32 = reinterpret_cast<float*>(ctx.vec(13, (size_t)n * nrhs * 4));
     float* t32 = reinterpret_cast<float*>(ctx.vec(14, (size_t)n * nrhs * 4));
     launch_convert(n * nrhs, r, kF64, r32, kF32, ctx.stream, skip);
     {
       ProfScope ps(ctx, kProfSweep, sweep_bytes(t1, nrhs));
-      launch_sweep(t1, r32, t32, kF32, nrhs, ctx.stream, skip, ctx.sweep_sync());
+      launch_sweep(t1, r32, t32, kF32, nrhs, ctx.stream, skip, ctx.sweep_sync(), gate);
     }
     {
       ProfScope ps(ctx, kProfSweep, sweep_bytes(t2, nrhs));
-      launch_sweep(t2, t32, r32, kF32, nrhs, ctx.stream, skip, ctx.sweep_sync());
+      launch_sweep(t2, t32, r32, kF32, nrhs, ctx.stream, skip, ctx.sweep_sync(), gate);
     }
     launch_convert(n * nrhs, r32, kF32, z, kF64, ctx.stream, skip);
   }

@@ -77,6 +77,12 @@ struct Context {
   hipEvent_t prof_begin();
   void prof_end(int kind, hipEvent_t e0, double bytes);
   void prof_flush();
+  // Bytes of launches whose work is only known on the device (row-gated
+  // products and sweeps): accumulated by device kernels per kind, added to
+  // prof_acc when read.
+  DevBuf prof_dev;
+  double* prof_dev_bytes();  // kProfKinds device doubles (zeroed on first use)
+  void prof_collect();       // adds and clears the device-side bytes
 };
 
 // Times the launches issued during its lifetime as one kernel of `kind`.
@@ -85,12 +91,18 @@ struct ProfScope {
   int kind;
   double bytes;
   hipEvent_t e0 = nullptr;
+  // row-gated launch: bytes = sum over parts of active[c] * part_bytes[c] (device)
+  const int* gate_active = nullptr;
+  const double* part_bytes = nullptr;
+  int nparts = 0;
   ProfScope(Context& ctx, int k, double b) : c(ctx), kind(k), bytes(b) {
     if (c.prof) e0 = c.prof_begin();
   }
-  ~ProfScope() {
-    if (c.prof && e0) c.prof_end(kind, e0, bytes);
+  ProfScope(Context& ctx, int k, const int* active, const double* pbytes, int np)
+      : c(ctx), kind(k), bytes(0.0), gate_active(active), part_bytes(pbytes), nparts(np) {
+    if (c.prof) e0 = c.prof_begin();
   }
+  ~ProfScope();
 };
 
 inline double spmv_bytes(const DevBsr& A, int nrhs) {
@@ -143,7 +155,7 @@ void precond_apply(Factor& F, Context& ctx, const double* r, double* z, int prec
 // z = M^-1 r through the sweep pair (t1 then t2) in the given precision
 // (fp64 vectors, or fp32 vectors converted on the way in and out).
 void precond_sweeps(const DevTri& t1, const DevTri& t2, Context& ctx, const double* r, double* z, int prec, int nrhs,
-                    const int* skip = nullptr);
+                    const int* skip = nullptr, RowGate gate = RowGate());
 
 // FGMRES(m) driver (fgmres.hip).
 void solve_fgmres(Matrix& A, Factor& F, Context& ctx, const SolveOptions& opt, const double* b, double* x,
@@ -175,6 +187,10 @@ struct Batch {
   int nchunks = 0;                    // reduction chunks (each inside one cell)
   std::vector<int> cell_chunk0;       // ncell + 1
   DevBuf chunk_cell, chunk_lo, chunk_hi, dcell_chunk0;
+  DevBuf row_cell;                    // cell of every block row of the union
+  // per cell: block rows, blocks of A, off-diagonal blocks of the forward
+  // and backward sweeps (the algorithmic bytes of row-gated launches)
+  std::vector<int64_t> cell_rows, cell_ablk, cell_fdep, cell_bdep;
   BatchTune tune;
   bool tuned = false;
   std::unique_ptr<BatchGraph> graph;
