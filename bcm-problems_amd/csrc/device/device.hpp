@@ -103,6 +103,7 @@ struct SweepSync {
   bool level_order = true;  // process rows sorted by dependency level (false: natural sweep order)
   void ensure(int nsup, hipStream_t s);
   int error(hipStream_t s);
+  void clear(hipStream_t s);  // resets the wait-bound error flag
 };
 
 // Per-block partial sums of the reductions. A reduction kernel writes one

@@ -655,9 +655,13 @@ void DevBuf::upload(const void* src, size_t nbytes, hipStream_t s) {
 }
 
 int device_cu_count() {
-  int dev = 0, cus = 0;
-  BCM_HIP(hipGetDevice(&dev));
-  BCM_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+  // one GPU per process: the first query holds for every later launch
+  static const int cus = [] {
+    int dev = 0, c = 0;
+    BCM_HIP(hipGetDevice(&dev));
+    BCM_HIP(hipDeviceGetAttribute(&c, hipDeviceAttributeMultiprocessorCount, dev));
+    return c;
+  }();
   return cus;
 }
 
@@ -843,6 +847,12 @@ void SweepSync::ensure(int n, hipStream_t s) {
   nsup = n;
 }
 
+void SweepSync::clear(hipStream_t s) {
+  if (!ctl.p) return;
+  BCM_HIP(hipMemsetAsync(ctl.p, 0, 64, s));
+  BCM_HIP(hipStreamSynchronize(s));
+}
+
 int SweepSync::error(hipStream_t s) {
   if (!ctl.p) return 0;
   int h[4] = {0, 0, 0, 0};
@@ -950,8 +960,12 @@ static void sweep_b(const DevTri& T, const void* x, void* y, hipStream_t s, cons
   if (sync) {
     // every wave must be resident: cooperative launch of at most the occupancy limit
     auto kern = k_sweep_free<B, NR, TA, TX>;
-    int per_cu = 0;
-    BCM_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, 64, 0));
+    // resident one-wave blocks per CU of this instantiation (queried once)
+    static const int per_cu = [&] {
+      int v = 0;
+      BCM_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&v, kern, 64, 0));
+      return v;
+    }();
     const int waves = std::max(1, std::min(T.nsup, std::min(per_cu, sync->waves_per_cu) * device_cu_count()));
     const int64_t ny = (int64_t)T.n * NR;
     TX* yy = static_cast<TX*>(y);

@@ -728,16 +728,45 @@ void solve(Matrix& A, Factor& F, Context& ctx, const SolveOptions& opt, const do
   st = SolveStats();
   A.to_device(opt.kind == kLanczos);
   if (A.d.n != F.u.n()) throw Error(kErrArg, "matrix and factor sizes differ");
-  switch (opt.kind) {
-    case kPcg: pcg(A, F, ctx, opt, b, x, st); break;
-    case kBicgstab: bicgstab(A, F, ctx, opt, b, x, st); break;
-    case kCgPar: cg_par(A, F, ctx, opt, b, x, st); break;
-    case kFgmres: solve_fgmres(A, F, ctx, opt, b, x, st); break;
-    case kLanczos: solve_lanczos(A, F, ctx, opt, b, x, st); break;
-    case kBlockGmres: solve_block_gmres(A, F, ctx, opt, 1, b, x, st); break;
-    default: throw Error(kErrArg, "solver kind not available");
+  auto run = [&] {
+    switch (opt.kind) {
+      case kPcg: pcg(A, F, ctx, opt, b, x, st); break;
+      case kBicgstab: bicgstab(A, F, ctx, opt, b, x, st); break;
+      case kCgPar: cg_par(A, F, ctx, opt, b, x, st); break;
+      case kFgmres: solve_fgmres(A, F, ctx, opt, b, x, st); break;
+      case kLanczos: solve_lanczos(A, F, ctx, opt, b, x, st); break;
+      case kBlockGmres: solve_block_gmres(A, F, ctx, opt, 1, b, x, st); break;
+      default: throw Error(kErrArg, "solver kind not available");
+    }
+    BCM_HIP(hipStreamSynchronize(ctx.stream));
+  };
+  // Sync-free sweeps are first launched as ordinary kernels: the grid never
+  // exceeds what the device holds at once, so on an otherwise idle GPU every
+  // wave is resident, without the cooperative launch's cost. If a wait hit
+  // its bound (another process held part of the GPU), the solve is repeated
+  // from the same x0 with cooperative launches, which guarantee residency.
+  const bool saved = ctx.sync.cooperative;
+  const bool fast = ctx.sweep_mode == 1 && saved;
+  const int64_t n = A.d.n;
+  double* x0 = nullptr;
+  if (fast) {
+    x0 = ctx.vec(87, (size_t)n * 8);
+    launch_copy(n, x, x0, ctx.stream);
+    ctx.sync.cooperative = false;
   }
-  BCM_HIP(hipStreamSynchronize(ctx.stream));
+  try {
+    run();
+  } catch (...) {
+    ctx.sync.cooperative = saved;
+    throw;
+  }
+  ctx.sync.cooperative = saved;
+  if (fast && ctx.sync.error(ctx.stream)) {
+    ctx.sync.clear(ctx.stream);
+    launch_copy(n, x0, x, ctx.stream);
+    st = SolveStats();
+    run();
+  }
   ctx.check_sweep_error();
   st.seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
 }

@@ -26,7 +26,10 @@ of the same cells for the comparison, and the reference's own serial Cg
 CPU baseline.
 
 `--workload drop-in` runs the plain FP64 nonsymmetric drop-in instead (32^3
-cells, block 24, ILU2 + BiCGStab).
+cells, block 24, ILU2 + BiCGStab); `--workload precision` the FP64
+FGMRES(30) around FP64 / FP32 / BF16 factor sweeps on the same system;
+`--workload batch` one batched solve of `--group` cells of uneven sizes
+8^3..24^3.
 """
 
 import argparse
@@ -318,6 +321,14 @@ def run_learned(args, B):
     plain_rate = nplain / plain_s
 
     roof, kernels = roofline_from(prof, sweep_single_launch=tune["sweep_mode"] != 0)
+    traffic_file = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles",
+                                "spmv_union_traffic_64cells_16cube_b24.json")
+    if roof["kernel"] == "spmv" and args.group == 64 and os.path.exists(traffic_file):
+        # measured HBM bytes per byte of algorithmic traffic of this SpMV, applied to this run's launches
+        t = json.load(open(traffic_file))
+        ratio = t["hbm_bytes_full_launch"] / t["algorithmic_bytes_full_launch"]
+        roof["traffic"] = ratio * roof["bytes_per_launch"]
+        roof["traffic_source"] = f"{t['source']}: {ratio:.4f} HBM bytes per algorithmic byte"
     kernels["sweep"]["launches_are"] = "sweeps (each one launch)" if tune["sweep_mode"] != 0 else \
         "sweeps (each one launch per dependency level)"
     value = timed_pairs / timed_s
@@ -464,12 +475,197 @@ def run_dropin(args, B):
     }), flush=True)
 
 
+def run_precision(args, B):
+    """FP64 flexible GMRES(30) around FP64, FP32 and BF16 ILU2 factor sweeps on
+    the 32^3 nonsymmetric system; every variant driven to FP64 true relative
+    residual 1e-10 and compared with the FP64-preconditioner solution."""
+    import oracle as O
+
+    seed = 20261203
+    t0 = time.perf_counter()
+    m, rhs = B.gen_seepage3d(32, 32, 32, 24, 0.3, seed)
+    order = m.rcm_order()
+    mo = m.reorder(order)
+    b = mo.reorder_vector(order, rhs)
+    f = B.Factor.ilu2(mo, B.default_params())
+    fct_s = time.perf_counter() - t0
+    log(f"precision system n={mo.n}, ILU2 {fct_s:.1f}s")
+    ctx = B.Context()
+    db, dx = B.DeviceArray.from_host(b), B.DeviceArray(mo.n)
+    res, xref, ok = {}, None, True
+    for name, prec in (("f64", B.F64), ("f32", B.F32), ("bf16", B.BF16)):
+        opts = B.default_opts(kind=B.FGMRES, eps=EPS, ittype=3, restart=30, precision=prec, check_every=4,
+                              niter=3000)
+        for _ in range(max(1, args.warmup)):  # also uploads this precision's factor image
+            dx.zero()
+            ctx.solve(mo, f, db, dx, opts)
+        ctx.sync()
+        t1 = time.perf_counter()
+        for _ in range(args.steps):
+            dx.zero()
+            st = ctx.solve(mo, f, db, dx, opts)
+        ctx.sync()
+        dt = (time.perf_counter() - t1) / args.steps
+        x = dx.host()
+        if xref is None:
+            xref = x
+        ok = ok and st.converged == 1 and st.relres_true <= EPS
+        res[name] = dict(ms_per_solve=dt * 1e3, solves_per_s=1.0 / dt, iterations=st.iterations,
+                         restarts=st.restarts, relres_true=st.relres_true,
+                         rel_diff_vs_f64=float(np.linalg.norm(x - xref) / np.linalg.norm(xref)))
+        log(f"{name}: {dt * 1e3:.2f} ms/solve, {st.iterations} iterations, relres {st.relres_true:.2e}, "
+            f"|x - x64|/|x64| {res[name]['rel_diff_vs_f64']:.2e}")
+    # kernel rates of one FP32 solve (outside the timed region)
+    ctx.profile(True)
+    ctx.profile_read(reset=True)
+    dx.zero()
+    ctx.solve(mo, f, db, dx, B.default_opts(kind=B.FGMRES, eps=EPS, ittype=3, restart=30, precision=B.F32,
+                                            check_every=1, niter=3000))
+    ctx.sync()
+    prof = ctx.profile_read()
+    ctx.profile(False)
+    roof, kernels = roofline_from(prof)
+    cpu = None
+    if not args.no_cpu:
+        A = O.Block(*mo.arrays())
+        L = O.Block(A.sprnd, *f.arrays(0, A.nsup))
+        U = O.Block(A.sprnd, *f.arrays(1, A.nsup))
+        t1 = time.perf_counter()
+        _, r = O.fgmres(A, L, U, b, m=30, niter=8, eps=1e-300)
+        per = (time.perf_counter() - t1) / max(r.nitperf, 1)
+        its = res["f64"]["iterations"]
+        cpu = dict(value=1.0 / (per * its), unit="solves/s", cores=1, kind="port",
+                   sample=f"{r.nitperf} FGMRES iterations (FP64 factors) by the oracle on one core of {cpu_model()}, "
+                          f"scaled to the {its} iterations of one solve")
+    print(json.dumps({
+        "metric": METRIC, "value": res["f32"]["solves_per_s"], "unit": "solves/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": res["f32"]["ms_per_solve"], "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f64 outer / f32 factors",
+        "data": "synthetic (seeded generator)",
+        "config": {"workload": f"mixed precision: FP64 FGMRES(30) with ILU2 factors stored and applied in FP32 "
+                               f"(value) / BF16 / FP64, 3D nonsymmetric 32^3 cells, block 24, N={mo.n}, to true "
+                               f"rel-res {EPS}", "seed": seed},
+        "valid": bool(ok), "precisions": res, "factor_seconds_host": fct_s,
+        "kernels": kernels, "roofline": roof, "cpu_baseline": cpu,
+    }), flush=True)
+
+
+def run_batch(args, B):
+    """64 periodic elasticity cells of uneven sizes 8^3..24^3 x 6 load cases
+    in one batched block-PCG solve (FP32 factors, FP64 outer recurrence,
+    autotuned schedule) from x0 = 0, against the plain FP64 drop-in one load
+    case at a time."""
+    rng = np.random.default_rng(20261207)
+    ncell = args.group
+    sizes = rng.integers(8, 25, ncell)
+    params = draw_params(ncell, 20261207)
+    fp = B.default_params(tau1=0.1, tau2=0.01)
+    orders = {}
+    for msz in sorted(set(int(v) for v in sizes)):
+        mat0, _ = B.gen_elastic_cell(msz, 0.3, 10.0, 0.3, 0.0, TEMPLATE_SEED)
+        orders[msz] = mat0.rcm_order()
+
+    def build(i):
+        msz = int(sizes[i])
+        r, lc, nu, sh = params[i]
+        mat, rhs = B.gen_elastic_cell(msz, r, 10 ** lc, nu, sh, TEMPLATE_SEED)
+        mo = mat.reorder(orders[msz])
+        return mo, B.Factor.ic2(mo, fp), mo.reorder_vector(orders[msz], rhs)
+
+    t0 = time.perf_counter()
+    with ThreadPoolExecutor(host_threads()) as ex:
+        cells = list(ex.map(build, range(ncell)))
+    setup_s = time.perf_counter() - t0
+    log(f"batch of {ncell} cells ({int(sizes.min())}^3..{int(sizes.max())}^3), host setup {setup_s:.1f}s")
+    bt = B.Batch([(mo, f) for mo, f, _ in cells], prec_mask=1 << B.F32)
+    b = bt.gather([c[2] for c in cells])
+    db = B.DeviceArray.from_host(b)
+    ctx = B.Context()
+    opts = B.default_opts(kind=B.BLOCKPCG, eps=EPS, precision=B.F32, niter=5000)
+    t = bt.autotune(ctx, db, opts, 6, allow_graph=not args.no_graph)
+    tune = dict(sweep_mode=t.sweep_mode, waves_per_cu=t.waves_per_cu, level_order=t.level_order, graph=t.graph,
+                check_every=t.check_every)
+    log(f"autotuned schedule {tune} ({t.ms_per_iter:.2f} ms/iteration)")
+    dx = B.DeviceArray(db.shape)
+    for _ in range(max(1, args.warmup)):
+        dx.zero()
+        bt.solve(ctx, db, dx, opts, 6)
+    ctx.sync()
+    timed, pairs, ok, its_all, rr_max = 0.0, 0, True, [], 0.0
+    for _ in range(args.steps):
+        dx.zero()
+        ctx.sync()
+        t1 = time.perf_counter()
+        st, its, rr = bt.solve(ctx, db, dx, opts, 6)
+        timed += time.perf_counter() - t1
+        pairs += st.pairs
+        its_all.append(its)
+        rr_max = max(rr_max, float(rr.max()))
+        ok = ok and st.converged == st.pairs and float(rr.max()) <= EPS
+    value = pairs / timed
+    # kernel rates of one more solve (outside the timed region)
+    bt.set_tune(**dict(tune, graph=0))
+    ctx.profile(True)
+    ctx.profile_read(reset=True)
+    dx.zero()
+    bt.solve(ctx, db, dx, opts, 6)
+    ctx.sync()
+    prof = ctx.profile_read()
+    ctx.profile(False)
+    roof, kernels = roofline_from(prof, sweep_single_launch=tune["sweep_mode"] != 0)
+    # plain FP64 drop-in on a sample of the cells, one load case at a time
+    popts = B.default_opts(kind=B.PCG, eps=EPS, ittype=3, check_every=8, niter=5000)
+    pctx = B.Context()
+    sample = cells[:args.plain_cells]
+    for mo, f, bb in sample:
+        d1 = B.DeviceArray.from_host(np.ascontiguousarray(bb[:, 0]))
+        x1 = B.DeviceArray(mo.n)
+        x1.zero()
+        pctx.solve(mo, f, d1, x1, popts)
+    nplain, plain_s = 0, 0.0
+    for mo, f, bb in sample:
+        for q in range(6):
+            d1 = B.DeviceArray.from_host(np.ascontiguousarray(bb[:, q]))
+            x1 = B.DeviceArray(mo.n)
+            x1.zero()
+            pctx.sync()
+            t1 = time.perf_counter()
+            stp = pctx.solve(mo, f, d1, x1, popts)
+            plain_s += time.perf_counter() - t1
+            nplain += 1
+            ok = ok and stp.converged == 1
+    # the sample's share of the batch's unknowns, to compare like with like
+    frac = sum(c[0].n for c in sample) / bt.n
+    cpu = None if args.no_cpu else cpu_baseline_ref(B, sample, 0.0)
+    its_all = np.concatenate([i.ravel() for i in its_all])
+    print(json.dumps({
+        "metric": METRIC, "value": value, "unit": "solves/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": timed / args.steps * 1e3, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f64 outer / f32 factors",
+        "data": "synthetic (seeded homogenization-cell generator)",
+        "config": {"workload": f"batching: {ncell} periodic elasticity cells of uneven sizes "
+                               f"{int(sizes.min())}^3..{int(sizes.max())}^3 (block 24, N={bt.n} in total), 6 load "
+                               f"cases each, IC2(0.1, 0.01) in FP32, one batched block-PCG solve from x0 = 0 to FP64 "
+                               f"true rel-res {EPS}", "cells": ncell, "solve": "(cell, load case) pairs"},
+        "valid": bool(ok), "iterations": {"mean": float(its_all.mean()), "max": int(its_all.max())},
+        "relres_true_max": rr_max, "schedule": tune,
+        "plain_fp64_gpu": {"value": nplain / plain_s, "unit": "solves/s",
+                           "sample": f"{nplain} solves of the first {len(sample)} cells ({frac:.3f} of the unknowns), "
+                                     f"one at a time, FP64 IC2, x0 = 0",
+                           "value_size_adjusted": 6 * ncell * frac / plain_s,
+                           "size_adjusted": "the sample's time per unknown and load case applied to the batch's "
+                                            "unknowns: solves/s the plain path would reach on this batch"},
+        "speedup_vs_plain_size_adjusted": value / (6 * ncell * frac / plain_s),
+        "host_setup_seconds": setup_s, "kernels": kernels, "roofline": roof, "cpu_baseline": cpu,
+    }), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=6, help="timed evaluation groups (learned) or solves (drop-in)")
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", choices=["learned", "drop-in"], default="learned")
+    ap.add_argument("--workload", choices=["learned", "drop-in", "precision", "batch"], default="learned")
     ap.add_argument("--train-cells", type=int, default=1024)
     ap.add_argument("--fit-steps", type=int, default=16000)
     ap.add_argument("--fit-lr", type=float, default=5e-3)
@@ -491,6 +687,10 @@ def main():
         raise SystemExit("no GPU visible: the solve phase runs on the GPU only")
     if args.workload == "learned":
         run_learned(args, B)
+    elif args.workload == "precision":
+        run_precision(args, B)
+    elif args.workload == "batch":
+        run_batch(args, B)
     else:
         run_dropin(args, B)
 
