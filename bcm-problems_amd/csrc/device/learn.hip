@@ -60,13 +60,18 @@ __global__ __launch_bounds__(256) void k_gemm_bf16(int M, int N, int K, const TA
   const int ia = tm * 16 + r;  // row of A this lane feeds
   const int jb = tn * 16 + r;  // column of B this lane feeds
   v4f acc = {0.f, 0.f, 0.f, 0.f};
-  for (int k0 = 0; k0 < K; k0 += 32) {
+  // split K: wave group blockIdx.y adds the K range [kz0, kz1) into its own
+  // partial tile (C + blockIdx.y * M * ldc); no epilogue then
+  const int kper = (((K + gridDim.y - 1) / gridDim.y) + 31) / 32 * 32;
+  const int kz0 = blockIdx.y * kper, kz1 = min(K, kz0 + kper);
+  C += (size_t)blockIdx.y * M * ldc;
+  for (int k0 = kz0; k0 < kz1; k0 += 32) {
     v8u av, bv;
 #pragma unroll
     for (int t = 0; t < 8; ++t) {
       const int k = k0 + 8 * g + t;
-      av[t] = (ia < M && k < K) ? f2bf(ldf(A, (size_t)ia * ars + (size_t)k * acs)) : (unsigned short)0;
-      bv[t] = (jb < N && k < K) ? f2bf(ldf(B, (size_t)k * brs + (size_t)jb * bcs)) : (unsigned short)0;
+      av[t] = (ia < M && k < kz1) ? f2bf(ldf(A, (size_t)ia * ars + (size_t)k * acs)) : (unsigned short)0;
+      bv[t] = (jb < N && k < kz1) ? f2bf(ldf(B, (size_t)k * brs + (size_t)jb * bcs)) : (unsigned short)0;
     }
     acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(v8bf, av), __builtin_bit_cast(v8bf, bv), acc,
                                                   0, 0, 0);
@@ -245,12 +250,39 @@ __global__ __launch_bounds__(256) void k_mm(int64_t n, int p, int q, const doubl
 
 int grid1(int64_t n, int cap = 4096) { return (int)std::max<int64_t>(1, std::min<int64_t>((n + 255) / 256, cap)); }
 
+// C(i,j) = sum over s of the split-K partial tiles P[s*M*N + i*N + j], in order.
+__global__ void k_ksum(int M, int N, int ks, const float* __restrict__ P, float* __restrict__ C, int ldc) {
+  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= (int64_t)M * N) return;
+  float v = 0.f;
+  for (int z = 0; z < ks; ++z) v += P[(size_t)z * M * N + e];
+  const int i = (int)(e / N), j = (int)(e - (int64_t)i * N);
+  C[(size_t)i * ldc + j] = v;
+}
+
 template <class TA, class TB>
 void gemm(hipStream_t s, int M, int N, int K, const TA* A, int ars, int acs, const TB* B, int brs, int bcs, float* C,
-          int ldc, const float* bias, int act, const float* dmul, int ldd) {
+          int ldc, const float* bias, int act, const float* dmul, int ldd, DevBuf* kscratch = nullptr) {
   const int tiles = ((M + 15) / 16) * ((N + 15) / 16);
-  hipLaunchKernelGGL((k_gemm_bf16<TA, TB>), dim3((tiles + 3) / 4), dim3(256), 0, s, M, N, K, A, ars, acs, B, brs, bcs,
-                     C, ldc, bias, act, dmul, ldd);
+  // few output tiles over a long K (the weight gradients): split K so that the
+  // whole device works, then add the partial tiles in a fixed order
+  int ks = 1;
+  if (kscratch && !bias && !act && !dmul) {
+    const int waves = (tiles + 3) / 4 * 4;
+    ks = std::max(1, std::min({16, K / 128, (4 * 256 * 4) / std::max(1, waves)}));
+  }
+  if (ks == 1) {
+    hipLaunchKernelGGL((k_gemm_bf16<TA, TB>), dim3((tiles + 3) / 4), dim3(256), 0, s, M, N, K, A, ars, acs, B, brs,
+                       bcs, C, ldc, bias, act, dmul, ldd);
+    BCM_HIP(hipGetLastError());
+    return;
+  }
+  const size_t part = (size_t)M * N;
+  if (kscratch->bytes < part * ks * 4) kscratch->alloc(part * ks * 4);
+  float* P = kscratch->as<float>();
+  hipLaunchKernelGGL((k_gemm_bf16<TA, TB>), dim3((tiles + 3) / 4, ks), dim3(256), 0, s, M, N, K, A, ars, acs, B, brs,
+                     bcs, P, N, (const float*)nullptr, 0, (const float*)nullptr, 0);
+  hipLaunchKernelGGL(k_ksum, dim3((unsigned)((part + 255) / 256)), dim3(256), 0, s, M, N, ks, P, C, ldc);
   BCM_HIP(hipGetLastError());
 }
 
@@ -349,19 +381,19 @@ void Mlp::backward(hipStream_t s, int M, const float* x, const float* t, double*
                        loss_dev);
   // layer 3: dW3 = H2^T dY, db3 = colsum(dY), dZ2 = (dY W3^T) * (1 - H2^2)
   gemm(s, nh, nout, M, h2.as<const float>(), 1, nh, dy.as<const float>(), nout, 1, G + w_off[2], nout,
-       (const float*)nullptr, 0, (const float*)nullptr, 0);
+       (const float*)nullptr, 0, (const float*)nullptr, 0, &kpart);
   colsum(s, M, nout, dy.as<const float>(), nout, G + b_off[2], colpart.as<float>());
   gemm(s, M, nh, nout, dy.as<const float>(), nout, 1, W + wb_off[2], 1, nout, dz2.as<float>(), nh,
        (const float*)nullptr, 0, h2.as<const float>(), nh);
   // layer 2
   gemm(s, nh, nh, M, h1.as<const float>(), 1, nh, dz2.as<const float>(), nh, 1, G + w_off[1], nh,
-       (const float*)nullptr, 0, (const float*)nullptr, 0);
+       (const float*)nullptr, 0, (const float*)nullptr, 0, &kpart);
   colsum(s, M, nh, dz2.as<const float>(), nh, G + b_off[1], colpart.as<float>());
   gemm(s, M, nh, nh, dz2.as<const float>(), nh, 1, W + wb_off[1], 1, nh, dz1.as<float>(), nh, (const float*)nullptr,
        0, h1.as<const float>(), nh);
   // layer 1
   gemm(s, nin, nh, M, x, 1, nin, dz1.as<const float>(), nh, 1, G + w_off[0], nh, (const float*)nullptr, 0,
-       (const float*)nullptr, 0);
+       (const float*)nullptr, 0, &kpart);
   colsum(s, M, nh, dz1.as<const float>(), nh, G + b_off[0], colpart.as<float>());
 }
 

@@ -18,7 +18,7 @@ struct Mlp {
   int64_t nparam = 0, nweight = 0;
   int64_t w_off[3], b_off[3], wb_off[3];
   DevBuf params, grads, m1, m2, wbf, is_w, wb_index;
-  DevBuf h1, h2, y, dz1, dz2, dy, partials, colpart;
+  DevBuf h1, h2, y, dz1, dz2, dy, partials, colpart, kpart;
   int cap_batch = 0;
   int step = 0;  // Adam steps taken
   Mlp(int nin, int nh, int nout, uint64_t seed);

@@ -10,7 +10,7 @@ tau2 = 0.01) computed on the host. The first 1024 cells are solved from
 x0 = 0 (batched block PCG, FP32 factors, FP64 outer recurrence) and train
 the initial-guess model (64 POD modes per right-hand side, MLP 4 -> 256 ->
 256 -> 384 trained on the GPU with BF16 MFMA kernels). Evaluation cells
-are then solved in groups of `--group` cells (64 by default) x 6
+are then solved in groups of `--group` cells (128 by default) x 6
 right-hand sides: MLP inference,
 POD reconstruction and one batched block-PCG launch sequence per group
 (fallback to x0 = 0 for any pair whose ||r0|| >= ||b||), every pair driven
@@ -29,7 +29,8 @@ CPU baseline.
 cells, block 24, ILU2 + BiCGStab); `--workload precision` the FP64
 FGMRES(30) around FP64 / FP32 / BF16 factor sweeps on the same system;
 `--workload batch` one batched solve of `--group` cells of uneven sizes
-8^3..24^3.
+8^3..24^3; `--workload heat` the CPU-runnable 2D heat system (64 x 64
+cells, block 9, IC2 + PCG) against the reference's own Cg.
 """
 
 import argparse
@@ -51,7 +52,7 @@ HBM_PEAK_GBS = 8000.0
 EPS = 1e-10
 METRIC = "solves/sec at rel-res 1e-10: mixed+batched+learned vs plain FP64 GPU; HBM GB/s"
 CELL_M = 16
-GROUP = 64  # default cells per batched solve
+GROUP = 128  # default cells per batched solve
 TEMPLATE_SEED = 20261206
 PARAM_SEED = 20261206
 
@@ -182,7 +183,7 @@ def make_group(B, fac, idx, prec_mask):
 
 def run_learned(args, B):
     ntrain = args.train_cells
-    group = args.group
+    group = args.group or GROUP
     neval = (args.warmup + args.steps) * group
     params = draw_params(ntrain + neval, PARAM_SEED)
     threads = host_threads()
@@ -322,8 +323,8 @@ def run_learned(args, B):
 
     roof, kernels = roofline_from(prof, sweep_single_launch=tune["sweep_mode"] != 0)
     traffic_file = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles",
-                                "spmv_union_traffic_64cells_16cube_b24.json")
-    if roof["kernel"] == "spmv" and args.group == 64 and os.path.exists(traffic_file):
+                                f"spmv_union_traffic_{group}cells_16cube_b24.json")
+    if roof["kernel"] == "spmv" and os.path.exists(traffic_file):
         # measured HBM bytes per byte of algorithmic traffic of this SpMV, applied to this run's launches
         t = json.load(open(traffic_file))
         ratio = t["hbm_bytes_full_launch"] / t["algorithmic_bytes_full_launch"]
@@ -556,7 +557,7 @@ def run_batch(args, B):
     autotuned schedule) from x0 = 0, against the plain FP64 drop-in one load
     case at a time."""
     rng = np.random.default_rng(20261207)
-    ncell = args.group
+    ncell = args.group or 64
     sizes = rng.integers(8, 25, ncell)
     params = draw_params(ncell, 20261207)
     fp = B.default_params(tau1=0.1, tau2=0.01)
@@ -660,17 +661,87 @@ def run_batch(args, B):
     }), flush=True)
 
 
+def run_heat(args, B):
+    """The CPU-runnable 2D heat system (64 x 64 cells, block 9, N = 36,864,
+    IC2(1e-2, 1e-4), PCG to rel-res 1e-10 relative to ||r0||) on the GPU,
+    against the reference's own serial Cg (compiled from its sources) on the
+    same factor: same iteration count expected."""
+    import oracle as O
+
+    seed = 20261201
+    m, rhs = B.gen_heat2d(64, 64, 9, seed)
+    order = m.rcm_order()
+    mo = m.reorder(order)
+    b = mo.reorder_vector(order, rhs)
+    f = B.Factor.ic2(mo, B.default_params(tau1=1e-2, tau2=1e-4))
+    ctx = B.Context()
+    db, dx = B.DeviceArray.from_host(b), B.DeviceArray(mo.n)
+    opts = B.default_opts(kind=B.PCG, eps=EPS, ittype=0, check_every=8, niter=5000)
+    for _ in range(max(1, args.warmup)):
+        dx.zero()
+        ctx.solve(mo, f, db, dx, opts)
+    ctx.sync()
+    iters, ok = [], True
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        dx.zero()
+        st = ctx.solve(mo, f, db, dx, opts)
+        iters.append(st.iterations)
+        ok = ok and st.converged == 1
+    ctx.sync()
+    dt = time.perf_counter() - t0
+    x = dx.host()
+    ctx.profile(True)
+    ctx.profile_read(reset=True)
+    dx.zero()
+    ctx.solve(mo, f, db, dx, B.default_opts(kind=B.PCG, eps=EPS, ittype=0, check_every=1, niter=5000))
+    ctx.sync()
+    prof = ctx.profile_read()
+    ctx.profile(False)
+    roof, kernels = roofline_from(prof)
+    cpu, ref_iters, rel = None, None, None
+    if not args.no_cpu:
+        A = O.Block(*mo.arrays())
+        U = O.Block(A.sprnd, *f.arrays(1, A.nsup))
+        use_ref = O.ref_available()
+        nsolve, t_sum = 0, 0.0
+        while t_sum < 10.0 and nsolve < 200:
+            t1 = time.perf_counter()
+            if use_ref:
+                xr, ref_iters, _ = O.ref_cg(A, U, b, eps=EPS, ittype=0, niter=5000)
+            else:
+                xr, r = O.cg(A, U, b, eps=EPS, ittype=0, niter=5000)
+                ref_iters = r.updates
+            t_sum += time.perf_counter() - t1
+            nsolve += 1
+        rel = float(np.linalg.norm(x - xr) / np.linalg.norm(xr))
+        who = "the reference's serial Cg (compiled from its sources)" if use_ref else "the oracle's Cg"
+        cpu = dict(value=nsolve / t_sum, unit="solves/s", cores=1, kind="reference" if use_ref else "port",
+                   sample=f"{nsolve} solves of the same system by {who}, {ref_iters} iterations each, "
+                          f"{t_sum:.1f}s on one core of {cpu_model()}")
+    print(json.dumps({
+        "metric": METRIC, "value": args.steps / dt, "unit": "solves/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": dt / args.steps * 1e3, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f64", "data": "synthetic (seeded generator)",
+        "config": {"workload": f"2D heat, 64 x 64 cells, block 9, N={mo.n}, RCM + IC2(1e-2, 1e-4) + PCG to rel-res "
+                               f"{EPS} (relative to ||r0||)", "seed": seed},
+        "valid": bool(ok), "iterations": iters, "reference_iterations": ref_iters,
+        "rel_diff_vs_reference": rel, "kernels": kernels, "roofline": roof, "cpu_baseline": cpu,
+    }), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=6, help="timed evaluation groups (learned) or solves (drop-in)")
+    ap.add_argument("--steps", type=int, default=3, help="timed evaluation groups (learned) or solves (drop-in)")
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", choices=["learned", "drop-in", "precision", "batch"], default="learned")
+    ap.add_argument("--workload", choices=["learned", "drop-in", "precision", "batch", "heat"], default="learned")
     ap.add_argument("--train-cells", type=int, default=1024)
     ap.add_argument("--fit-steps", type=int, default=16000)
     ap.add_argument("--fit-lr", type=float, default=5e-3)
     ap.add_argument("--plain-cells", type=int, default=4)
-    ap.add_argument("--group", type=int, default=GROUP, help="cells per batched solve")
+    ap.add_argument("--group", type=int, default=None,
+                    help=f"cells per batched solve (learned: {GROUP}, batch: 64)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-graph", action="store_true", help="leave hipGraph replay out of the batch schedules")
     ap.add_argument("--tune", default=None,
@@ -691,6 +762,8 @@ def main():
         run_precision(args, B)
     elif args.workload == "batch":
         run_batch(args, B)
+    elif args.workload == "heat":
+        run_heat(args, B)
     else:
         run_dropin(args, B)
 
