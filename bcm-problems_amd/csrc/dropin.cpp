@@ -211,7 +211,8 @@ int run(bool symmetric, bool par, bcm_params* p, const Callbacks& cb) {
     bcm::SolveStats st;
     bcm::solve(mat, f, ctx, opt, db.p, dx.p, st);
     BCM_HIP(hipMemcpy(x.data(), dx.p, n * 8, hipMemcpyDeviceToHost));
-    prm.nitperf = st.iterations;
+    // serial Cg reports its loop index at exit, one past the last solution update (cg.cpp:209)
+    prm.nitperf = opt.kind == bcm::kPcg ? st.iterations + 1 : st.iterations;
     prm.resfin = st.relres_true;
     prm.timeitr = st.seconds;
     if (prm.ordtype != 0) bcm::ord_vect('I', a.sprnd, sp_new, order, x.data(), a.n(), 1);

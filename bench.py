@@ -717,7 +717,7 @@ def run_heat(args, B):
         rel = float(np.linalg.norm(x - xr) / np.linalg.norm(xr))
         who = "the reference's serial Cg (compiled from its sources)" if use_ref else "the oracle's Cg"
         cpu = dict(value=nsolve / t_sum, unit="solves/s", cores=1, kind="reference" if use_ref else "port",
-                   sample=f"{nsolve} solves of the same system by {who}, {ref_iters} iterations each, "
+                   sample=f"{nsolve} solves of the same system by {who}, nitperf {ref_iters} each, "
                           f"{t_sum:.1f}s on one core of {cpu_model()}")
     print(json.dumps({
         "metric": METRIC, "value": args.steps / dt, "unit": "solves/s", "n_gpus": 1, "steps": args.steps,
@@ -725,7 +725,8 @@ def run_heat(args, B):
         "vs_baseline": None, "dtype": "f64", "data": "synthetic (seeded generator)",
         "config": {"workload": f"2D heat, 64 x 64 cells, block 9, N={mo.n}, RCM + IC2(1e-2, 1e-4) + PCG to rel-res "
                                f"{EPS} (relative to ||r0||)", "seed": seed},
-        "valid": bool(ok), "iterations": iters, "reference_iterations": ref_iters,
+        "valid": bool(ok), "iterations": iters, "nitperf": [i + 1 for i in iters],
+        "reference_nitperf": ref_iters,
         "rel_diff_vs_reference": rel, "kernels": kernels, "roofline": roof, "cpu_baseline": cpu,
     }), flush=True)
 

@@ -139,6 +139,14 @@ def test_dropin_abstract_solvers(B, O, symmetric, ittype):
     xd = np.linalg.solve(m.dense(), rhs)
     assert rel(x, xd) < 1e-8
     assert p.nitperf > 0 and p.timefct > 0 and p.density >= 1.0
+    if symmetric:  # nitperf as the reference's Cg reports it: its loop index at exit (cg.cpp:209)
+        order = m.rcm_order()
+        mo = m.reorder(order)
+        f = B.Factor.ic2(mo, B.default_params(ittype=ittype, sttype=1, eps=1e-10, niter=500))
+        A = O.Block(*mo.arrays())
+        U = O.Block(A.sprnd, *f.arrays(1, A.nsup))
+        _, r = O.cg(A, U, mo.reorder_vector(order, rhs), eps=1e-10, ittype=ittype)
+        assert p.nitperf == r.nitperf, (p.nitperf, r.nitperf)
 
 
 @pytest.mark.parametrize("ittype,sttype,eps", [(0, 1, 1e-10), (2, 2, 1e-9), (1, 0, 1e-8)])
