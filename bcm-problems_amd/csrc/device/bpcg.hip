@@ -89,13 +89,14 @@ __global__ __launch_bounds__(256) void k_bp_dots(int64_t n, const int* __restric
   block_partials<NR * K>(acc, partials);
 }
 
-// The flexible recurrence's three dots (r.z, r.r, r_old.z) with z kept in
-// FP32 (the preconditioner's output precision; the products are formed in
-// FP64, the same values and order as with an FP64 copy of z).
+// The flexible recurrence's three dots (r.z, r.r, q.z) with z kept in FP32
+// (the preconditioner's output precision; the products are formed in FP64).
+// The Polak-Ribiere numerator (z, r - r_old) is -alpha (z, q), q = A p of the
+// update that produced r, so r_old need not be kept.
 template <int NR>
 __global__ __launch_bounds__(256) void k_bp_dots_flex(int64_t n, const int* __restrict__ lo,
                                                       const int* __restrict__ hi, const double* __restrict__ r,
-                                                      const double* __restrict__ rold, const float* __restrict__ z,
+                                                      const double* __restrict__ qv, const float* __restrict__ z,
                                                       double* partials, const int* skip) {
   if (skip && *skip) return;
   double acc[NR * 3];
@@ -109,7 +110,7 @@ __global__ __launch_bounds__(256) void k_bp_dots_flex(int64_t n, const int* __re
       const double ri = r[off + i], zi = (double)z[off + i];
       acc[q * 3 + 0] += ri * zi;
       acc[q * 3 + 1] += ri * ri;
-      acc[q * 3 + 2] += rold[off + i] * zi;
+      acc[q * 3 + 2] += qv[off + i] * zi;
     }
   }
   block_partials<NR * 3>(acc, partials);
@@ -194,7 +195,8 @@ __global__ __launch_bounds__(kPairThreads) void k_bp_beta(int npair, int nrhs, c
     }
     double beta = 0.0;
     if (!s.fresh[p]) {
-      const double num = K == 3 ? rz - chunk_sum(partials, cell_chunk0, c, q, nrhs, K, 2) : rz;
+      // flexible: (z, r - r_old) with r - r_old = -alpha q of the last update
+      const double num = K == 3 ? -s.alpha[p] * chunk_sum(partials, cell_chunk0, c, q, nrhs, K, 2) : rz;
       beta = num / s.gamma[p];
     }
     s.fresh[p] = 0;
@@ -246,14 +248,13 @@ __global__ __launch_bounds__(kPairThreads) void k_bp_alpha(int npair, int nrhs, 
   cell_activity(s, npair, nrhs);
 }
 
-// x += alpha p; (r_old = r;) r -= alpha q for active pairs; with FLEX the
-// FP32 copy of r that the preconditioner reads is written as well.
+// x += alpha p; r -= alpha q for active pairs; with FLEX the FP32 copy of r
+// that the preconditioner reads is written as well.
 template <int NR, bool FLEX>
 __global__ __launch_bounds__(256) void k_bp_xr(int64_t n, const int* __restrict__ ccell, const int* __restrict__ lo,
                                                const int* __restrict__ hi, BPtrs s, const double* __restrict__ pv,
                                                const double* __restrict__ qv, double* __restrict__ x,
-                                               double* __restrict__ r, double* __restrict__ rold,
-                                               float* __restrict__ r32) {
+                                               double* __restrict__ r, float* __restrict__ r32) {
   if (s.state[0]) return;
   const int c = ccell[blockIdx.x], a = lo[blockIdx.x], e = hi[blockIdx.x];
 #pragma unroll
@@ -264,7 +265,6 @@ __global__ __launch_bounds__(256) void k_bp_xr(int64_t n, const int* __restrict_
     const size_t off = (size_t)q * n;
     for (int i = a + threadIdx.x; i < e; i += blockDim.x) {
       const double ri = r[off + i];
-      if (FLEX) rold[off + i] = ri;
       const double rn = ri + (-alpha) * qv[off + i];
       r[off + i] = rn;
       if (FLEX) r32[off + i] = (float)rn;
@@ -314,7 +314,7 @@ int grid_vec(int64_t n) { return (int)std::max<int64_t>(1, std::min<int64_t>((n 
 
 // Work vectors of a batched solve (all n x nrhs, fp64) and the pair state.
 struct BWork {
-  double *x, *r, *z, *p, *q, *rold, *b, *partials;
+  double *x, *r, *z, *p, *q, *b, *partials;
   // profiling: algorithmic bytes per cell of the SpMV, forward and backward
   // sweep (3 x ncell, device), counted for the cells a gated launch computes
   const double* pbytes = nullptr;
@@ -337,7 +337,6 @@ BWork work(Batch& bt, Context& ctx, int nrhs, bool flex) {
   w.z = ctx.vec(42, vb);
   w.p = ctx.vec(43, vb);
   w.q = ctx.vec(44, vb);
-  w.rold = ctx.vec(45, vb);
   w.b = ctx.vec(46, vb);
   w.partials = ctx.vec(47, (size_t)bt.nchunks * nrhs * 3 * 8 + 64);
   const size_t pd = ((size_t)npair + 1) & ~size_t(1);  // keep the int arrays 8-byte aligned
@@ -401,7 +400,7 @@ void iteration(Batch& bt, Context& ctx, const BWork& w, const SolveOptions& opt,
       launch_sweep(t2, w.t32, w.z32, kF32, NR, s, done, ctx.sweep_sync(), gate);
     }
     hipLaunchKernelGGL(k_bp_dots_flex<NR>, dim3(bt.nchunks), dim3(256), 0, s, (int64_t)bt.n, bt.chunk_lo.as<int>(),
-                       bt.chunk_hi.as<int>(), w.r, w.rold, w.z32, w.partials, done);
+                       bt.chunk_hi.as<int>(), w.r, w.q, w.z32, w.partials, done);
   } else {
     double* tmp = ctx.vec(12, (size_t)bt.n * NR * 8);
     {
@@ -439,11 +438,11 @@ void iteration(Batch& bt, Context& ctx, const BWork& w, const SolveOptions& opt,
                      w.s);
   if (flex)
     hipLaunchKernelGGL((k_bp_xr<NR, true>), dim3(bt.nchunks), dim3(256), 0, s, (int64_t)bt.n, bt.chunk_cell.as<int>(),
-                       bt.chunk_lo.as<int>(), bt.chunk_hi.as<int>(), w.s, w.p, w.q, w.x, w.r, w.rold, w.r32);
+                       bt.chunk_lo.as<int>(), bt.chunk_hi.as<int>(), w.s, w.p, w.q, w.x, w.r, w.r32);
   else
     hipLaunchKernelGGL((k_bp_xr<NR, false>), dim3(bt.nchunks), dim3(256), 0, s, (int64_t)bt.n,
                        bt.chunk_cell.as<int>(), bt.chunk_lo.as<int>(), bt.chunk_hi.as<int>(), w.s, w.p, w.q, w.x,
-                       w.r, w.rold, (float*)nullptr);
+                       w.r, (float*)nullptr);
 }
 
 }  // namespace
@@ -602,7 +601,7 @@ void solve_nr(Batch& bt, Context& ctx, const SolveOptions& opt, const double* b,
     const bool use_graph = tu.graph && !ctx.prof;
     if (use_graph) {
       // every buffer the iteration touches exists before capture (no allocation inside it)
-      std::vector<const void*> key = {w.x, w.r, w.z, w.p, w.q, w.rold, w.partials, w.s.state, w.s.cact,
+      std::vector<const void*> key = {w.x, w.r, w.z, w.p, w.q, w.partials, w.s.state, w.s.cact,
                                       ctx.sync.ctl.p};
       if (opt.prec == kF64) {
         key.push_back(ctx.vec(12, (size_t)nn * 8));

@@ -70,6 +70,23 @@ __device__ __forceinline__ void load_vec(const TA* p, double (&out)[V]) {
   }
 }
 
+// The same through the cache hierarchy: the multi-RHS SpMV's matrix stream
+// measured faster with ordinary loads (the sweeps with non-temporal ones).
+template <int V, class TA>
+__device__ __forceinline__ void load_vec_cached(const TA* p, double (&out)[V]) {
+  using VT = typename VecOf<TA, V>::T;
+  const VT v = *reinterpret_cast<const VT*>(p);
+  if constexpr (V == 1) {
+    out[0] = (double)ldv(&v);
+  } else {
+#pragma unroll
+    for (int k = 0; k < V; ++k) {
+      const TA e = v[k];
+      out[k] = (double)ldv(&e);
+    }
+  }
+}
+
 template <int B, int V, int NR, class TA>
 __global__ __launch_bounds__(256) void k_spmv(int nsup, int n, const int* __restrict__ ia,
                                               const int* __restrict__ ja, const TA* __restrict__ a,
@@ -231,7 +248,7 @@ __global__ __launch_bounds__(256) void k_spmv_mr(int nsup, int n, const int* __r
 #pragma unroll 2
         for (; t < t1; t += G) {
           double av[V];
-          load_vec<V>(ap + (size_t)t * B, av);
+          load_vec_cached<V>(ap + (size_t)t * B, av);
           const double* xp = xs + (t - t0) * NRP;
           double xv[NRP];
 #pragma unroll

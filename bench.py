@@ -10,7 +10,7 @@ tau2 = 0.01) computed on the host. The first 1024 cells are solved from
 x0 = 0 (batched block PCG, FP32 factors, FP64 outer recurrence) and train
 the initial-guess model (64 POD modes per right-hand side, MLP 4 -> 256 ->
 256 -> 384 trained on the GPU with BF16 MFMA kernels). Evaluation cells
-are then solved in groups of `--group` cells (128 by default) x 6
+are then solved in groups of `--group` cells (256 by default) x 6
 right-hand sides: MLP inference,
 POD reconstruction and one batched block-PCG launch sequence per group
 (fallback to x0 = 0 for any pair whose ||r0|| >= ||b||), every pair driven
@@ -52,7 +52,7 @@ HBM_PEAK_GBS = 8000.0
 EPS = 1e-10
 METRIC = "solves/sec at rel-res 1e-10: mixed+batched+learned vs plain FP64 GPU; HBM GB/s"
 CELL_M = 16
-GROUP = 128  # default cells per batched solve
+GROUP = 256  # default cells per batched solve
 TEMPLATE_SEED = 20261206
 PARAM_SEED = 20261206
 
@@ -188,10 +188,12 @@ def run_learned(args, B):
     params = draw_params(ntrain + neval, PARAM_SEED)
     threads = host_threads()
     t_setup0 = time.perf_counter()
-    fac = CellFactory(B, params, threads, window=2 * group + threads)
+    fac = CellFactory(B, params, threads, window=group + 2 * threads)
     ctx = B.Context()
     n = 24 * CELL_M ** 3
-    opts = B.default_opts(kind=B.BLOCKPCG, eps=EPS, precision=B.F32, niter=5000)
+    fprec = B.BF16 if args.factor_precision == "bf16" else B.F32
+    fmask = 1 << fprec
+    opts = B.default_opts(kind=B.BLOCKPCG, eps=EPS, precision=fprec, niter=5000)
     guess = B.Guess(n, 6, 4, nmodes=64, nhidden=256, capacity=ntrain, seed=11)
     tune = None
     if args.tune:
@@ -218,12 +220,12 @@ def run_learned(args, B):
 
     groups = [list(range(k, min(ntrain, k + group))) for k in range(0, ntrain, group)]
     evalg = [list(range(ntrain + k * group, ntrain + (k + 1) * group)) for k in range(args.warmup + args.steps)]
-    th = threading.Thread(target=prefetch, args=(groups[0], 0b010))
+    th = threading.Thread(target=prefetch, args=(groups[0], fmask))
     th.start()
     for gi, idx in enumerate(groups):
         bt, db, cells = take()
         nxt = groups[gi + 1] if gi + 1 < len(groups) else evalg[0]
-        th = threading.Thread(target=prefetch, args=(nxt, 0b010))
+        th = threading.Thread(target=prefetch, args=(nxt, fmask))
         th.start()
         if tune is None:
             t = bt.autotune(ctx, db, opts, 6, allow_graph=not args.no_graph)
@@ -290,7 +292,7 @@ def run_learned(args, B):
                 f"{its.mean():.1f} iterations ({st.seconds * 1e3:.0f} ms)")
         fac.resume()
         if gi + 1 < len(evalg):
-            th = threading.Thread(target=prefetch, args=(evalg[gi + 1], 0b010))
+            th = threading.Thread(target=prefetch, args=(evalg[gi + 1], fmask))
             th.start()
         log(f"eval group {gi + 1}/{len(evalg)}{'' if timed else ' (warm-up)'}: {dt * 1e3:.0f} ms, mean iterations "
             f"{its.mean():.1f}, max relres {rr.max():.2e}")
@@ -347,11 +349,11 @@ def run_learned(args, B):
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64 outer / f32 factors / bf16 MLP",
+        "dtype": f"f64 outer / {args.factor_precision} factors / bf16 MLP",
         "data": "synthetic (seeded homogenization-cell generator)",
         "config": {
             "workload": f"learned initial guess: periodic {CELL_M}^3 elasticity cells (block 24, N={n}), 6 strain "
-                        f"load cases each, IC2(0.1, 0.01) factors in FP32, batched block PCG in groups of {group} "
+                        f"load cases each, IC2(0.1, 0.01) factors in {args.factor_precision.upper()}, batched block PCG in groups of {group} "
                         f"cells to FP64 true rel-res {EPS}; {ntrain} training cells, {args.steps} timed groups of "
                         f"{group} evaluation cells",
             "train_cells": ntrain,
@@ -741,6 +743,8 @@ def main():
     ap.add_argument("--fit-steps", type=int, default=16000)
     ap.add_argument("--fit-lr", type=float, default=5e-3)
     ap.add_argument("--plain-cells", type=int, default=4)
+    ap.add_argument("--factor-precision", choices=["f32", "bf16"], default="f32",
+                    help="storage of the batched solves' IC2 factors (the outer recurrence stays FP64)")
     ap.add_argument("--group", type=int, default=None,
                     help=f"cells per batched solve (learned: {GROUP}, batch: 64)")
     ap.add_argument("--no-cpu", action="store_true")
