@@ -55,17 +55,17 @@ template <> struct VecOf<uint16_t, 1> { using T = uint16_t; };
 template <> struct VecOf<uint16_t, 2> { typedef uint16_t T __attribute__((ext_vector_type(2))); };
 template <> struct VecOf<uint16_t, 4> { typedef uint16_t T __attribute__((ext_vector_type(4))); };
 
-template <int V, class TA>
-__device__ __forceinline__ void load_vec(const TA* p, double (&out)[V]) {
+template <int V, class TA, class TO>
+__device__ __forceinline__ void load_vec(const TA* p, TO (&out)[V]) {
   using VT = typename VecOf<TA, V>::T;
   const VT v = __builtin_nontemporal_load(reinterpret_cast<const VT*>(p));
   if constexpr (V == 1) {
-    out[0] = (double)ldv(&v);
+    out[0] = (TO)ldv(&v);
   } else {
 #pragma unroll
     for (int k = 0; k < V; ++k) {
       const TA e = v[k];
-      out[k] = (double)ldv(&e);
+      out[k] = (TO)ldv(&e);
     }
   }
 }
@@ -156,13 +156,21 @@ __global__ __launch_bounds__(256) void k_spmv(int nsup, int n, const int* __rest
 }
 
 // Stages the vector values of nb block columns (cols[0..nb)) for NR
-// right-hand sides into this wave's LDS area: xs[(kb*B + c)*NRP + q]. Each
-// lane issues U loads before their LDS stores (a chunk of up to 64*U values
-// costs one memory round trip; a small U keeps short-lived waves light).
-// Brackets itself with wave-level LDS ordering.
+// right-hand sides into this wave's LDS area: xs[(kb*B + c)*NRP + q]. For
+// chunks of up to 16 block columns the indices are wave-uniform and read
+// first (scalar loads), so no value load waits on an index load of its own:
+// each lane issues U value loads back to back before their LDS stores (a
+// chunk of up to 64*U values costs one memory round trip). Brackets itself
+// with wave-level LDS ordering.
 template <int B, int NR, int NRP, int KB, int U, class TV, class TS>
 __device__ __forceinline__ void stage_chunk(int lane, int nb, const int* __restrict__ cols, const TV* __restrict__ v,
                                             int n, TS* xs) {
+  constexpr bool kScalarIdx = KB <= 16;
+  int cidx[kScalarIdx ? KB : 1];
+  if constexpr (kScalarIdx) {
+#pragma unroll
+    for (int k = 0; k < KB; ++k) cidx[k] = cols[k < nb ? k : 0];
+  }
   const int cnt = nb * NR * B;
   wave_lds_sync();  // earlier reads of the area are done
   for (int base = 0; base < cnt; base += 64 * U) {
@@ -173,7 +181,15 @@ __device__ __forceinline__ void stage_chunk(int lane, int nb, const int* __restr
       if (e < cnt) {
         const int c = e % B, rest = e / B;
         const int q = rest % NR, kb = rest / NR;
-        val[i] = v[(size_t)q * n + (size_t)cols[kb] * B + c];
+        int cj;
+        if constexpr (kScalarIdx) {
+          cj = cidx[0];
+#pragma unroll
+          for (int k = 1; k < KB; ++k) cj = kb == k ? cidx[k] : cj;
+        } else {
+          cj = cols[kb];
+        }
+        val[i] = v[(size_t)q * n + (size_t)cj * B + c];
       }
     }
 #pragma unroll
@@ -183,6 +199,48 @@ __device__ __forceinline__ void stage_chunk(int lane, int nb, const int* __restr
         const int c = e % B, rest = e / B;
         const int q = rest % NR, kb = rest / NR;
         xs[(kb * B + c) * NRP + q] = (TS)val[i];
+      }
+    }
+  }
+  wave_lds_sync();
+}
+
+// The same staging with lanes laid over the NR*B values of one block column
+// (lane slot m = q*B + c): the per-lane offsets are computed once, a block
+// column's base is a scalar, and the values of KG block columns are in
+// flight together. Used where a block column holds at least a wave's worth
+// of values.
+template <int B, int NR, int NRP, int KG, class TV, class TS>
+__device__ __forceinline__ void stage_cols(int lane, int nb, const int* __restrict__ cols, const TV* __restrict__ v,
+                                           int n, TS* xs) {
+  constexpr int M = NR * B;
+  constexpr int S = (M + 63) / 64;
+  size_t goff[S];
+  int loff[S];
+#pragma unroll
+  for (int s = 0; s < S; ++s) {
+    const int m = lane + 64 * s, q = m / B, c = m - q * B;
+    goff[s] = (size_t)q * n + c;
+    loff[s] = c * NRP + q;
+  }
+  wave_lds_sync();  // earlier reads of the area are done
+  for (int kb0 = 0; kb0 < nb; kb0 += KG) {
+    TV val[KG][S];
+#pragma unroll
+    for (int k = 0; k < KG; ++k) {
+      if (kb0 + k < nb) {
+        const TV* vb = v + (size_t)cols[kb0 + k] * B;
+#pragma unroll
+        for (int s = 0; s < S; ++s)
+          if (s + 1 < S || lane + 64 * s < M) val[k][s] = vb[goff[s]];
+      }
+    }
+#pragma unroll
+    for (int k = 0; k < KG; ++k) {
+      if (kb0 + k < nb) {
+#pragma unroll
+        for (int s = 0; s < S; ++s)
+          if (s + 1 < S || lane + 64 * s < M) xs[(kb0 + k) * B * NRP + loff[s]] = (TS)val[k][s];
       }
     }
   }
@@ -220,6 +278,7 @@ __global__ __launch_bounds__(256) void k_spmv_mr(int nsup, int n, const int* __r
   constexpr int WL = WL0 > G * B ? WL0 : G * B;                      // doubles per wave
   constexpr int QF = WL / (G * B) < NR ? WL / (G * B) : NR;          // right-hand sides per fold pass
   static_assert(QF >= 1, "the fold needs room for one right-hand side");
+  constexpr int D = 2;                                               // matrix columns in flight per lane
   __shared__ __align__(16) double lds[4][WL];
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int lr = lane % L, g = lane / L;
@@ -242,22 +301,42 @@ __global__ __launch_bounds__(256) void k_spmv_mr(int nsup, int n, const int* __r
     int t = g;  // this lane group's next column of the row's stream
     for (int kb0 = 0; kb0 < nblk; kb0 += KB) {
       const int nb = min(KB, nblk - kb0);
-      stage_chunk<B, NR, NRP, KB, (KB * NR * B + 63) / 64>(lane, nb, ja + jb + kb0, x, n, xs);
+      const int t0 = kb0 * B, t1 = (kb0 + nb) * B;
+      // The lane's first D matrix columns of the chunk depend only on the row
+      // pointer, so they are in flight while the x values are staged; each
+      // column consumed then issues the load of the column D steps ahead.
+      double av[D][V];
       if (act) {
-        const int t0 = kb0 * B, t1 = (kb0 + nb) * B;
-#pragma unroll 2
-        for (; t < t1; t += G) {
-          double av[V];
-          load_vec_cached<V>(ap + (size_t)t * B, av);
-          const double* xp = xs + (t - t0) * NRP;
-          double xv[NRP];
 #pragma unroll
-          for (int q = 0; q < NRP; ++q) xv[q] = xp[q];
+        for (int d = 0; d < D; ++d)
+          if (t + d * G < t1) load_vec_cached<V>(ap + (size_t)(t + d * G) * B, av[d]);
+      }
+      if constexpr (NR * B >= 64)
+        stage_cols<B, NR, NRP, KB>(lane, nb, ja + jb + kb0, x, n, xs);
+      else
+        stage_chunk<B, NR, NRP, KB, (KB * NR * B + 63) / 64>(lane, nb, ja + jb + kb0, x, n, xs);
+      if (act) {
+        for (int tc = t; tc < t1; tc += D * G) {
 #pragma unroll
-          for (int q = 0; q < NR; ++q)
+          for (int d = 0; d < D; ++d) {
+            const int tt = tc + d * G;
+            if (tt < t1) {
+              double cur[V];
 #pragma unroll
-            for (int k = 0; k < V; ++k) acc[q][k] += av[k] * xv[q];
+              for (int k = 0; k < V; ++k) cur[k] = av[d][k];
+              if (tt + D * G < t1) load_vec_cached<V>(ap + (size_t)(tt + D * G) * B, av[d]);
+              const double* xp = xs + (tt - t0) * NRP;
+              double xv[NRP];
+#pragma unroll
+              for (int q = 0; q < NRP; ++q) xv[q] = xp[q];
+#pragma unroll
+              for (int q = 0; q < NR; ++q)
+#pragma unroll
+                for (int k = 0; k < V; ++k) acc[q][k] += cur[k] * xv[q];
+            }
+          }
         }
+        if (t < t1) t += (t1 - t + G - 1) / G * G;
       }
     }
 #pragma unroll
@@ -295,9 +374,11 @@ __global__ __launch_bounds__(256) void k_spmv_mr(int nsup, int n, const int* __r
 // row of a level depends only on rows of earlier levels, which earlier
 // launches on the same stream have completed, so no wave ever waits on
 // another. One wavefront per block row. A level's waves start together and
-// each does little work, so the row's right-hand side values and inverted
-// diagonal block are loaded as soon as the row is known, overlapping the
-// dependency staging instead of following it.
+// each does little work, so a level costs about one wave's chain of
+// dependent memory round trips: the row's right-hand side values and
+// inverted diagonal block are loaded as soon as the row is known, the first
+// factor columns while the dependencies are staged, and the staging itself
+// takes one round trip for up to SKG block columns.
 template <int B, int V, int NR, class TA, class TX>
 __global__ __launch_bounds__(256) void k_sweep_level(int nrows, const int* __restrict__ rows, int n,
                                                      const int* __restrict__ ia, const int* __restrict__ ja,
@@ -313,6 +394,7 @@ __global__ __launch_bounds__(256) void k_sweep_level(int nrows, const int* __res
   constexpr int NRP = NR % 2 ? NR + 1 : NR;
   constexpr int KB = (G * NR * B + B * NRP - 1) / (B * NRP);
   constexpr int WL = KB * B * NRP;
+  constexpr int SD = 2, SKG = 4;  // factor columns in flight per lane; block columns staged per round trip
   __shared__ __align__(16) Acc lds[4][WL + NR * B];
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int lr = lane % L, g = lane / L;
@@ -348,18 +430,38 @@ __global__ __launch_bounds__(256) void k_sweep_level(int nrows, const int* __res
   int t = g;
   for (int kb0 = 0; kb0 < nblk; kb0 += KB) {
     const int nb = min(KB, nblk - kb0);
-    stage_chunk<B, NR, NRP, KB, 1>(lane, nb, ja + jb + kb0, y, n, xs);
+    const int t0 = kb0 * B, t1 = (kb0 + nb) * B;
+    // The lane's first SD factor columns of the chunk are in flight while the
+    // y values are staged; each column consumed issues the load SD ahead.
+    Acc av[SD][V];
     if (act) {
-      const int t0 = kb0 * B, t1 = (kb0 + nb) * B;
-      for (; t < t1; t += G) {
-        double av[V];
-        load_vec<V>(ap + (size_t)t * B, av);
-        const Acc* xp = xs + (t - t0) * NRP;
 #pragma unroll
-        for (int q = 0; q < NR; ++q)
+      for (int d = 0; d < SD; ++d)
+        if (t + d * G < t1) load_vec<V>(ap + (size_t)(t + d * G) * B, av[d]);
+    }
+    if constexpr (NR * B >= 64)
+      stage_cols<B, NR, NRP, SKG>(lane, nb, ja + jb + kb0, y, n, xs);
+    else
+      stage_chunk<B, NR, NRP, KB, 1>(lane, nb, ja + jb + kb0, y, n, xs);
+    if (act) {
+      for (int tc = t; tc < t1; tc += SD * G) {
 #pragma unroll
-          for (int k = 0; k < V; ++k) acc[q][k] += (Acc)av[k] * xp[q];
+        for (int d = 0; d < SD; ++d) {
+          const int tt = tc + d * G;
+          if (tt < t1) {
+            Acc cur[V];
+#pragma unroll
+            for (int k = 0; k < V; ++k) cur[k] = av[d][k];
+            if (tt + SD * G < t1) load_vec<V>(ap + (size_t)(tt + SD * G) * B, av[d]);
+            const Acc* xp = xs + (tt - t0) * NRP;
+#pragma unroll
+            for (int q = 0; q < NR; ++q)
+#pragma unroll
+              for (int k = 0; k < V; ++k) acc[q][k] += cur[k] * xp[q];
+          }
+        }
       }
+      if (t < t1) t += (t1 - t + G - 1) / G * G;
     }
   }
   wave_lds_sync();
