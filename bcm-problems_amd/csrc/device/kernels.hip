@@ -442,7 +442,7 @@ __global__ __launch_bounds__(256) void k_sweep_level(int nrows, const int* __res
     if constexpr (NR * B >= 64)
       stage_cols<B, NR, NRP, SKG>(lane, nb, ja + jb + kb0, y, n, xs);
     else
-      stage_chunk<B, NR, NRP, KB, 1>(lane, nb, ja + jb + kb0, y, n, xs);
+      stage_chunk<B, NR, NRP, KB, (KB * NR * B + 63) / 64>(lane, nb, ja + jb + kb0, y, n, xs);
     if (act) {
       for (int tc = t; tc < t1; tc += SD * G) {
 #pragma unroll

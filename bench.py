@@ -349,7 +349,7 @@ def run_learned(args, B):
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": f"f64 outer / {args.factor_precision} factors / bf16 MLP",
+        "dtype": "f64", "mixed_precision": f"{args.factor_precision} factor sweeps inside the f64 recurrence; bf16 MLP",
         "data": "synthetic (seeded homogenization-cell generator)",
         "config": {
             "workload": f"learned initial guess: periodic {CELL_M}^3 elasticity cells (block 24, N={n}), 6 strain "
@@ -543,7 +543,7 @@ def run_precision(args, B):
     print(json.dumps({
         "metric": METRIC, "value": res["f32"]["solves_per_s"], "unit": "solves/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": res["f32"]["ms_per_solve"], "higher_is_better": True,
-        "scaling": "weak", "vs_baseline": None, "dtype": "f64 outer / f32 factors",
+        "scaling": "weak", "vs_baseline": None, "dtype": "f64", "mixed_precision": "f32 factor sweeps",
         "data": "synthetic (seeded generator)",
         "config": {"workload": f"mixed precision: FP64 FGMRES(30) with ILU2 factors stored and applied in FP32 "
                                f"(value) / BF16 / FP64, 3D nonsymmetric 32^3 cells, block 24, N={mo.n}, to true "
@@ -644,7 +644,7 @@ def run_batch(args, B):
     print(json.dumps({
         "metric": METRIC, "value": value, "unit": "solves/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": timed / args.steps * 1e3, "higher_is_better": True,
-        "scaling": "weak", "vs_baseline": None, "dtype": "f64 outer / f32 factors",
+        "scaling": "weak", "vs_baseline": None, "dtype": "f64", "mixed_precision": "f32 factor sweeps",
         "data": "synthetic (seeded homogenization-cell generator)",
         "config": {"workload": f"batching: {ncell} periodic elasticity cells of uneven sizes "
                                f"{int(sizes.min())}^3..{int(sizes.max())}^3 (block 24, N={bt.n} in total), 6 load "
