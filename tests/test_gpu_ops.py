@@ -56,6 +56,64 @@ def test_spmv_symmetric_and_residual(B, O, ctx):
     assert rel(dr.host(), h["b"] - ax) < 1e-13
 
 
+def ragged_matrix(B, O, nsup, bla, maxblk, seed):
+    """Random nonsymmetric block matrix whose rows hold 1..maxblk blocks (the
+    diagonal always), so rows span several staging chunks of the products
+    and sweeps, and some hold the diagonal block alone."""
+    rng = np.random.default_rng(seed)
+    ia, ja = [0], []
+    for i in range(nsup):
+        k = 1 if i % 7 == 3 else int(rng.integers(1, maxblk + 1))
+        cols = set(rng.choice(nsup, size=min(k, nsup), replace=False).tolist()) | {i}
+        ja.extend(sorted(cols))
+        ia.append(len(ja))
+    a = rng.standard_normal(len(ja) * bla * bla) / np.sqrt(bla * maxblk)
+    off = 0
+    for i in range(nsup):  # dominant diagonal blocks keep the factorization stable
+        for j in range(ia[i], ia[i + 1]):
+            if ja[j] == i:
+                a[off:off + bla * bla] += 4.0 * np.eye(bla).reshape(-1)
+            off += bla * bla
+    m = B.Matrix.from_arrays(np.arange(nsup + 1) * bla, ia, ja, a, symmetric=False)
+    return m, O.Block(*m.arrays())
+
+
+@pytest.mark.parametrize("bla", [12, 16, 24])
+@pytest.mark.parametrize("nrhs", [3, 6])
+def test_spmv_long_ragged_rows(B, O, ctx, bla, nrhs):
+    """Rows of up to 22 blocks: several LDS staging chunks per row, rows
+    shorter than the matrix prefetch, a diagonal-only row."""
+    m, A = ragged_matrix(B, O, 48, bla, 22, seed=bla + nrhs)
+    print(f"ragged spmv bla={bla} nrhs={nrhs} n={A.n}", flush=True)
+    x = np.random.default_rng(bla).standard_normal((A.n, nrhs))
+    dx, dy = dev(B, x), B.DeviceArray((A.n, nrhs))
+    ctx.mvm(m, dx, dy, nrhs)
+    ctx.sync()
+    assert rel(dy.host(), O.mvma(A, x, nrhs)) < 1e-13
+    ctx.mvm_t(m, dx, dy, nrhs)
+    ctx.sync()
+    assert rel(dy.host(), O.mvmat(A, x, nrhs)) < 1e-13
+
+
+@pytest.mark.parametrize("nrhs", [1, 6])
+def test_sweeps_long_ragged_rows(B, O, ctx, nrhs):
+    """ILU2 factors of a matrix with rows of up to 16 blocks: factor rows that
+    span several dependency-staging rounds of the level sweep."""
+    m, A = ragged_matrix(B, O, 40, 24, 16, seed=31 + nrhs)
+    f = B.Factor.ilu2(m)
+    L = O.Block(A.sprnd, *f.arrays(0, A.nsup))
+    U = O.Block(A.sprnd, *f.arrays(1, A.nsup))
+    print(f"ragged sweeps nrhs={nrhs} n={A.n}", flush=True)
+    x = np.random.default_rng(12).standard_normal((A.n, nrhs))
+    dx, dy, dz = dev(B, x), B.DeviceArray((A.n, nrhs)), B.DeviceArray((A.n, nrhs))
+    ctx.solve_l(f, 0, dx, dy, nrhs)
+    ctx.solve_u(f, 1, dy, dz, nrhs)
+    ctx.sync()
+    y = O.solvel(L, x, nrhs)
+    assert rel(dy.host(), y) < 1e-12
+    assert rel(dz.host(), O.solveu(U, y, nrhs)) < 1e-12
+
+
 @pytest.mark.parametrize("nrhs", [1, 6])
 def test_sweeps_ic2(B, O, ctx, nrhs):
     h = heat(B, O, 9, 8, 9, seed=5)
