@@ -8,8 +8,8 @@ spherical inclusion with radius U[0.1, 0.45], stiffness contrast
 macroscopic-strain right-hand sides each, IC2 factors (tau1 = 0.1,
 tau2 = 0.01) computed on the host. The first 1024 cells are solved from
 x0 = 0 (batched block PCG, FP32 factors, FP64 outer recurrence) and train
-the initial-guess model (64 POD modes per right-hand side, MLP 4 -> 256 ->
-256 -> 384 trained on the GPU with BF16 MFMA kernels). Evaluation cells
+the initial-guess model (128 POD modes per right-hand side, MLP 4 -> 256 ->
+256 -> 768 trained on the GPU with BF16 MFMA kernels). Evaluation cells
 are then solved in groups of `--group` cells (256 by default) x 6
 right-hand sides: MLP inference,
 POD reconstruction and one batched block-PCG launch sequence per group
@@ -194,7 +194,7 @@ def run_learned(args, B):
     fprec = B.BF16 if args.factor_precision == "bf16" else B.F32
     fmask = 1 << fprec
     opts = B.default_opts(kind=B.BLOCKPCG, eps=EPS, precision=fprec, niter=5000)
-    guess = B.Guess(n, 6, 4, nmodes=64, nhidden=256, capacity=ntrain, seed=11)
+    guess = B.Guess(n, 6, 4, nmodes=args.pod_modes, nhidden=args.mlp_hidden, capacity=ntrain, seed=11)
     tune = None
     if args.tune:
         keys = ("sweep_mode", "waves_per_cu", "level_order", "graph", "check_every")
@@ -742,6 +742,8 @@ def main():
     ap.add_argument("--train-cells", type=int, default=1024)
     ap.add_argument("--fit-steps", type=int, default=16000)
     ap.add_argument("--fit-lr", type=float, default=5e-3)
+    ap.add_argument("--pod-modes", type=int, default=128, help="POD modes per load case of the initial-guess model")
+    ap.add_argument("--mlp-hidden", type=int, default=256, help="width of the guess MLP's two hidden layers")
     ap.add_argument("--plain-cells", type=int, default=4)
     ap.add_argument("--factor-precision", choices=["f32", "bf16"], default="f32",
                     help="storage of the batched solves' IC2 factors (the outer recurrence stays FP64)")
