@@ -72,17 +72,17 @@ __device__ __forceinline__ void load_vec(const TA* p, TO (&out)[V]) {
 
 // The same through the cache hierarchy: the multi-RHS SpMV's matrix stream
 // measured faster with ordinary loads (the sweeps with non-temporal ones).
-template <int V, class TA>
-__device__ __forceinline__ void load_vec_cached(const TA* p, double (&out)[V]) {
+template <int V, class TA, class TO>
+__device__ __forceinline__ void load_vec_cached(const TA* p, TO (&out)[V]) {
   using VT = typename VecOf<TA, V>::T;
   const VT v = *reinterpret_cast<const VT*>(p);
   if constexpr (V == 1) {
-    out[0] = (double)ldv(&v);
+    out[0] = (TO)ldv(&v);
   } else {
 #pragma unroll
     for (int k = 0; k < V; ++k) {
       const TA e = v[k];
-      out[k] = (double)ldv(&e);
+      out[k] = (TO)ldv(&e);
     }
   }
 }
