@@ -168,6 +168,15 @@ class CellFactory:
         self.pool.shutdown(wait=False, cancel_futures=True)
 
 
+def parse_tune(text):
+    """Batch schedule from 'sweep_mode,waves_per_cu,level_order,graph,check_every'."""
+    keys = ("sweep_mode", "waves_per_cu", "level_order", "graph", "check_every")
+    vals = [int(v) for v in text.split(",")]
+    if len(vals) != len(keys):
+        raise SystemExit("--tune takes five comma-separated integers")
+    return dict(zip(keys, vals))
+
+
 def draw_params(k, seed):
     rng = np.random.default_rng(seed)
     return np.stack([rng.uniform(0.1, 0.45, k), rng.uniform(0.0, 3.0, k), rng.uniform(0.2, 0.45, k),
@@ -195,11 +204,7 @@ def run_learned(args, B):
     fmask = 1 << fprec
     opts = B.default_opts(kind=B.BLOCKPCG, eps=EPS, precision=fprec, niter=5000)
     guess = B.Guess(n, 6, 4, nmodes=args.pod_modes, nhidden=args.mlp_hidden, capacity=ntrain, seed=11)
-    tune = None
-    if args.tune:
-        keys = ("sweep_mode", "waves_per_cu", "level_order", "graph", "check_every")
-        tune = dict(zip(keys, (int(v) for v in args.tune.split(","))))
-        assert len(tune) == len(keys), "--tune takes five comma-separated integers"
+    tune = parse_tune(args.tune) if args.tune else None
     train_pairs, train_s, train_iters = 0, 0.0, []
     # prefetch the next group's device images while the current one solves
     next_group = [None]
@@ -585,10 +590,14 @@ def run_batch(args, B):
     db = B.DeviceArray.from_host(b)
     ctx = B.Context()
     opts = B.default_opts(kind=B.BLOCKPCG, eps=EPS, precision=B.F32, niter=5000)
-    t = bt.autotune(ctx, db, opts, 6, allow_graph=not args.no_graph)
-    tune = dict(sweep_mode=t.sweep_mode, waves_per_cu=t.waves_per_cu, level_order=t.level_order, graph=t.graph,
-                check_every=t.check_every)
-    log(f"autotuned schedule {tune} ({t.ms_per_iter:.2f} ms/iteration)")
+    if args.tune:
+        tune = parse_tune(args.tune)
+        bt.set_tune(**tune)
+    else:
+        t = bt.autotune(ctx, db, opts, 6, allow_graph=not args.no_graph)
+        tune = dict(sweep_mode=t.sweep_mode, waves_per_cu=t.waves_per_cu, level_order=t.level_order, graph=t.graph,
+                    check_every=t.check_every)
+        log(f"autotuned schedule {tune} ({t.ms_per_iter:.2f} ms/iteration)")
     dx = B.DeviceArray(db.shape)
     for _ in range(max(1, args.warmup)):
         dx.zero()
