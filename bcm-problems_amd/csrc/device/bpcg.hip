@@ -470,21 +470,23 @@ void batch_build(Batch& bt, const std::vector<const Matrix*>& mats, const std::v
   if (mats.empty() || mats.size() != facts.size()) throw Error(kErrArg, "batch: one factor per matrix");
   bt.ncell = (int)mats.size();
   bt.kind = facts[0]->kind;
-  std::vector<BlockCsr> full;
-  full.reserve(mats.size());
-  std::vector<const BlockCsr*> parts, lower, upper;
+  std::vector<BlockCsr> full(mats.size());
+  std::vector<const BlockCsr*> parts(mats.size()), lower, upper;
   for (size_t c = 0; c < mats.size(); ++c) {
     if (facts[c]->kind != bt.kind) throw Error(kErrArg, "batch: mixed IC2 and ILU2 factors");
     if (facts[c]->u.n() != mats[c]->h.n()) throw Error(kErrArg, "batch: matrix and factor sizes differ");
-    if (mats[c]->symmetric) {
-      full.push_back(symm_mtr(mats[c]->h));
-      parts.push_back(&full.back());
-    } else {
-      parts.push_back(&mats[c]->h);
-    }
     lower.push_back(&facts[c]->lower());
     upper.push_back(&facts[c]->u);
   }
+  // upper-stored (symmetric) cells are expanded to full storage, one cell per host thread
+  parallel_for(bt.ncell, host_threads(), [&](int c) {
+    if (mats[c]->symmetric) {
+      full[c] = symm_mtr(mats[c]->h);
+      parts[c] = &full[c];
+    } else {
+      parts[c] = &mats[c]->h;
+    }
+  });
   dev_bsr_upload_parts(parts, kF64, bt.A);
   auto offdiag = [](const BlockCsr& m) {
     int64_t k = 0;
@@ -502,7 +504,7 @@ void batch_build(Batch& bt, const std::vector<const Matrix*>& mats, const std::v
     bt.cell_fdep[c] = offdiag(*lower[c]);
     bt.cell_bdep[c] = offdiag(*upper[c]);
   }
-  full.clear();
+  full = std::vector<BlockCsr>();
   bt.b = bt.A.b;
   bt.n = bt.A.n;
   bt.elem0.assign(bt.ncell + 1, 0);

@@ -16,6 +16,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <functional>
 #include <string>
 #include <vector>
 
@@ -139,6 +140,12 @@ struct RowGate {
   const int* row_cell = nullptr;
   const int* active = nullptr;
 };
+
+// Runs body(i) for i in [0, n) on up to `threads` host threads (each with
+// the caller's HIP device current); the first exception is rethrown after
+// every thread has joined.
+void parallel_for(int n, int threads, const std::function<void(int)>& body);
+int host_threads();
 
 void launch_spmv(const DevBsr& A, const double* x, double* y, const double* bsub, int nrhs,
                  double* dot_out, const double* wdot, RedWork& rw, hipStream_t s, const int* skip = nullptr,

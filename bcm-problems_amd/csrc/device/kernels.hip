@@ -21,7 +21,11 @@
 #include "dev_common.hpp"
 
 #include <algorithm>
+#include <atomic>
 #include <cstring>
+#include <exception>
+#include <mutex>
+#include <thread>
 
 namespace bcm {
 
@@ -784,6 +788,39 @@ int device_cu_count() {
   return cus;
 }
 
+int host_threads() {
+  const unsigned h = std::thread::hardware_concurrency();
+  return (int)std::max(1u, std::min(16u, h ? h : 1u));
+}
+
+void parallel_for(int n, int threads, const std::function<void(int)>& body) {
+  const int nt = std::max(1, std::min(n, threads));
+  if (nt <= 1) {
+    for (int i = 0; i < n; ++i) body(i);
+    return;
+  }
+  int dev = 0;
+  BCM_HIP(hipGetDevice(&dev));
+  std::atomic<int> next{0};
+  std::exception_ptr err;
+  std::mutex mu;
+  std::vector<std::thread> pool;
+  pool.reserve(nt);
+  for (int t = 0; t < nt; ++t)
+    pool.emplace_back([&] {
+      try {
+        BCM_HIP(hipSetDevice(dev));
+        for (int i = next++; i < n; i = next++) body(i);
+      } catch (...) {
+        std::lock_guard<std::mutex> g(mu);
+        if (!err) err = std::current_exception();
+        next = n;  // the other threads stop at their next item
+      }
+    });
+  for (auto& th : pool) th.join();
+  if (err) std::rethrow_exception(err);
+}
+
 static uint16_t to_bf16(float f) {
   uint32_t u;
   std::memcpy(&u, &f, 4);
@@ -839,14 +876,20 @@ void dev_bsr_upload_parts(const std::vector<const BlockCsr*>& parts, int prec, D
   const size_t bb = (size_t)b * b, es = prec_bytes(prec);
   std::vector<int> ia(nsup + 1, 0), ja(std::max<int64_t>(nzja, 1), 0);
   d.a.alloc(std::max<size_t>(nzja * bb * es, es));
-  int64_t r0 = 0, j0 = 0;
-  for (const BlockCsr* p : parts) {
-    for (int i = 0; i < p->nsup; ++i) ia[r0 + i + 1] = (int)(j0 + p->ia[i + 1]);
-    for (int j = 0; j < p->nzja(); ++j) ja[j0 + j] = (int)(r0 + p->ja[j]);
-    upload_values_at(p->a.data(), p->a.size(), prec, static_cast<char*>(d.a.p) + (size_t)j0 * bb * es);
-    r0 += p->nsup;
-    j0 += p->nzja();
+  const int np = (int)parts.size();
+  std::vector<int64_t> r0(np + 1, 0), j0(np + 1, 0);  // first row and block of each part
+  for (int q = 0; q < np; ++q) {
+    r0[q + 1] = r0[q] + parts[q]->nsup;
+    j0[q + 1] = j0[q] + parts[q]->nzja();
   }
+  // parts are independent: index shifts, value conversion and the copy to the
+  // device run on host threads
+  parallel_for(np, host_threads(), [&](int q) {
+    const BlockCsr* p = parts[q];
+    for (int i = 0; i < p->nsup; ++i) ia[r0[q] + i + 1] = (int)(j0[q] + p->ia[i + 1]);
+    for (int j = 0; j < p->nzja(); ++j) ja[j0[q] + j] = (int)(r0[q] + p->ja[j]);
+    upload_values_at(p->a.data(), p->a.size(), prec, static_cast<char*>(d.a.p) + (size_t)j0[q] * bb * es);
+  });
   d.ia.upload(ia.data(), ia.size() * 4);
   d.ja.upload(ja.data(), ja.size() * 4);
 }
@@ -874,10 +917,19 @@ static void build_tri_parts(const std::vector<const BlockCsr*>& parts, bool forw
   std::vector<int> ia_all(nsup + 1, 0), ja_all(std::max<int64_t>(ndep, 1), 0), lev_all(nsup, 0);
   t.off.alloc(std::max<size_t>(ndep * bb * es, es));
   t.diag.alloc(nsup * bb * es);
-  int64_t r0 = 0, d0 = 0;
-  int mx = 0;
-  for (const BlockCsr* fp : parts) {
-    const BlockCsr& f = *fp;
+  const int np = (int)parts.size();
+  std::vector<int64_t> rs(np + 1, 0), ds(np + 1, 0);  // first row and dependency block of each part
+  for (int q = 0; q < np; ++q) {
+    rs[q + 1] = rs[q] + parts[q]->nsup;
+    ds[q + 1] = ds[q] + parts[q]->nzja() - parts[q]->nsup;
+  }
+  std::vector<int> part_mx(np, 0);
+  // parts are independent (a row depends on rows of its own part only): the
+  // transposition, levels, conversion and copy of each run on host threads
+  parallel_for(np, host_threads(), [&](int q) {
+    const BlockCsr& f = *parts[q];
+    const int64_t r0 = rs[q], d0 = ds[q];
+    int mx = 0;
     const int ns = f.nsup;
     std::vector<int> ia(ns + 1, 0), ja;
     std::vector<double> off, diag((size_t)ns * bb);
@@ -923,26 +975,25 @@ static void build_tri_parts(const std::vector<const BlockCsr*>& parts, bool forw
       lev_all[r0 + row] = l;
       mx = std::max(mx, l);
     }
+    if ((int64_t)ja.size() != ds[q + 1] - d0) throw Error(kErrInternal, "factor image: dependency count");
     for (int i = 0; i < ns; ++i) ia_all[r0 + i + 1] = (int)(d0 + ia[i + 1]);
     for (size_t j = 0; j < ja.size(); ++j) ja_all[d0 + j] = (int)(r0 + ja[j]);
     upload_values_at(off.data(), off.size(), prec, static_cast<char*>(t.off.p) + (size_t)d0 * bb * es);
     upload_values_at(diag.data(), diag.size(), prec, static_cast<char*>(t.diag.p) + (size_t)r0 * bb * es);
-    r0 += ns;
-    d0 += (int64_t)ja.size();
-  }
+    part_mx[q] = mx;
+  });
+  const int mx = np ? *std::max_element(part_mx.begin(), part_mx.end()) : 0;
   t.levels = nsup ? mx + 1 : 0;
   t.level_ptr.assign(t.levels + 1, 0);
   for (int64_t i = 0; i < nsup; ++i) t.level_ptr[lev_all[i] + 1]++;
   for (int l = 0; l < t.levels; ++l) t.level_ptr[l + 1] += t.level_ptr[l];
   std::vector<int> rows(nsup), fill(t.level_ptr.begin(), t.level_ptr.end() - 1);
-  r0 = 0;
-  for (const BlockCsr* fp : parts) {
-    const int ns = fp->nsup;
+  for (int q = 0; q < np; ++q) {
+    const int ns = parts[q]->nsup;
     for (int s = 0; s < ns; ++s) {
-      const int row = (int)r0 + (forward ? s : ns - 1 - s);
+      const int row = (int)rs[q] + (forward ? s : ns - 1 - s);
       rows[fill[lev_all[row]]++] = row;
     }
-    r0 += ns;
   }
   t.level_rows.upload(rows.data(), std::max<size_t>(rows.size(), 1) * 4);
   t.ia.upload(ia_all.data(), ia_all.size() * 4);
