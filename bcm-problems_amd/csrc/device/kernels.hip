@@ -829,19 +829,59 @@ static uint16_t to_bf16(float f) {
   return (uint16_t)((u + rounding) >> 16);
 }
 
-// Converts `count` fp64 values to `prec` and copies them to dst (device).
+// Pinned host staging buffers for the image uploads, shared by the host
+// threads of a build: a thread takes one for a part and returns it after.
+// They are allocated on first use and kept until the process ends.
+namespace {
+constexpr size_t kStageBytes = 32u << 20;
+std::mutex g_stage_mu;
+std::vector<void*> g_stage_free;
+
+void* stage_take() {
+  {
+    std::lock_guard<std::mutex> g(g_stage_mu);
+    if (!g_stage_free.empty()) {
+      void* p = g_stage_free.back();
+      g_stage_free.pop_back();
+      return p;
+    }
+  }
+  void* p = nullptr;
+  BCM_HIP(hipHostMalloc(&p, kStageBytes, hipHostMallocDefault));
+  return p;
+}
+
+void stage_give(void* p) {
+  std::lock_guard<std::mutex> g(g_stage_mu);
+  g_stage_free.push_back(p);
+}
+}  // namespace
+
+// Converts `count` fp64 values to `prec` and copies them to dst (device),
+// through a pinned staging buffer one chunk at a time.
 static void upload_values_at(const double* v, size_t count, int prec, void* dst) {
   if (count == 0) return;
-  if (prec == kF64) {
-    BCM_HIP(hipMemcpy(dst, v, count * 8, hipMemcpyHostToDevice));
-  } else if (prec == kF32) {
-    std::vector<float> f(v, v + count);
-    BCM_HIP(hipMemcpy(dst, f.data(), count * 4, hipMemcpyHostToDevice));
-  } else {
-    std::vector<uint16_t> h(count);
-    for (size_t i = 0; i < count; ++i) h[i] = to_bf16((float)v[i]);
-    BCM_HIP(hipMemcpy(dst, h.data(), count * 2, hipMemcpyHostToDevice));
+  const size_t es = prec_bytes(prec), chunk = kStageBytes / es;
+  void* st = stage_take();
+  try {
+    for (size_t i0 = 0; i0 < count; i0 += chunk) {
+      const size_t m = std::min(chunk, count - i0);
+      if (prec == kF64) {
+        std::memcpy(st, v + i0, m * 8);
+      } else if (prec == kF32) {
+        float* f = static_cast<float*>(st);
+        for (size_t i = 0; i < m; ++i) f[i] = (float)v[i0 + i];
+      } else {
+        uint16_t* h = static_cast<uint16_t*>(st);
+        for (size_t i = 0; i < m; ++i) h[i] = to_bf16((float)v[i0 + i]);
+      }
+      BCM_HIP(hipMemcpy(static_cast<char*>(dst) + i0 * es, st, m * es, hipMemcpyHostToDevice));
+    }
+  } catch (...) {
+    stage_give(st);
+    throw;
   }
+  stage_give(st);
 }
 
 static int uniform_block(const BlockCsr& h) {
