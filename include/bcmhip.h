@@ -199,7 +199,10 @@ typedef struct bcm_batch_tune {
   double ms_per_iter;
 } bcm_batch_tune;
 /* Builds the union of ncell matrices and their IC2 factors (one block size).
- * prec_mask: bit p keeps the factor sweeps in precision p (bit 0 F64, 1 F32, 2 BF16). */
+ * prec_mask: bit p keeps the factor sweeps in precision p (bit 0 F64, 1 F32, 2 BF16).
+ * The per-cell host work runs on up to 16 host threads of its own; the copies
+ * go through pinned staging buffers (32 MB each, at most one per thread) that
+ * the library keeps for reuse until the process ends. */
 int bcm_batch_create(int ncell, const bcm_matrix* mats, const bcm_factor* facts, int prec_mask, bcm_batch* out);
 int bcm_batch_destroy(bcm_batch b);
 /* Union length n and the element offset of every cell (elem0: ncell + 1 entries, may be NULL). */
