@@ -478,16 +478,28 @@ void batch_build(Batch& bt, const std::vector<const Matrix*>& mats, const std::v
     lower.push_back(&facts[c]->lower());
     upper.push_back(&facts[c]->u);
   }
-  // upper-stored (symmetric) cells are expanded to full storage, one cell per host thread
-  parallel_for(bt.ncell, host_threads(), [&](int c) {
-    if (mats[c]->symmetric) {
-      full[c] = symm_mtr(mats[c]->h);
-      parts[c] = &full[c];
-    } else {
-      parts[c] = &mats[c]->h;
-    }
-  });
-  dev_bsr_upload_parts(parts, kF64, bt.A);
+  // upper-stored (symmetric) cells: only the upper blocks are copied and the
+  // device expands them; otherwise each cell is expanded on a host thread
+  bool all_symm = true;
+  for (const Matrix* m : mats) all_symm = all_symm && m->symmetric;
+  std::vector<int64_t> full_blocks;
+  bool on_device = false;
+  if (all_symm) {
+    std::vector<const BlockCsr*> ups;
+    for (const Matrix* m : mats) ups.push_back(&m->h);
+    on_device = dev_bsr_upload_symm_parts(ups, bt.A, full_blocks);
+  }
+  if (!on_device) {
+    parallel_for(bt.ncell, host_threads(), [&](int c) {
+      if (mats[c]->symmetric) {
+        full[c] = symm_mtr(mats[c]->h);
+        parts[c] = &full[c];
+      } else {
+        parts[c] = &mats[c]->h;
+      }
+    });
+    dev_bsr_upload_parts(parts, kF64, bt.A);
+  }
   auto offdiag = [](const BlockCsr& m) {
     int64_t k = 0;
     for (int i = 0; i < m.nsup; ++i)
@@ -499,8 +511,8 @@ void batch_build(Batch& bt, const std::vector<const Matrix*>& mats, const std::v
   bt.cell_fdep.resize(mats.size());
   bt.cell_bdep.resize(mats.size());
   for (size_t c = 0; c < mats.size(); ++c) {
-    bt.cell_rows[c] = parts[c]->nsup;
-    bt.cell_ablk[c] = parts[c]->ia[parts[c]->nsup];
+    bt.cell_rows[c] = mats[c]->h.nsup;
+    bt.cell_ablk[c] = on_device ? full_blocks[c] : parts[c]->ia[parts[c]->nsup];
     bt.cell_fdep[c] = offdiag(*lower[c]);
     bt.cell_bdep[c] = offdiag(*upper[c]);
   }

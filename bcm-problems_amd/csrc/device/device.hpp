@@ -66,6 +66,12 @@ struct DevBsr {
 void dev_bsr_upload(const BlockCsr& h, int prec, DevBsr& d);
 // Block-diagonal union of several matrices (same block size), part after part.
 void dev_bsr_upload_parts(const std::vector<const BlockCsr*>& parts, int prec, DevBsr& d);
+// The same for upper-stored symmetric parts, expanded to full storage on the
+// device: only the upper blocks cross the bus. Returns false (and leaves d
+// alone) when a part holds a block below its diagonal; full_blocks[q] is the
+// full block count of part q.
+bool dev_bsr_upload_symm_parts(const std::vector<const BlockCsr*>& uppers, DevBsr& d,
+                               std::vector<int64_t>& full_blocks);
 
 struct DevTri {
   int nsup = 0, n = 0, b = 0, ndep = 0;

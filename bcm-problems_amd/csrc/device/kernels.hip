@@ -936,6 +936,114 @@ void dev_bsr_upload_parts(const std::vector<const BlockCsr*>& parts, int prec, D
 
 void dev_bsr_upload(const BlockCsr& h, int prec, DevBsr& d) { dev_bsr_upload_parts({&h}, prec, d); }
 
+// Full storage of a symmetric matrix from its upper blocks: full block k is
+// upper block src[k] (src[k] >= 0) or the transpose of upper block ~src[k]
+// (a block below the diagonal). Blocks are b x b, column-major.
+__global__ __launch_bounds__(256) void k_expand_symm(int64_t nblk, int b, const int64_t* __restrict__ src,
+                                                     const double* __restrict__ up, double* __restrict__ full) {
+  const int bb = b * b;
+  for (int64_t k = blockIdx.x; k < nblk; k += gridDim.x) {
+    const int64_t sk = src[k];
+    const bool tr = sk < 0;
+    const double* sp = up + (tr ? ~sk : sk) * bb;
+    double* dp = full + k * bb;
+    for (int e = threadIdx.x; e < bb; e += blockDim.x) {
+      const int c = e / b, r = e - c * b;  // element (r, c) of the full block
+      dp[e] = tr ? sp[r * b + c] : sp[e];
+    }
+  }
+}
+
+bool dev_bsr_upload_symm_parts(const std::vector<const BlockCsr*>& uppers, DevBsr& d,
+                               std::vector<int64_t>& full_blocks) {
+  const int b = parts_block(uppers, "device matrices");
+  const int np = (int)uppers.size();
+  for (const BlockCsr* u : uppers)
+    for (int i = 0; i < u->nsup; ++i)
+      for (int p = u->ia[i]; p < u->ia[i + 1]; ++p)
+        if (u->ja[p] < i) return false;
+  // per part: full row i = the transposes of upper blocks (j, i), j < i, in
+  // ascending j, then upper row i (the same order as the host expansion)
+  std::vector<std::vector<int>> pia(np), pja(np);
+  std::vector<std::vector<int64_t>> psrc(np);
+  parallel_for(np, host_threads(), [&](int q) {
+    const BlockCsr& u = *uppers[q];
+    const int ns = u.nsup;
+    std::vector<int> cnt(ns + 1, 0);
+    for (int i = 0; i < ns; ++i)
+      for (int p = u.ia[i]; p < u.ia[i + 1]; ++p)
+        if (u.ja[p] > i) cnt[u.ja[p]]++;
+    std::vector<int>& ia = pia[q];
+    ia.assign(ns + 1, 0);
+    for (int i = 0; i < ns; ++i) ia[i + 1] = ia[i] + cnt[i] + (u.ia[i + 1] - u.ia[i]);
+    pja[q].assign(ia[ns], 0);
+    psrc[q].assign(ia[ns], 0);
+    std::vector<int> fill(ia.begin(), ia.end() - 1);
+    for (int i = 0; i < ns; ++i)  // lower parts, rows j receive (j, i) in ascending i
+      for (int p = u.ia[i]; p < u.ia[i + 1]; ++p)
+        if (u.ja[p] > i) {
+          const int j = u.ja[p], at = fill[j]++;
+          pja[q][at] = i;
+          psrc[q][at] = ~(int64_t)p;
+        }
+    for (int i = 0; i < ns; ++i)
+      for (int p = u.ia[i]; p < u.ia[i + 1]; ++p) {
+        const int at = fill[i]++;
+        pja[q][at] = u.ja[p];
+        psrc[q][at] = p;
+      }
+  });
+  std::vector<int64_t> r0(np + 1, 0), f0(np + 1, 0), u0(np + 1, 0);
+  for (int q = 0; q < np; ++q) {
+    r0[q + 1] = r0[q] + uppers[q]->nsup;
+    f0[q + 1] = f0[q] + pia[q][uppers[q]->nsup];
+    u0[q + 1] = u0[q] + uppers[q]->nzja();
+  }
+  const int64_t nsup = r0[np], nfull = f0[np], nup = u0[np];
+  if (nsup * b > INT32_MAX || nfull > INT32_MAX) throw Error(kErrArg, "device matrix exceeds 32-bit indexing");
+  const size_t bb = (size_t)b * b;
+  std::vector<int> ia(nsup + 1, 0), ja(std::max<int64_t>(nfull, 1), 0);
+  std::vector<int64_t> src(std::max<int64_t>(nfull, 1), 0);
+  DevBuf up;
+  up.alloc(std::max<size_t>(nup * bb * 8, 8));
+  full_blocks.assign(np, 0);
+  parallel_for(np, host_threads(), [&](int q) {
+    const BlockCsr& u = *uppers[q];
+    for (int i = 0; i < u.nsup; ++i) ia[r0[q] + i + 1] = (int)(f0[q] + pia[q][i + 1]);
+    for (size_t k = 0; k < pja[q].size(); ++k) {
+      ja[f0[q] + k] = (int)(r0[q] + pja[q][k]);
+      const int64_t sk = psrc[q][k];
+      src[f0[q] + k] = sk < 0 ? ~(u0[q] + ~sk) : u0[q] + sk;
+    }
+    full_blocks[q] = (int64_t)pja[q].size();
+    upload_values_at(u.a.data(), u.a.size(), kF64, static_cast<char*>(up.p) + (size_t)u0[q] * bb * 8);
+  });
+  d.nsup = (int)nsup;
+  d.n = (int)(nsup * b);
+  d.b = b;
+  d.nzja = (int)nfull;
+  d.prec = kF64;
+  d.a.alloc(std::max<size_t>(nfull * bb * 8, 8));
+  DevBuf dsrc;
+  dsrc.upload(src.data(), src.size() * 8);
+  if (nfull > 0) {
+    // a stream of its own, so that the build does not wait for work other
+    // streams run meanwhile (e.g. the solve of the previous batch)
+    hipStream_t xs = nullptr;
+    BCM_HIP(hipStreamCreateWithFlags(&xs, hipStreamNonBlocking));
+    hipLaunchKernelGGL(k_expand_symm, dim3((unsigned)std::min<int64_t>(nfull, 1 << 16)), dim3(256), 0, xs, nfull,
+                       b, dsrc.as<int64_t>(), up.as<double>(), d.a.as<double>());
+    const hipError_t le = hipGetLastError();
+    const hipError_t se = hipStreamSynchronize(xs);
+    (void)hipStreamDestroy(xs);
+    BCM_HIP(le);
+    BCM_HIP(se);
+  }
+  d.ia.upload(ia.data(), ia.size() * 4);
+  d.ja.upload(ja.data(), ja.size() * 4);
+  return true;
+}
+
 // Sweep image of the block-diagonal union of factors (one per part, rows of
 // part p after those of parts < p). Forward: rows gather the transposed
 // blocks of the column-stored factor; backward: rows of the row-stored factor.
