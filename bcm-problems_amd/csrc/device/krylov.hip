@@ -293,19 +293,27 @@ void set_pending(Context& ctx, int blocks, int k) {
 // ---------------------------------------------------------------------------
 // Device images
 
+// Full storage of an upper-stored symmetric matrix: the upper blocks are
+// copied and expanded on the device; on the host only if a block lies below
+// the diagonal.
+static void upload_symmetric(const BlockCsr& upper, DevBsr& d) {
+  std::vector<int64_t> full_blocks;
+  if (!dev_bsr_upload_symm_parts({&upper}, d, full_blocks)) dev_bsr_upload(symm_mtr(upper), kF64, d);
+}
+
 void Matrix::to_device(bool with_transpose) {
   if (!has_d) {
     if (symmetric) {
-      BlockCsr full = symm_mtr(h);
-      dev_bsr_upload(full, kF64, d);
-      if (with_transpose) { dev_bsr_upload(full, kF64, dt); has_dt = true; }
+      upload_symmetric(h, d);
+      if (with_transpose) { upload_symmetric(h, dt); has_dt = true; }
     } else {
       dev_bsr_upload(h, kF64, d);
     }
     has_d = true;
   }
   if (with_transpose && !has_dt) {
-    dev_bsr_upload(symmetric ? symm_mtr(h) : transp_mtr(h), kF64, dt);
+    if (symmetric) upload_symmetric(h, dt);
+    else dev_bsr_upload(transp_mtr(h), kF64, dt);
     has_dt = true;
   }
 }
