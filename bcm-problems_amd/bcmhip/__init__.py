@@ -68,6 +68,7 @@ class Stats(C.Structure):
     _fields_ = [
         ("iterations", C.c_int), ("restarts", C.c_int), ("breakdown", C.c_int), ("converged", C.c_int),
         ("relres_true", C.c_double), ("relres_rec", C.c_double), ("seconds", C.c_double),
+        ("errmin", C.c_double),
     ]
 
 
@@ -119,6 +120,9 @@ SIGNATURES = {
                                        C.c_ulonglong, C.POINTER(_P), _DP]),
     "bcm_rcm_order": (C.c_int, [_P, _IP]),
     "bcm_matrix_reorder": (C.c_int, [_P, _IP, C.POINTER(_P)]),
+    "bcm_combine_lu": (C.c_int, [_P, C.POINTER(_P), C.POINTER(_P)]),
+    "bcm_dsyev": (C.c_int, [C.c_int, _DP, _DP]),
+    "bcm_dgesvd": (C.c_int, [C.c_int, _DP, _DP, _DP, _DP]),
     "bcm_vector_reorder": (C.c_int, [_P, _IP, C.c_char, _DP, C.c_int]),
     "bcm_factor_ic2": (C.c_int, [_P, C.POINTER(Params), C.POINTER(_P)]),
     "bcm_factor_ilu2": (C.c_int, [_P, C.POINTER(Params), C.POINTER(_P)]),
@@ -323,6 +327,12 @@ class Matrix:
         _check(lib().bcm_matrix_reorder(self.h, _ip(order), C.byref(out)))
         return Matrix(out.value)
 
+    def combine_lu(self):
+        """CombineLU (smatrixrs.cpp:1645): (L by columns, U by rows), union-padded."""
+        lo, up = C.c_void_p(), C.c_void_p()
+        _check(lib().bcm_combine_lu(self.h, C.byref(lo), C.byref(up)))
+        return Matrix(lo.value), Matrix(up.value)
+
     def reorder_vector(self, order, v: np.ndarray, inverse: bool = False) -> np.ndarray:
         order = np.ascontiguousarray(order, dtype=np.int32)
         v = np.array(v, dtype=np.float64, order="F", copy=True)
@@ -411,6 +421,26 @@ class Factor:
         a = np.zeros(i["nza"], np.float64)
         _check(lib().bcm_factor_export(self.h, which, _ip(ia), _ip(ja), _dp(a)))
         return ia, ja, a
+
+
+def dsyev(a: np.ndarray):
+    """dsyev_("V", "U") (LAPACK 3.0 DSYEV): (w ascending, eigenvectors in columns)."""
+    n = a.shape[0]
+    z = np.array(a, dtype=np.float64, order="F", copy=True)
+    w = np.zeros(n)
+    _check(lib().bcm_dsyev(n, _dp(z), _dp(w)))
+    return w, z
+
+
+def dgesvd(a: np.ndarray):
+    """dgesvd_("A", "A") (LAPACK 3.0 DGESVD) of a square matrix: (s, U, VT)."""
+    n = a.shape[0]
+    x = np.array(a, dtype=np.float64, order="F", copy=True)
+    s = np.zeros(n)
+    u = np.zeros((n, n), order="F")
+    vt = np.zeros((n, n), order="F")
+    _check(lib().bcm_dgesvd(n, _dp(x), _dp(s), _dp(u), _dp(vt)))
+    return s, u, vt
 
 
 class Context:

@@ -112,6 +112,7 @@ void fill_stats(const bcm::SolveStats& s, bcm_stats* st) {
   st->relres_true = s.relres_true;
   st->relres_rec = s.relres_rec;
   st->seconds = s.seconds;
+  st->errmin = s.errmin;
 }
 
 bcm::BlockCsr make_csr(int nsup, const int* sprnd, const int* ia, const int* ja, const double* a) {
@@ -326,6 +327,40 @@ int bcm_matrix_reorder(bcm_matrix m, const int* order, bcm_matrix* out) {
       throw;
     }
     *out = h;
+  });
+}
+
+int bcm_dsyev(int n, double* a, double* w) {
+  return guarded([&] {
+    need(n >= 0 && (n == 0 || (a && w)), "dsyev: bad argument");
+    bcm::sym_eig(n, a, w);
+  });
+}
+
+int bcm_dgesvd(int n, double* a, double* s, double* u, double* vt) {
+  return guarded([&] {
+    need(n >= 0 && (n == 0 || (a && s && u && vt)), "dgesvd: bad argument");
+    std::vector<double> work((size_t)10 * n + 1);
+    const int info = bcm::lapack30::dgesvd_aa(n, a, n, s, u, n, vt, n, work.data(), 10 * n);
+    if (info != 0) throw bcm::Error(bcm::kErrInternal, "Error in the Lapack routine DGESVD");
+  });
+}
+
+int bcm_combine_lu(bcm_matrix a, bcm_matrix* lower, bcm_matrix* upper) {
+  return guarded([&] {
+    need(a && lower && upper, "combine_lu: null argument");
+    need(!a->m.symmetric, "combine_lu: expects a matrix in full storage");
+    auto* l = new bcm_matrix_s;
+    auto* u = new bcm_matrix_s;
+    try {
+      bcm::combine_lu(a->m.h, l->m.h, u->m.h);
+    } catch (...) {
+      delete l;
+      delete u;
+      throw;
+    }
+    *lower = l;
+    *upper = u;
   });
 }
 

@@ -44,6 +44,8 @@ enum : int {
   kCpUau = 11, kCpRu = 12, kCpAcoef = 13, kCpBcoef = 14, kCpRr = 15, kCpVau = 16, kCpThr2 = 17,
   // shared
   kBnorm2 = 20, kR0norm2 = 21, kTrue0 = 22, kTrue1 = 23,
+  // PCG, ittype 2: max_i |r_i| of the residual at the check (MaxProdSVect with a unit norm vector)
+  kErrloc = 24,
 };
 
 __device__ double threshold(const KState* ks, int ittype, double eps2) {
@@ -76,8 +78,12 @@ __global__ __launch_bounds__(256) void k_scalar(int op, KState* ks, double eps2,
     case kOpPcgCheck: {
       const int k = ++st[1];
       if (k == 1) sc[kEnrsq0] = sc[kEnrsq];
-      const double thr = ittype == 0 ? eps2 * sc[kEnrsq0] : threshold(ks, ittype, eps2);
-      if (!(sc[kEnrsq] > thr)) { st[0] = 1; st[4] = 1; return; }
+      if (ittype == 2) {  // cg.cpp:241: errmin <= |eps|, eps2 carries |eps|
+        if (!(sc[kErrloc] > eps2)) { st[0] = 1; st[4] = 1; return; }
+      } else {
+        const double thr = ittype == 0 ? eps2 * sc[kEnrsq0] : threshold(ks, ittype, eps2);
+        if (!(sc[kEnrsq] > thr)) { st[0] = 1; st[4] = 1; return; }
+      }
       if (k >= niter) { st[0] = 1; return; }
       if (!(sc[kGamma] > 0.0)) { st[0] = 1; st[2] = 1; return; }
       sc[kBeta] = k > 1 ? sc[kGamma] / sc[kGamold] : 0.0;
@@ -161,6 +167,35 @@ __global__ void k_pcg_xr(int64_t n, const KState* ks, const double* __restrict__
     r[i] = r[i] + (-alpha) * q[i];
     x[i] = x[i] + alpha * p[i];
   }
+}
+
+// max_i |r_i| (MaxProdSVect, svector.cpp:376-385, with the unit norm vector
+// every Abstract*Solver passes, solvers.cpp:5554,5762): per-block maxima,
+// then one block folds them into sc[slot]. max is exact in any order.
+constexpr int kMaxBlocks = 1024;
+__global__ __launch_bounds__(256) void k_maxabs(int64_t n, const int* skip, const double* __restrict__ r,
+                                                double* part) {
+  if (*skip) return;
+  __shared__ double sm[4];
+  double m = 0.0;
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) m = fmax(m, fabs(r[i]));
+  for (int off = 32; off > 0; off >>= 1) m = fmax(m, __shfl_down(m, off));
+  if ((threadIdx.x & 63) == 0) sm[threadIdx.x >> 6] = m;
+  __syncthreads();
+  if (threadIdx.x == 0) part[blockIdx.x] = fmax(fmax(sm[0], sm[1]), fmax(sm[2], sm[3]));
+}
+
+__global__ __launch_bounds__(256) void k_maxabs_fold(int nb, const int* skip, const double* __restrict__ part,
+                                                     KState* ks, int slot) {
+  if (*skip) return;
+  __shared__ double sm[4];
+  double m = 0.0;
+  for (int i = threadIdx.x; i < nb; i += blockDim.x) m = fmax(m, part[i]);
+  for (int off = 32; off > 0; off >>= 1) m = fmax(m, __shfl_down(m, off));
+  if ((threadIdx.x & 63) == 0) sm[threadIdx.x >> 6] = m;
+  __syncthreads();
+  if (threadIdx.x == 0) ks->sc[slot] = fmax(fmax(sm[0], sm[1]), fmax(sm[2], sm[3]));
 }
 
 // ---- BiCGStab vector kernels ----
@@ -516,7 +551,11 @@ void pcg(Matrix& A, Factor& F, Context& ctx, const SolveOptions& opt, const doub
   double* q = ctx.vec(3, vb);
   KState* ks = ctx.dstate();
   const int* done = &ks->st[0];
-  const double eps2 = opt.eps * opt.eps;
+  // ittype 2 compares max|r_i| with |eps| (cg.cpp:241); the others compare squared 2-norms
+  const bool maxnorm = opt.ittype == 2;
+  const double eps2 = maxnorm ? std::fabs(opt.eps) : opt.eps * opt.eps;
+  const int gm = std::min(kMaxBlocks, ew_grid(n));
+  double* mpart = maxnorm ? ctx.vec(15, (size_t)kMaxBlocks * 8) : nullptr;
   hipStream_t s = ctx.stream;
   int total = 0;
   for (int attempt = 0; attempt <= opt.max_restarts; ++attempt) {
@@ -536,6 +575,10 @@ void pcg(Matrix& A, Factor& F, Context& ctx, const SolveOptions& opt, const doub
         const double* xs[2] = {r, r};
         const double* ys[2] = {r, z};
         launch_multidot(n, 2, xs, ys, kDeferred, ctx.rw, s, done);
+        if (maxnorm) {
+          hipLaunchKernelGGL(k_maxabs, dim3(gm), dim3(256), 0, s, (int64_t)n, done, r, mpart);
+          hipLaunchKernelGGL(k_maxabs_fold, dim3(1), dim3(256), 0, s, gm, done, mpart, ks, (int)kErrloc);
+        }
         scalar_step(ctx, kOpPcgCheck, eps2, opt.ittype, opt.niter - total, kEnrsq);  // (r,r) -> 8, (r,z) -> 9
         if (exact) {  // no launch is queued past the convergence test
           ctx.pull_state();
@@ -558,6 +601,7 @@ void pcg(Matrix& A, Factor& F, Context& ctx, const SolveOptions& opt, const doub
     total += std::max(0, k - 1);
     st.breakdown = hs.st[2];
     st.relres_rec = hs.sc[kBnorm2] > 0 ? std::sqrt(hs.sc[kEnrsq] / hs.sc[kBnorm2]) : std::sqrt(hs.sc[kEnrsq]);
+    st.errmin = maxnorm ? hs.sc[kErrloc] : 0.0;
     TrueResidual tr = true_residual(A, ctx, b, x, r);
     st.relres_true = tr.relres;
     const bool ok = opt.ittype == 3 ? tr.relres <= opt.eps : hs.st[4] != 0;

@@ -126,7 +126,7 @@ struct SolveOptions {
   int kind = kPcg;
   int niter = 1000;
   double eps = 1e-10;
-  int ittype = 3;        // 0: ||r|| <= eps ||r0||, 1: ||r|| <= eps, 3: ||r|| <= eps ||b||
+  int ittype = 3;        // 0: ||r|| <= eps ||r0||, 1: ||r|| <= eps, 2 (PCG): max|r_i| <= |eps|, 3: ||r|| <= eps ||b||
   int check_every = 8;   // host polls the device state every k iterations
   int restart = 30;      // FGMRES cycle length
   int prec = kF64;       // precision of the preconditioner application
@@ -142,6 +142,7 @@ struct SolveStats {
   double relres_rec = 0.0;    // from the recursion
   double seconds = 0.0;
   int launches = 0;
+  double errmin = 0.0;        // PCG with ittype 2: max|r_i| at the last check (Cg's errmin, cg.cpp:210)
 };
 
 // Solves A x = b for one right-hand side (b, x device pointers, x holds x0).

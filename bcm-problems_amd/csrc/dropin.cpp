@@ -137,12 +137,24 @@ void pick_solver(bool symmetric, bool par, const bcm_params& prm, bcm::SolveOpti
     }
     return;
   }
-  opt.kind = symmetric ? bcm::kPcg : (prm.ittype == 1 ? bcm::kLanczos : bcm::kBicgstab);
-  // Cg's stopping rules (cg.cpp:238-241): 0 relative, 1 absolute, 2 max-norm
-  // (absolute, norm vector of ones); the GPU drivers test the 2-norm for 2 as well.
+  if (symmetric) {
+    // Cg's stopping rules (cg.cpp:238-241): 0 relative to ||r0||, 1 absolute,
+    // 2 max_i |r_i| <= |eps| (the unit norm vector of solvers.cpp:5554), and 3
+    // (no reference counterpart) relative to ||b||.
+    if (prm.ittype < 0 || prm.ittype > 3) throw bcm::Error(bcm::kErrArg, "AbstractSymmSolver: ittype must be 0..3");
+    opt.kind = bcm::kPcg;
+    opt.ittype = prm.ittype;
+    return;
+  }
+  // Solver(RS) (solvers.cpp:1710-1800): ittype 1 two-sided Lanczos per
+  // right-hand side; 2, 3, 4 the block Lanczos variants Central, Right, Left.
+  if (prm.ittype != 1)
+    throw bcm::Error(bcm::kErrArg,
+                     "AbstractSolver: only ittype 1 (two-sided Lanczos) is available; the block Lanczos variants "
+                     "of ittype 2-4 are not");
+  opt.kind = bcm::kLanczos;
   // Lanczos sttype (lanczos.cpp:263): 0 absolute, 1 relative to ||r0||, 2 max-norm.
-  opt.ittype = symmetric ? (prm.ittype == 0 ? 0 : prm.ittype == 3 ? 3 : 1)
-                         : (prm.sttype == 1 ? 0 : (prm.sttype == 2 && opt.kind == bcm::kLanczos ? 2 : 1));
+  opt.ittype = prm.sttype == 1 ? 0 : prm.sttype == 2 ? 2 : 1;
 }
 
 int run(bool symmetric, bool par, bcm_params* p, const Callbacks& cb) {
@@ -213,7 +225,9 @@ int run(bool symmetric, bool par, bcm_params* p, const Callbacks& cb) {
     BCM_HIP(hipMemcpy(x.data(), dx.p, n * 8, hipMemcpyDeviceToHost));
     // serial Cg reports its loop index at exit, one past the last solution update (cg.cpp:209)
     prm.nitperf = opt.kind == bcm::kPcg ? st.iterations + 1 : st.iterations;
-    prm.resfin = st.relres_true;
+    // Cg reports errmin (cg.cpp:210): max_i |r_i| under ittype 2; the other
+    // rules report the FP64 true relative residual here (the reference leaves 0)
+    prm.resfin = opt.kind == bcm::kPcg && opt.ittype == 2 ? st.errmin : st.relres_true;
     prm.timeitr = st.seconds;
     if (prm.ordtype != 0) bcm::ord_vect('I', a.sprnd, sp_new, order, x.data(), a.n(), 1);
     for (int i = 0; i < a.nsup; ++i) cb.fstoresol(cb.obj, i, &x[a.sprnd[i]]);

@@ -104,13 +104,26 @@ void ord_vect(char type, const std::vector<int>& sprnd_old, const std::vector<in
 // Upper-triangular part (j >= i) of a full matrix.
 BlockCsr upper_part(const BlockCsr& a);
 
-// ---- small dense kernels (dense_small.cpp) ----
+// ---- small dense kernels (lapack30.cpp, dense_small.cpp) ----
+
+namespace lapack30 {
+// DSYEV with JOBZ = 'V', UPLO = 'U' (LAPACK 3.0): eigenvalues ascending in w,
+// eigenvectors in the columns of a. work holds lwork >= 3n - 1 doubles.
+// Returns INFO (0, > 0: DSTEQR did not converge, < 0: bad argument).
+int dsyev_vu(int n, double* a, int lda, double* w, double* work, int lwork);
+// DGESVD with JOBU = JOBVT = 'A' on a square n x n matrix (LAPACK 3.0):
+// a = U diag(s) VT, s descending; a is overwritten. work holds lwork >= 5n
+// doubles. Returns INFO (> 0: DBDSQR did not converge, < 0: bad argument).
+int dgesvd_aa(int n, double* a, int lda, double* s, double* u, int ldu, double* vt, int ldvt, double* work,
+              int lwork);
+}  // namespace lapack30
 
 // Symmetric eigen-decomposition of an n x n column-major matrix: eigenvalues
-// ascending in w, orthonormal eigenvectors in the columns of a (overwritten).
+// ascending in w, orthonormal eigenvectors in the columns of a (overwritten):
+// dsyev_vu with the workspace the reference's factorizations pass.
 void sym_eig(int n, double* a, double* w);
 // Singular value decomposition a = U diag(s) V^T (square n x n, column-major):
-// s descending; u = U, vt = V^T (both column-major).
+// s descending; u = U, vt = V^T (both column-major): dgesvd_aa on a copy of a.
 void svd_square(int n, const double* a, double* s, double* u, double* vt);
 
 // ---- factorizations (ic2.cpp, ilu2.cpp) ----

@@ -49,7 +49,8 @@ typedef struct bcm_params {
   int ordtype;    /* 1: RCM profile ordering (OrderPrfMtr), 0: none */
   double memory;  /* factor workspace in units of nza(A); <= 0: unbounded */
   double tau1, tau2, theta, sclmin, pivmin;
-  int ittype;     /* Cg stopping rule: 0 rel. to ||r0||, 1 absolute, 2 max-norm, 3 rel. to ||b|| */
+  int ittype;     /* Cg stopping rule (cg.cpp:238-241): 0 ||r|| <= eps ||r0||, 1 ||r|| <= eps,
+                     2 max_i |r_i| <= |eps| (MaxProdSVect with a unit norm vector), 3 ||r|| <= eps ||b|| */
   int sttype;
   int niter;
   int ichk;
@@ -65,7 +66,8 @@ typedef struct bcm_solve_opts {
   int kind;          /* BCM_PCG, BCM_BICGSTAB, BCM_FGMRES, BCM_LANCZOS, BCM_CGPAR, BCM_BLOCKGMRES */
   int niter;
   double eps;
-  int ittype;        /* as bcm_params.ittype; 3 = ||b - A x|| <= eps ||b|| */
+  int ittype;        /* 0 ||r|| <= eps ||r0||, 1 ||r|| <= eps, 2 (BCM_PCG only) max_i |r_i| <= |eps|,
+                        3 ||b - A x|| <= eps ||b|| */
   int check_every;   /* device state polled every k iterations */
   int restart;       /* FGMRES cycle length */
   int precision;     /* preconditioner storage/application: BCM_F64, BCM_F32, BCM_BF16 */
@@ -75,6 +77,7 @@ typedef struct bcm_solve_opts {
 typedef struct bcm_stats {
   int iterations, restarts, breakdown, converged;
   double relres_true, relres_rec, seconds;
+  double errmin;  /* BCM_PCG with ittype 2: max_i |r_i| at the last check (Cg's errmin, cg.cpp:194-210) */
 } bcm_stats;
 
 /* ---- library ---- */
@@ -108,6 +111,22 @@ int bcm_matrix_reorder(bcm_matrix m, const int* order, bcm_matrix* out);
 /* CSVector::OrdVect (svector.cpp:418): type 'D' applies order, 'I' undoes it. */
 int bcm_vector_reorder(bcm_matrix original, const int* order, char type, double* v, int nrhs);
 
+/* ---- dense kernels of the factorizations ---- */
+/* dsyev_("V", "U") as the IC2 calls it (lapackd.cpp:865-878 -> LAPACK 3.0
+ * DSYEV, ich2rs.cpp:110): a (n x n column-major, upper triangle read) is
+ * overwritten by the eigenvectors, w receives the eigenvalues ascending. */
+int bcm_dsyev(int n, double* a, double* w);
+/* dgesvd_("A", "A") on a square matrix as the ILU2 calls it (lapackd.cpp:883-905
+ * -> LAPACK 3.0 DGESVD, ilu2rs.cpp:126): a = U diag(s) VT, s descending; a is
+ * overwritten. */
+int bcm_dgesvd(int n, double* a, double* s, double* u, double* vt);
+
+/* CSMatrixRS::CombineLU (smatrixrs.cpp:1645-1803): splits a full-storage
+ * matrix into L stored by columns (as the transpose of its strict lower part,
+ * diagonal blocks transposed) and U by rows, both padded with zero blocks to
+ * the union of the two patterns: the input pair of bcm_factor_ilu2. */
+int bcm_combine_lu(bcm_matrix a, bcm_matrix* lower, bcm_matrix* upper);
+
 /* ---- host factorizations ---- */
 /* CSMatrixRS::Ich2 (ich2rs.cpp:1075) of an upper-stored SPD matrix. */
 int bcm_factor_ic2(bcm_matrix upper, bcm_params* p, bcm_factor* out);
@@ -134,7 +153,8 @@ int bcm_context_sync(bcm_context c);
  * 1 = one sync-free cooperative launch per sweep (default): resident waves
  * own positions of the rows sorted by dependency level round-robin and wait
  * (bounded) on the values of the rows they depend on; 2 = as 1 over the
- * natural sweep order. All three give bit-identical results. */
+ * natural sweep order. The schedules agree to rounding: per row they add
+ * the same products, grouped differently across lanes for some block sizes. */
 int bcm_context_set_sweep_mode(bcm_context c, int mode);
 
 /* Per-kernel-class timing with HIP events on the context's stream (the

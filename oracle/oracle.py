@@ -96,6 +96,11 @@ def ref() -> C.CDLL:
         l.ref_order_prf.argtypes = mat + [_IP]
         l.ref_ordmtr.argtypes = [C.c_int] + mat + [_IP, _IP, _IP, _IP, _IP, _DP]
         l.ref_combinelu.argtypes = [C.c_int] + mat + [_IP, _IP, _IP, _IP, _DP]
+        l.ref_ich2.argtypes = mat + [C.c_double] * 4 + [C.c_char_p, C.c_int]
+        l.ref_ilu2.argtypes = mat + [C.c_double] * 6 + [C.c_char_p, C.c_int]
+        l.ref_fct_take.argtypes = [C.c_int, _IP, _IP, _IP, _IP, _DP]
+        l.ref_dsyev.argtypes = [C.c_int, _DP, _DP, _DP, C.c_int, _IP]
+        l.ref_dgesvd.argtypes = [C.c_int, _DP, _DP, _DP, _DP, _DP, C.c_int, _IP]
         _ref = l
     return _ref
 
@@ -291,3 +296,53 @@ def ref_ordmtr(m: Block, order, symmetric: bool):
 
 def ref_combinelu(m: Block, which: int):
     return _take(ref().ref_combinelu, which, *m.args(), nsup=m.nsup)
+
+
+class RefFactorError(RuntimeError):
+    pass
+
+
+def ref_ich2(m: Block, tau1=1e-2, tau2=1e-4, theta=0.1, memory=10.0):
+    """The reference's CSMatrixRS::Ich2 (ich2rs.cpp:1075) on an upper-stored
+    matrix: the U factor (ia, ja, a) in the reference layout."""
+    msg = C.create_string_buffer(256)
+    if ref().ref_ich2(*m.args(), tau1, tau2, theta, memory, msg, 256):
+        raise RefFactorError(msg.value.decode(errors="replace"))
+    return _take(ref().ref_fct_take, 1, nsup=m.nsup)
+
+
+def ref_ilu2(m: Block, tau1=1e-2, tau2=1e-4, theta=0.1, sclmin=1e-8, pivmin=1e-8, memory=10.0):
+    """CombineLU + the reference's CSMatrixRS::Ilu2 (ilu2rs.cpp:1458) on a
+    full-storage matrix: ((ia, ja, a) of L by columns, (ia, ja, a) of U by rows)."""
+    msg = C.create_string_buffer(256)
+    if ref().ref_ilu2(*m.args(), tau1, tau2, theta, sclmin, pivmin, memory, msg, 256):
+        raise RefFactorError(msg.value.decode(errors="replace"))
+    return _take(ref().ref_fct_take, 0, nsup=m.nsup), _take(ref().ref_fct_take, 1, nsup=m.nsup)
+
+
+def ref_dsyev(a):
+    """dsyev_("V", "U") through lapackd.cpp -> lapackdm.f -> DSYEV_SM: (w, Z column-major)."""
+    n = a.shape[0]
+    A = np.array(a, dtype=np.float64, order="F", copy=True)
+    w = np.zeros(n)
+    lw = 10 * n
+    work = np.zeros(lw)
+    info = C.c_int()
+    ref().ref_dsyev(n, A.ctypes.data_as(_DP), w.ctypes.data_as(_DP),
+                    work.ctypes.data_as(_DP), lw, C.byref(info))
+    return w, A, info.value
+
+
+def ref_dgesvd(a):
+    """dgesvd_("A", "A") through lapackd.cpp -> lapackdm.f -> DGESVD_SM: (s, U, VT), column-major."""
+    n = a.shape[0]
+    A = np.array(a, dtype=np.float64, order="F", copy=True)
+    s = np.zeros(n)
+    U = np.zeros((n, n), order="F")
+    VT = np.zeros((n, n), order="F")
+    lw = 10 * n
+    work = np.zeros(lw)
+    info = C.c_int()
+    ref().ref_dgesvd(n, A.ctypes.data_as(_DP), s.ctypes.data_as(_DP), U.ctypes.data_as(_DP),
+                     VT.ctypes.data_as(_DP), work.ctypes.data_as(_DP), lw, C.byref(info))
+    return s, U, VT, info.value

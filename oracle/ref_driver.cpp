@@ -5,7 +5,9 @@
 // and placed into CSMatrixRS objects through the reference's public
 // setters, the way AbstractSymmSolver builds its matrix (solvers.cpp:5383-5528).
 // The library is loaded lazily (RTLD_LAZY): the parts of the compiled sources
-// that would need MPI, METIS or LAPACK are never called.
+// that would need MPI or METIS are never called. The IC2 / ILU2 entries run
+// the reference's Ich2 / Ilu2 over its own LAPACK 3.0 (lapacksm/*.f and
+// Bslib/lapackdm.f, compiled with flang into the same library).
 #include <cstring>
 #include <fstream>
 
@@ -84,9 +86,78 @@ CSVector vec(int n, int nrhs, const double* x) {
   return v;
 }
 
+CSMatrixRS g_fct[2];  // last factor: [0] L by columns, [1] U by rows
+
+void set_fct_params(CSlvParam& p, double tau1, double tau2, double theta, double sclmin, double pivmin,
+                    double memory) {
+  p.msglev = 0;
+  p.tau1 = tau1;
+  p.tau2 = tau2;
+  p.theta = theta;
+  p.sclmin = sclmin;
+  p.pivmin = pivmin;
+  p.memory = memory;
+}
+
 }  // namespace
 
 extern "C" {
+
+// CSMatrixRS::Ich2 (ich2rs.cpp:1075) of an upper-stored SPD matrix, as SolveSymm
+// calls it (solvers.cpp:4545). Returns 0, or 1 when the reference threw (msg
+// receives the text). The factor is kept for ref_fct_take(1, ...).
+int ref_ich2(int nsup, const int* sprnd, const int* ia, const int* ja, const double* a, double tau1, double tau2,
+             double theta, double memory, char* msg, int msglen) {
+  CSMatrixRS am = make(nsup, sprnd, ia, ja, a);
+  CSlvParam p;
+  set_fct_params(p, tau1, tau2, theta, p.sclmin, p.pivmin, memory);
+  std::ofstream fout("/dev/null");
+  try {
+    g_fct[1] = am.Ich2(fout, p);
+    g_fct[0] = CSMatrixRS();
+  } catch (const char* e) {
+    std::strncpy(msg, e, msglen - 1);
+    msg[msglen - 1] = 0;
+    return 1;
+  }
+  return 0;
+}
+
+// CombineLU + CSMatrixRS::Ilu2 (ilu2rs.cpp:1458) of a full-storage matrix, as
+// Solver(RS) calls them (solvers.cpp:1640-1660). Factors kept for ref_fct_take.
+int ref_ilu2(int nsup, const int* sprnd, const int* ia, const int* ja, const double* a, double tau1, double tau2,
+             double theta, double sclmin, double pivmin, double memory, char* msg, int msglen) {
+  CSMatrixRS am = make(nsup, sprnd, ia, ja, a);
+  CSMatrixRS al, au, dummy;
+  am.CombineLU(al, au);
+  CSlvParam p;
+  set_fct_params(p, tau1, tau2, theta, sclmin, pivmin, memory);
+  std::ofstream fout("/dev/null");
+  try {
+    dummy.Ilu2(fout, p, al, au, g_fct[0], g_fct[1]);
+  } catch (const char* e) {
+    std::strncpy(msg, e, msglen - 1);
+    msg[msglen - 1] = 0;
+    return 1;
+  }
+  return 0;
+}
+
+// The last factor's part which (0 L, 1 U): sizes, then arrays when non-null.
+void ref_fct_take(int which, int* nzja, int* nza, int* oia, int* oja, double* oa) {
+  take(g_fct[which], nzja, nza, oia, oja, oa);
+}
+
+// dsyev_ / dgesvd_ as the factorizations call them (lapackd.cpp:865-905), for
+// pinning the dense kernels on their own: job strings are the literals the
+// callers pass, so the pointer comparison inside lapackd.cpp sees the same
+// (merged) literals as in ich2rs.cpp / ilu2rs.cpp.
+void ref_dsyev(int n, double* a, double* w, double* work, int lwork, int* info) {
+  dsyev_("V", "U", &n, a, &n, w, work, &lwork, info);
+}
+void ref_dgesvd(int n, double* a, double* s, double* u, double* vt, double* work, int lwork, int* info) {
+  dgesvd_("A", "A", &n, &n, a, &n, s, u, &n, vt, &n, work, &lwork, info);
+}
 
 void ref_mvma(int nsup, const int* sprnd, const int* ia, const int* ja, const double* a, const double* x, double* y,
               int nrhs) {

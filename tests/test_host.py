@@ -1,17 +1,16 @@
 """Host stages of the path: assembly, RCM ordering, reordering, L/U split and
 the IC2 / ILU2 factorizations.
 
-Ordering and structural transforms are compared bit-for-bit with the
-reference's own compiled code (oracle/_ref). The factorizations cannot be
-compiled from the reference here (they call Fortran LAPACK, and no Fortran
-compiler exists in the image), so they are pinned by
-* the special case that reduces them to exact block Cholesky / LU
-  (tau1 = tau2 = theta = 0 keeps every block),
-* an independent dense numpy restatement of the second-order algorithm
-  (scaling, left-looking updates without second-order x second-order
-  products, tau2 filtering with theta diagonal compensation, tau1 split),
-  compared through the preconditioner operator, which is invariant to the sign
-  choices of the eigen/singular vectors.
+Ordering, structural transforms and the factorizations are compared bit for
+bit with the reference's own compiled code (oracle/_ref: the Bslib sources
+plus the reference's LAPACK 3.0, lapacksm/*.f and lapackdm.f, compiled with
+flang). The IC2 / ILU2 factors must equal the reference's Ich2 / Ilu2 in
+pattern and in every value, and the dense DSYEV / DGESVD restatements must
+equal the reference's LAPACK on adversarial blocks. Independent pins remain
+beside them: the special case that reduces the factorizations to exact block
+Cholesky / LU (tau1 = tau2 = theta = 0 keeps every block), and a dense numpy
+restatement of the second-order algorithm compared through the
+preconditioner operator.
 """
 
 import numpy as np
@@ -106,8 +105,25 @@ def test_reorder_and_combine_lu_match_reference(B, O):
     li, lj, la = O.ref_combinelu(s["A"], 0)
     ui, uj, ua = O.ref_combinelu(s["A"], 1)
     assert np.array_equal(li, ui) and np.array_equal(lj, uj)
-    # with no dropping the factor pattern is the profile fill of exactly this pattern
     assert lj.size >= s["A"].nsup
+
+
+@pytest.mark.parametrize("kind", ["seepage-b3", "seepage-b24", "seepage-skew0"])
+def test_combine_lu_matches_reference(B, O, kind):
+    """The product's combine_lu (through bcm_combine_lu) against the
+    reference's own CombineLU (smatrixrs.cpp:1645-1803) built in oracle/_ref:
+    patterns and values bit for bit, for both halves."""
+    need_ref(O)
+    bla, skew, dims = {"seepage-b3": (3, 0.3, (4, 4, 3)), "seepage-b24": (24, 0.3, (3, 3, 2)),
+                       "seepage-skew0": (6, 0.0, (5, 3, 2))}[kind]
+    s = seepage(B, O, *dims, bla, skew, seed=17)
+    lo, up = s["mo"].combine_lu()
+    for which, mine in ((0, lo), (1, up)):
+        ri, rj, ra = O.ref_combinelu(s["A"], which)
+        sp, ia, ja, a = mine.arrays()
+        assert np.array_equal(sp, s["A"].sprnd)
+        assert np.array_equal(ia, ri) and np.array_equal(ja, rj), which
+        assert np.array_equal(a, ra), which
 
 
 def test_vector_reorder_roundtrip(B):
@@ -296,3 +312,101 @@ def test_ic2_workspace_bound(B):
     f = B.Factor.ic2(mo, B.default_params(memory=4.0))   # bounded but sufficient: same factor
     g = B.Factor.ic2(mo)
     assert np.array_equal(f.arrays(1, 64)[2], g.arrays(1, 64)[2])
+
+
+# ---- dense kernels and factors against the reference's own LAPACK 3.0 build ----
+
+def _dense_cases(n, rng):
+    """Matrices that stress the eigen/singular solvers: random, repeated
+    eigenvalues in a random basis (isotropic blocks), exactly diagonal with
+    ties, rank deficient, graded, and scaled into DLASCL's ranges."""
+    q, _ = np.linalg.qr(rng.standard_normal((n, n)))
+    g = rng.standard_normal((n, n))
+    rep = np.repeat(np.arange(1, (n + 2) // 3 + 1, dtype=float), 3)[:n]
+    cases = {
+        "random": g,
+        "repeated": (q * rep) @ q.T,
+        "diag-ties": np.diag(np.repeat([3.0, 1.0], (n + 1) // 2)[:n]),
+        "rank-def": g[:, : max(1, n // 2)] @ g[: max(1, n // 2), :],
+        "graded": g * np.logspace(0, -12, n)[None, :],
+        "tiny": g * 1e-160,
+        "huge": g * 1e160,
+        "zero": np.zeros((n, n)),
+        "identity": np.eye(n),
+    }
+    return cases
+
+
+@pytest.mark.parametrize("n", [1, 2, 3, 9, 24, 32])
+def test_dsyev_matches_reference_lapack(B, O, n):
+    """bcm_dsyev (the product's LAPACK 3.0 DSYEV restatement) against
+    dsyev_("V", "U") of the reference (lapackd.cpp -> lapackdm.f ->
+    lapacksm/dsyev.f, built with flang): eigenvalues and eigenvectors bit for
+    bit, including repeated eigenvalues, where the basis is arbitrary."""
+    need_ref(O)
+    rng = np.random.default_rng(100 + n)
+    for name, a in _dense_cases(n, rng).items():
+        s = a + a.T
+        w, z = B.dsyev(s)
+        wr, zr, info = O.ref_dsyev(s)
+        assert info == 0, name
+        assert np.array_equal(w, wr) and np.array_equal(z, zr), (n, name)
+        if name == "random":  # and it is an eigen-decomposition
+            assert np.allclose(z @ np.diag(w) @ z.T, s, atol=1e-12 * np.abs(s).max())
+
+
+@pytest.mark.parametrize("n", [1, 2, 3, 9, 24, 32])
+def test_dgesvd_matches_reference_lapack(B, O, n):
+    """bcm_dgesvd against dgesvd_("A", "A") of the reference's LAPACK 3.0:
+    singular values and both singular vector sets bit for bit."""
+    need_ref(O)
+    rng = np.random.default_rng(200 + n)
+    for name, a in _dense_cases(n, rng).items():
+        s, u, vt = B.dgesvd(a)
+        sr, ur, vtr, info = O.ref_dgesvd(a)
+        assert info == 0, name
+        assert np.array_equal(s, sr) and np.array_equal(u, ur) and np.array_equal(vt, vtr), (n, name)
+        if name == "random":
+            assert np.allclose((u * s) @ vt, a, atol=1e-12 * np.abs(a).max())
+
+
+def _fct_params(B, tau):
+    tau1, tau2 = tau
+    return B.default_params(tau1=tau1, tau2=tau2, memory=10.0)
+
+
+@pytest.mark.parametrize("kind,tau", [("heat", (1e-2, 1e-4)), ("heat", (0.1, 0.01)),
+                                      ("elastic", (1e-2, 1e-4)), ("elastic", (0.1, 0.01))])
+def test_ic2_bitwise_equals_reference(B, O, kind, tau):
+    """The product's IC2 factor (pattern and every value) equals the
+    reference's own CSMatrixRS::Ich2 (ich2rs.cpp:1075-1378) compiled with its
+    LAPACK 3.0, on the same RCM-ordered upper-stored matrix."""
+    need_ref(O)
+    if kind == "heat":
+        m, _ = B.gen_heat2d(10, 9, 9, 41)
+    else:
+        m, _ = B.gen_elastic_cell(4, 0.35, 300.0, 0.3, 0.05, 42)
+    mo = m.reorder(m.rcm_order())
+    A = O.Block(*mo.arrays())
+    f = B.Factor.ic2(mo, _fct_params(B, tau))
+    ia, ja, a = f.arrays(1, A.nsup)
+    ri, rj, ra = O.ref_ich2(A, tau1=tau[0], tau2=tau[1], memory=10.0)
+    assert np.array_equal(ia, ri) and np.array_equal(ja, rj)
+    assert np.array_equal(a, ra)
+
+
+@pytest.mark.parametrize("bla,skew,tau", [(24, 0.3, (1e-2, 1e-4)), (6, 0.3, (1e-2, 1e-4)), (9, 0.0, (0.05, 5e-3))])
+def test_ilu2_bitwise_equals_reference(B, O, bla, skew, tau):
+    """The product's ILU2 factors L (by columns) and U (by rows) equal the
+    reference's CombineLU + CSMatrixRS::Ilu2 (ilu2rs.cpp:1458-1817) with its
+    LAPACK 3.0 DGESVD, bit for bit."""
+    need_ref(O)
+    m, _ = B.gen_seepage3d(4, 4, 3, bla, skew, 43)
+    mo = m.reorder(m.rcm_order())
+    A = O.Block(*mo.arrays())
+    f = B.Factor.ilu2(mo, _fct_params(B, tau))
+    (li, lj, la), (ui, uj, ua) = O.ref_ilu2(A, tau1=tau[0], tau2=tau[1], memory=10.0)
+    for which, (ri, rj, ra) in ((0, (li, lj, la)), (1, (ui, uj, ua))):
+        ia, ja, a = f.arrays(which, A.nsup)
+        assert np.array_equal(ia, ri) and np.array_equal(ja, rj), which
+        assert np.array_equal(a, ra), which
