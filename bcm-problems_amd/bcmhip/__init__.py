@@ -147,6 +147,7 @@ SIGNATURES = {
     "bcm_solve_l": (C.c_int, [_P, _P, C.c_int, C.c_void_p, C.c_void_p, C.c_int]),
     "bcm_solve_u": (C.c_int, [_P, _P, C.c_int, C.c_void_p, C.c_void_p, C.c_int]),
     "bcm_precond": (C.c_int, [_P, _P, C.c_void_p, C.c_void_p, C.c_int, C.c_int, C.c_int]),
+    "bcm_precond_noalloc": (C.c_int, [_P, _P, C.c_void_p, C.c_void_p, C.c_int, C.c_int, C.c_int]),
     "bcm_dot": (C.c_int, [_P, C.c_int, C.c_void_p, C.c_void_p, _DP]),
     "bcm_solve": (C.c_int, [_P, _P, _P, C.POINTER(SolveOpts), C.c_void_p, C.c_void_p, C.POINTER(Stats)]),
     "bcm_solve_block": (C.c_int, [_P, _P, _P, C.POINTER(SolveOpts), C.c_int, C.c_void_p, C.c_void_p,
@@ -488,6 +489,10 @@ class Context:
 
     def precond(self, f: Factor, r, z, nrhs: int = 1, precision: int = F64, transpose: bool = False):
         _check(lib().bcm_precond(self.h, f.h, ptr(r), ptr(z), nrhs, precision, int(transpose)))
+
+    def precond_noalloc(self, f: Factor, r, z, nrhs: int = 1, precision: int = F64, transpose: bool = False):
+        """bcm_precond without device allocation (fixed 32 MiB scratch per GPU)."""
+        _check(lib().bcm_precond_noalloc(self.h, f.h, ptr(r), ptr(z), nrhs, precision, int(transpose)))
 
     def dot(self, n: int, x, y) -> float:
         out = C.c_double()

@@ -616,6 +616,15 @@ int bcm_precond(bcm_context c, bcm_factor f, const double* r, double* z, int nrh
   });
 }
 
+int bcm_precond_noalloc(bcm_context c, bcm_factor f, const double* r, double* z, int nrhs, int precision,
+                        int transpose) {
+  return guarded([&] {
+    need(c && f && r && z && nrhs > 0, "precond_noalloc: bad argument");
+    need(r != z, "precond_noalloc: r and z must not alias");
+    bcm::precond_noalloc(f->f, c->c, r, z, precision, transpose != 0, nrhs);
+  });
+}
+
 int bcm_dot(bcm_context c, int n, const double* x, const double* y, double* result) {
   return guarded([&] {
     need(c && x && y && result && n >= 0, "dot: bad argument");

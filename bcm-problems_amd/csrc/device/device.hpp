@@ -104,6 +104,7 @@ void dev_tri_backward_parts(const std::vector<const BlockCsr*>& f, int prec, Dev
 // as an ordinary launch of a grid that fits the device at once.
 struct SweepSync {
   DevBuf ctl;
+  int* ext_ctl = nullptr;  // caller-provided control words (at least 16 ints, zeroed) used instead of ctl
   int nsup = 0;
   int waves_per_cu = 8;
   bool cooperative = true;

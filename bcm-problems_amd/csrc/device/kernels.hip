@@ -1158,7 +1158,7 @@ void dev_tri_backward_parts(const std::vector<const BlockCsr*>& f, int prec, Dev
 }
 
 void SweepSync::ensure(int n, hipStream_t s) {
-  if (ctl.p) return;
+  if (ext_ctl || ctl.p) return;
   ctl.alloc(64);
   BCM_HIP(hipMemsetAsync(ctl.p, 0, 64, s));
   BCM_HIP(hipStreamSynchronize(s));
@@ -1295,7 +1295,7 @@ static void sweep_b(const DevTri& T, const void* x, void* y, hipStream_t s, cons
     const TA* off = T.off.as<TA>();
     const TA* dg = T.diag.as<TA>();
     const TX* xx = static_cast<const TX*>(x);
-    int* ct = sync->ctl.as<int>();
+    int* ct = sync->ext_ctl ? sync->ext_ctl : sync->ctl.as<int>();
     const int* ord = sync->level_order ? T.level_rows.as<int>() : nullptr;
     void* args[] = {&nsup, &n, &fwd, &ord, &ia, &ja, &off, &dg, &xx, &yy, &ct, (void*)&skip};
     if (sync->cooperative)

@@ -153,6 +153,12 @@ void solve(Matrix& A, Factor& F, Context& ctx, const SolveOptions& opt, const do
 void precond_apply(Factor& F, Context& ctx, const double* r, double* z, int prec, bool transpose, int nrhs,
                    const int* skip = nullptr);
 
+// precond_apply without any allocation (noalloc.hip): the vector between the
+// sweeps lives in a fixed per-device array of precond_noalloc_max_elems()
+// doubles; the factor image must already be on the device.
+void precond_noalloc(Factor& F, Context& ctx, const double* r, double* z, int prec, bool transpose, int nrhs);
+size_t precond_noalloc_max_elems();
+
 // z = M^-1 r through the sweep pair (t1 then t2) in the given precision
 // (fp64 vectors, or fp32 vectors converted on the way in and out).
 void precond_sweeps(const DevTri& t1, const DevTri& t2, Context& ctx, const double* r, double* z, int prec, int nrhs,

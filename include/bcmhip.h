@@ -181,6 +181,21 @@ int bcm_solve_l(bcm_context c, bcm_factor f, int which, const double* x, double*
 int bcm_solve_u(bcm_context c, bcm_factor f, int which, const double* x, double* y, int nrhs);
 /* z = M^-1 r (CParSolveLU::SolveLU, gsmatrix.h:1008-1071) in the given precision; transpose: M^-T. */
 int bcm_precond(bcm_context c, bcm_factor f, const double* r, double* z, int nrhs, int precision, int transpose);
+/* No counterpart in the reference: bcm_precond for callers that cannot
+ * allocate device memory (code under HIP stream capture, serving loops that
+ * forbid hipMalloc). Same result as bcm_precond, bit for bit; allocates
+ * nothing: the vector between the two sweeps lives in a fixed device array
+ * of 32 MiB (plus 256 bytes of control words) per GPU, kept until the
+ * process ends, so n * nrhs must not
+ * exceed 4,194,304 (BCM_E_ARG otherwise). The factor's device image for this
+ * precision and direction must exist already (one bcm_precond call with the
+ * same precision and transpose flag builds it; BCM_E_ARG otherwise). The
+ * sweeps follow the context's sweep mode; the sync-free modes keep their
+ * control words in the same array, cleared by a kernel on every call. Every
+ * call on one GPU shares the array: calls that overlap on one GPU, from any
+ * stream or context, race. r and z must not alias. */
+int bcm_precond_noalloc(bcm_context c, bcm_factor f, const double* r, double* z, int nrhs, int precision,
+                        int transpose);
 /* ScProdSVect (svector.cpp:364): result written to host memory. */
 int bcm_dot(bcm_context c, int n, const double* x, const double* y, double* result);
 
