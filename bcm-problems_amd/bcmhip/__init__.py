@@ -27,6 +27,7 @@ LIB_PATH = os.path.join(os.path.dirname(_HERE), "lib", "libbcmhip.so")
 OK = 0
 E_ARG, E_NOT_SPD, E_WORKSPACE, E_BREAKDOWN, E_HIP, E_NOCONV, E_NODEVICE, E_INTERNAL = range(1, 9)
 PCG, BICGSTAB, FGMRES, LANCZOS, CGPAR, BLOCKPCG, BLOCKGMRES = 0, 1, 2, 3, 4, 5, 6
+BLZ_CENTRAL, BLZ_RIGHT, BLZ_LEFT = 7, 8, 9
 F64, F32, BF16 = 0, 1, 2
 
 
@@ -52,7 +53,7 @@ class SolveOpts(C.Structure):
     _fields_ = [
         ("kind", C.c_int), ("niter", C.c_int), ("eps", C.c_double), ("ittype", C.c_int),
         ("check_every", C.c_int), ("restart", C.c_int), ("precision", C.c_int),
-        ("max_restarts", C.c_int),
+        ("max_restarts", C.c_int), ("ichk", C.c_int),
     ]
 
 
@@ -122,6 +123,10 @@ SIGNATURES = {
     "bcm_matrix_reorder": (C.c_int, [_P, _IP, C.POINTER(_P)]),
     "bcm_combine_lu": (C.c_int, [_P, C.POINTER(_P), C.POINTER(_P)]),
     "bcm_dsyev": (C.c_int, [C.c_int, _DP, _DP]),
+    "bcm_matrix_write_bin": (C.c_int, [_P, C.c_char_p]),
+    "bcm_matrix_read_bin": (C.c_int, [C.c_char_p, C.c_int, C.POINTER(_P)]),
+    "bcm_vector_write_bin": (C.c_int, [C.c_char_p, C.c_int, C.c_int, _DP]),
+    "bcm_vector_read_bin": (C.c_int, [C.c_char_p, _IP, _IP, _DP]),
     "bcm_dgesvd": (C.c_int, [C.c_int, _DP, _DP, _DP, _DP]),
     "bcm_vector_reorder": (C.c_int, [_P, _IP, C.c_char, _DP, C.c_int]),
     "bcm_factor_ic2": (C.c_int, [_P, C.POINTER(Params), C.POINTER(_P)]),
@@ -328,6 +333,16 @@ class Matrix:
         _check(lib().bcm_matrix_reorder(self.h, _ip(order), C.byref(out)))
         return Matrix(out.value)
 
+    def write_bin(self, path: str):
+        """CSMatrixRS::WriteMtrBin layout (smatrixrs.cpp:3210)."""
+        _check(lib().bcm_matrix_write_bin(self.h, path.encode()))
+
+    @classmethod
+    def read_bin(cls, path: str, symmetric: bool = False) -> "Matrix":
+        out = C.c_void_p()
+        _check(lib().bcm_matrix_read_bin(path.encode(), int(symmetric), C.byref(out)))
+        return cls(out.value)
+
     def combine_lu(self):
         """CombineLU (smatrixrs.cpp:1645): (L by columns, U by rows), union-padded."""
         lo, up = C.c_void_p(), C.c_void_p()
@@ -422,6 +437,22 @@ class Factor:
         a = np.zeros(i["nza"], np.float64)
         _check(lib().bcm_factor_export(self.h, which, _ip(ia), _ip(ja), _dp(a)))
         return ia, ja, a
+
+
+def vector_write_bin(path: str, v):
+    """CSVector::WriteVectBin layout (svector.cpp:537) of an n or n x nrhs array."""
+    v = np.asfortranarray(np.asarray(v, dtype=np.float64))
+    n, nrhs = v.shape[0], (1 if v.ndim == 1 else v.shape[1])
+    _check(lib().bcm_vector_write_bin(path.encode(), n, nrhs, _dp(v)))
+
+
+def vector_read_bin(path: str) -> np.ndarray:
+    """CSVector::ReadVectBin (svector.cpp:488): an n x nrhs array."""
+    n, nrhs = C.c_int(), C.c_int()
+    _check(lib().bcm_vector_read_bin(path.encode(), C.byref(n), C.byref(nrhs), None))
+    v = np.zeros((n.value, nrhs.value), order="F")
+    _check(lib().bcm_vector_read_bin(path.encode(), C.byref(n), C.byref(nrhs), _dp(v)))
+    return v
 
 
 def dsyev(a: np.ndarray):

@@ -100,6 +100,7 @@ bcm::SolveOptions to_opts(const bcm_solve_opts* o) {
   s.restart = std::max(1, o->restart);
   s.prec = o->precision;
   s.max_restarts = std::max(0, o->max_restarts);
+  s.ichk = std::max(1, o->ichk);
   return s;
 }
 
@@ -195,6 +196,7 @@ void bcm_solve_opts_default(bcm_solve_opts* o) {
   o->restart = s.restart;
   o->precision = s.prec;
   o->max_restarts = s.max_restarts;
+  o->ichk = s.ichk;
 }
 
 int bcm_matrix_create(int nsup, const int* sprnd, const int* ia, const int* ja, const double* a, int symmetric,
@@ -327,6 +329,45 @@ int bcm_matrix_reorder(bcm_matrix m, const int* order, bcm_matrix* out) {
       throw;
     }
     *out = h;
+  });
+}
+
+int bcm_matrix_write_bin(bcm_matrix m, const char* path) {
+  return guarded([&] {
+    need(m && path, "matrix_write_bin: bad argument");
+    bcm::write_mtr_bin(m->m.h, path);
+  });
+}
+
+int bcm_matrix_read_bin(const char* path, int symmetric, bcm_matrix* out) {
+  return guarded([&] {
+    need(path && out, "matrix_read_bin: bad argument");
+    auto* h = new bcm_matrix_s;
+    try {
+      h->m.h = bcm::read_mtr_bin(path);
+      h->m.symmetric = symmetric != 0;
+      h->m.h.validate(symmetric != 0);
+    } catch (...) {
+      delete h;
+      throw;
+    }
+    *out = h;
+  });
+}
+
+int bcm_vector_write_bin(const char* path, int n, int nrhs, const double* v) {
+  return guarded([&] {
+    need(path && n >= 0 && nrhs >= 1 && (v || n == 0), "vector_write_bin: bad argument");
+    bcm::write_vect_bin(path, n, nrhs, v);
+  });
+}
+
+int bcm_vector_read_bin(const char* path, int* n, int* nrhs, double* v) {
+  return guarded([&] {
+    need(path && n && nrhs, "vector_read_bin: bad argument");
+    std::vector<double> buf;
+    bcm::read_vect_bin(path, n, nrhs, v ? &buf : nullptr);
+    if (v && !buf.empty()) std::memcpy(v, buf.data(), buf.size() * sizeof(double));
   });
 }
 
@@ -655,10 +696,12 @@ int bcm_solve_block(bcm_context c, bcm_matrix a, bcm_factor f, const bcm_solve_o
     need(c && a && f && b && x, "solve_block: bad argument");
     need(nrhs >= 1 && nrhs <= 8, "solve_block: 1..8 right-hand sides");
     bcm::SolveOptions opt = to_opts(o);
-    opt.kind = bcm::kBlockGmres;
+    const int v = bcm::blz_variant(opt.kind);
+    if (v < 0) opt.kind = bcm::kBlockGmres;
     bcm::SolveStats s;
     auto t0 = std::chrono::steady_clock::now();
-    bcm::solve_block_gmres(a->m, f->f, c->c, opt, nrhs, b, x, s, resi, hist);
+    if (v >= 0) bcm::solve_block_lanczos(a->m, f->f, c->c, opt, v, nrhs, b, x, s, resi);
+    else bcm::solve_block_gmres(a->m, f->f, c->c, opt, nrhs, b, x, s, resi, hist);
     BCM_HIP(hipStreamSynchronize(c->c.stream));
     c->c.check_sweep_error();
     s.seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();

@@ -54,12 +54,18 @@ struct DevBuf {
   template <class T> T* as() const { return static_cast<T*>(p); }
 };
 
+// b == 0: any block partition (the sizes per block row in sp, the value
+// offset of every block in bo, nval values); otherwise every block is b x b.
 struct DevBsr {
   int nsup = 0, n = 0, b = 0, nzja = 0;
   int prec = kF64;
   DevBuf ia, ja, a;
-  int64_t value_bytes() const { return (int64_t)nzja * b * b * (int64_t)prec_bytes(prec); }
-  int64_t index_bytes() const { return 4LL * (nzja + nsup + 1); }
+  DevBuf sp, bo;
+  int64_t nval = 0;
+  int64_t value_bytes() const {
+    return (b ? (int64_t)nzja * b * b : nval) * (int64_t)prec_bytes(prec);
+  }
+  int64_t index_bytes() const { return 4LL * (nzja + nsup + 1) + (b ? 0 : 8LL * nzja + 4LL * (nsup + 1)); }
 };
 
 // Builds a device block-CSR from host storage (uniform block size required).
@@ -74,7 +80,7 @@ bool dev_bsr_upload_symm_parts(const std::vector<const BlockCsr*>& uppers, DevBs
                                std::vector<int64_t>& full_blocks);
 
 struct DevTri {
-  int nsup = 0, n = 0, b = 0, ndep = 0;
+  int nsup = 0, n = 0, b = 0, ndep = 0;  // b == 0: any block partition (sp, doff, dgo, nval)
   bool forward = true;
   int prec = kF64;
   int levels = 0;
@@ -83,9 +89,17 @@ struct DevTri {
   DevBuf ia, ja;      // dependency lists per row (row index space of the sweep)
   DevBuf off;         // ndep blocks, row after row
   DevBuf diag;        // nsup inverted diagonal blocks
-  int64_t value_bytes() const { return ((int64_t)ndep + nsup) * b * b * (int64_t)prec_bytes(prec); }
-  int64_t index_bytes() const { return 4LL * (ndep + nsup + 1); }
+  DevBuf sp, doff, dgo;  // variable blocks: partition, value offsets of dependency and diagonal blocks
+  int64_t nval = 0;
+  int64_t value_bytes() const {
+    return (b ? ((int64_t)ndep + nsup) * b * b : nval) * (int64_t)prec_bytes(prec);
+  }
+  int64_t index_bytes() const { return 4LL * (ndep + nsup + 1) + (b ? 0 : 8LL * (ndep + nsup) + 4LL * (nsup + 1)); }
 };
+
+// Block sizes with tuned kernels; matrices and factors with other sizes, or
+// with more than one size, take the variable-block kernels.
+bool fast_block_size(int b);
 
 // Forward sweep through a factor stored by columns (reference SolveL):
 // y_k = D_k^T (x_k - sum_{i<k} F_ik^T y_i).

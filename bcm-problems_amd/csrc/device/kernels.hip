@@ -884,6 +884,252 @@ static void upload_values_at(const double* v, size_t count, int prec, void* dst)
   stage_give(st);
 }
 
+// ---------------------------------------------------------------------------
+// Any block partition (CSPRNDS gives one size per block row; blocks up to 64
+// rows): products and sweeps with the sizes read per row, the path for block
+// sizes the tuned kernels above are not instantiated for and for variable
+// partitions. One 64-lane wave per block row; lane = g * b_i + r with
+// G = 64 / b_i lane groups, group g taking the block columns c = g, g + G, ...
+// of every block of the row in ja order; the G partial sums of a row element
+// are folded in group order, so results do not depend on the launch shape.
+
+// y = A x or y = bsub - A x over nrhs columns (leading dimension n).
+__global__ __launch_bounds__(256) void k_spmv_var(int nsup, int64_t n, const int* __restrict__ sp,
+                                                  const int* __restrict__ ia, const int* __restrict__ ja,
+                                                  const int64_t* __restrict__ bo, const double* __restrict__ a,
+                                                  const double* __restrict__ x, double* __restrict__ y,
+                                                  const double* __restrict__ bsub, int nrhs, const int* skip) {
+  if (skip && *skip) return;
+  __shared__ double red[4][64];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int row = blockIdx.x * 4 + wave;
+  if (row >= nsup) return;
+  const int r0 = sp[row], bi = sp[row + 1] - r0;
+  const int G = 64 / bi;
+  const int g = lane / bi, r = lane - g * bi;
+  const bool act = g < G;
+  for (int q = 0; q < nrhs; ++q) {
+    const double* xq = x + (size_t)q * n;
+    double acc = 0.0;
+    if (act) {
+      for (int j = ia[row]; j < ia[row + 1]; ++j) {
+        const int col = ja[j];
+        const int c0 = sp[col], bj = sp[col + 1] - c0;
+        const double* blk = a + bo[j];
+        for (int c = g; c < bj; c += G) acc += blk[(size_t)c * bi + r] * xq[c0 + c];
+      }
+    }
+    red[wave][lane] = acc;
+    wave_lds_sync();
+    if (lane < bi) {
+      double s = 0.0;
+      for (int h = 0; h < G; ++h) s += red[wave][h * bi + lane];
+      const size_t o = (size_t)q * n + r0 + lane;
+      y[o] = bsub ? bsub[o] - s : s;
+    }
+    wave_lds_sync();
+  }
+}
+
+// One dependency level of a sweep: y_k = D_k (x_k - sum_i G_ki y_i) for the
+// rows k of the level, every block stored with b_k rows (column-major).
+template <class TA, class TX>
+__global__ __launch_bounds__(256) void k_sweep_var(int nrows, const int* __restrict__ rows, int64_t n,
+                                                   const int* __restrict__ sp, const int* __restrict__ ia,
+                                                   const int* __restrict__ ja, const int64_t* __restrict__ doff,
+                                                   const TA* __restrict__ off, const int64_t* __restrict__ dgo,
+                                                   const TA* __restrict__ diag, const TX* __restrict__ x,
+                                                   TX* __restrict__ y, int nrhs, const int* skip) {
+  if (skip && *skip) return;
+  using Acc = typename AccOf<TX>::type;
+  __shared__ Acc red[4][64];
+  __shared__ Acc vs[4][64];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int t = blockIdx.x * 4 + wave;
+  if (t >= nrows) return;
+  const int k = rows[t];
+  const int r0 = sp[k], bk = sp[k + 1] - r0;
+  const int G = 64 / bk;
+  const int g = lane / bk, r = lane - g * bk;
+  const bool act = g < G;
+  const TA* d = diag + dgo[k];
+  for (int q = 0; q < nrhs; ++q) {
+    const TX* xq = x + (size_t)q * n;
+    TX* yq = y + (size_t)q * n;
+    Acc acc = Acc(0);
+    if (act) {
+      for (int j = ia[k]; j < ia[k + 1]; ++j) {
+        const int i = ja[j];
+        const int c0 = sp[i], bi = sp[i + 1] - c0;
+        const TA* blk = off + doff[j];
+        for (int c = g; c < bi; c += G) acc += (Acc)ldv(blk + (size_t)c * bk + r) * (Acc)yq[c0 + c];
+      }
+    }
+    red[wave][lane] = acc;
+    wave_lds_sync();
+    if (lane < bk) {
+      Acc s = Acc(0);
+      for (int h = 0; h < G; ++h) s += red[wave][h * bk + lane];
+      vs[wave][lane] = (Acc)xq[r0 + lane] - s;
+    }
+    wave_lds_sync();
+    acc = Acc(0);
+    if (act)
+      for (int c = g; c < bk; c += G) acc += (Acc)ldv(d + (size_t)c * bk + r) * vs[wave][c];
+    red[wave][lane] = acc;
+    wave_lds_sync();
+    if (lane < bk) {
+      Acc s = Acc(0);
+      for (int h = 0; h < G; ++h) s += red[wave][h * bk + lane];
+      yq[r0 + lane] = (TX)s;
+    }
+    wave_lds_sync();
+  }
+}
+
+// Device image of a matrix with any block partition (values fp64, as stored).
+static void dev_bsr_upload_var(const BlockCsr& h, DevBsr& d) {
+  for (int i = 0; i < h.nsup; ++i)
+    if (h.bla(i) < 1 || h.bla(i) > 64) throw Error(kErrArg, "device matrices: blocks of 1..64 rows");
+  d.nsup = h.nsup;
+  d.n = h.n();
+  d.b = 0;
+  d.nzja = h.nzja();
+  d.prec = kF64;
+  d.nval = h.nza();
+  d.ia.upload(h.ia.data(), h.ia.size() * 4);
+  d.ja.upload(h.ja.data(), std::max<size_t>(h.ja.size(), 1) * 4);
+  d.sp.upload(h.sprnd.data(), h.sprnd.size() * 4);
+  d.bo.upload(h.bsa.data(), std::max<size_t>(h.bsa.size(), 1) * 8);
+  d.a.alloc(std::max<size_t>(h.a.size(), 1) * 8);
+  upload_values_at(h.a.data(), h.a.size(), kF64, d.a.p);
+}
+
+// Sweep image for any block partition: the same dependency lists and levels
+// as the uniform builder, blocks of row k stored with b_k rows.
+static void build_tri_var(const BlockCsr& f, bool forward, int prec, DevTri& t) {
+  const int ns = f.nsup;
+  for (int i = 0; i < ns; ++i)
+    if (f.bla(i) < 1 || f.bla(i) > 64) throw Error(kErrArg, "device factors: blocks of 1..64 rows");
+  t.nsup = ns;
+  t.n = f.n();
+  t.b = 0;
+  t.forward = forward;
+  t.prec = prec;
+  t.ndep = f.nzja() - ns;
+  std::vector<int> ia(ns + 1, 0), ja;
+  std::vector<int64_t> doff, dgo(ns);
+  std::vector<double> off, diag;
+  int64_t dpos = 0;
+  for (int i = 0; i < ns; ++i) {
+    if (f.ia[i] == f.ia[i + 1] || f.ja[f.ia[i]] != i) throw Error(kErrArg, "factor rows must start with the diagonal block");
+    const int b = f.bla(i);
+    const double* d = &f.a[f.bsa[f.ia[i]]];
+    dgo[i] = dpos;
+    for (int c = 0; c < b; ++c)
+      for (int r = 0; r < b; ++r) diag.push_back(forward ? d[r * b + c] : d[c * b + r]);
+    dpos += (int64_t)b * b;
+  }
+  if (!forward) {
+    for (int i = 0; i < ns; ++i) {
+      for (int j = f.ia[i] + 1; j < f.ia[i + 1]; ++j) {
+        ja.push_back(f.ja[j]);
+        doff.push_back((int64_t)off.size());
+        const size_t sz = (size_t)f.bla(i) * f.bla(f.ja[j]);
+        off.insert(off.end(), f.a.begin() + f.bsa[j], f.a.begin() + f.bsa[j] + sz);
+      }
+      ia[i + 1] = (int)ja.size();
+    }
+  } else {
+    // row k gathers G_ki = F_ik^T (b_k x b_i) from the factor stored by columns
+    std::vector<int> cnt(ns + 1, 0);
+    for (int i = 0; i < ns; ++i)
+      for (int j = f.ia[i] + 1; j < f.ia[i + 1]; ++j) cnt[f.ja[j] + 1]++;
+    for (int k = 0; k < ns; ++k) ia[k + 1] = ia[k] + cnt[k + 1];
+    ja.resize(ia[ns]);
+    std::vector<std::vector<int>> src(ns);
+    for (int i = 0; i < ns; ++i)
+      for (int j = f.ia[i] + 1; j < f.ia[i + 1]; ++j) src[f.ja[j]].push_back(j);
+    doff.resize(ia[ns]);
+    for (int k = 0; k < ns; ++k) {
+      const int bk = f.bla(k);
+      int p = ia[k];
+      for (int j : src[k]) {
+        const int i = std::upper_bound(f.ia.begin(), f.ia.end(), j) - f.ia.begin() - 1;
+        const int bi = f.bla(i);
+        ja[p] = i;
+        doff[p] = (int64_t)off.size();
+        const double* s = &f.a[f.bsa[j]];  // F_ik: b_i x b_k
+        for (int c = 0; c < bi; ++c)
+          for (int r = 0; r < bk; ++r) off.push_back(s[r * bi + c]);
+        ++p;
+      }
+    }
+  }
+  std::vector<int> lev(ns, 0);
+  int mx = 0;
+  for (int s = 0; s < ns; ++s) {
+    const int row = forward ? s : ns - 1 - s;
+    int l = 0;
+    for (int j = ia[row]; j < ia[row + 1]; ++j) l = std::max(l, lev[ja[j]] + 1);
+    lev[row] = l;
+    mx = std::max(mx, l);
+  }
+  t.levels = ns ? mx + 1 : 0;
+  t.level_ptr.assign(t.levels + 1, 0);
+  for (int i = 0; i < ns; ++i) t.level_ptr[lev[i] + 1]++;
+  for (int l = 0; l < t.levels; ++l) t.level_ptr[l + 1] += t.level_ptr[l];
+  std::vector<int> rows(ns), fill(t.level_ptr.begin(), t.level_ptr.end() - 1);
+  for (int s = 0; s < ns; ++s) {
+    const int row = forward ? s : ns - 1 - s;
+    rows[fill[lev[row]]++] = row;
+  }
+  const size_t es = prec_bytes(prec);
+  t.nval = (int64_t)off.size() + (int64_t)diag.size();
+  t.off.alloc(std::max<size_t>(off.size() * es, es));
+  t.diag.alloc(std::max<size_t>(diag.size() * es, es));
+  upload_values_at(off.data(), off.size(), prec, t.off.p);
+  upload_values_at(diag.data(), diag.size(), prec, t.diag.p);
+  t.level_rows.upload(rows.data(), std::max<size_t>(rows.size(), 1) * 4);
+  t.ia.upload(ia.data(), ia.size() * 4);
+  t.ja.upload(ja.data(), std::max<size_t>(ja.size(), 1) * 4);
+  t.sp.upload(f.sprnd.data(), f.sprnd.size() * 4);
+  t.doff.upload(doff.data(), std::max<size_t>(doff.size(), 1) * 8);
+  t.dgo.upload(dgo.data(), dgo.size() * 8);
+}
+
+template <class TA, class TX>
+static void sweep_var_levels(const DevTri& T, const void* x, void* y, int nrhs, hipStream_t s, const int* skip) {
+  const int* rows = T.level_rows.as<int>();
+  for (int l = 0; l < T.levels; ++l) {
+    const int r0 = T.level_ptr[l], nr = T.level_ptr[l + 1] - r0;
+    hipLaunchKernelGGL((k_sweep_var<TA, TX>), dim3((nr + 3) / 4), dim3(256), 0, s, nr, rows + r0, (int64_t)T.n,
+                       T.sp.as<int>(), T.ia.as<int>(), T.ja.as<int>(), T.doff.as<int64_t>(), T.off.as<TA>(),
+                       T.dgo.as<int64_t>(), T.diag.as<TA>(), static_cast<const TX*>(x), static_cast<TX*>(y), nrhs,
+                       skip);
+  }
+  BCM_HIP(hipGetLastError());
+}
+
+static void launch_sweep_var(const DevTri& T, const void* x, void* y, int vec_prec, int nrhs, hipStream_t s,
+                             const int* skip) {
+  if (T.prec == kF64 && vec_prec == kF64) sweep_var_levels<double, double>(T, x, y, nrhs, s, skip);
+  else if (T.prec == kF32 && vec_prec == kF32) sweep_var_levels<float, float>(T, x, y, nrhs, s, skip);
+  else if (T.prec == kBF16 && vec_prec == kF32) sweep_var_levels<uint16_t, float>(T, x, y, nrhs, s, skip);
+  else throw Error(kErrArg, "unsupported factor/vector precision pair (fp64/fp64, fp32/fp32, bf16/fp32)");
+}
+
+static void launch_spmv_var(const DevBsr& A, const double* x, double* y, const double* bsub, int nrhs,
+                            hipStream_t s, const int* skip) {
+  hipLaunchKernelGGL(k_spmv_var, dim3((A.nsup + 3) / 4), dim3(256), 0, s, A.nsup, (int64_t)A.n, A.sp.as<int>(),
+                     A.ia.as<int>(), A.ja.as<int>(), A.bo.as<int64_t>(), A.a.as<double>(), x, y, bsub, nrhs, skip);
+  BCM_HIP(hipGetLastError());
+}
+
+bool fast_block_size(int b) {
+  return b == 3 || b == 6 || b == 8 || b == 9 || b == 12 || b == 16 || b == 24 || b == 32;
+}
+
 static int uniform_block(const BlockCsr& h) {
   const int b = h.nsup ? h.bla(0) : 0;
   for (int i = 0; i < h.nsup; ++i)
@@ -934,7 +1180,14 @@ void dev_bsr_upload_parts(const std::vector<const BlockCsr*>& parts, int prec, D
   d.ja.upload(ja.data(), ja.size() * 4);
 }
 
-void dev_bsr_upload(const BlockCsr& h, int prec, DevBsr& d) { dev_bsr_upload_parts({&h}, prec, d); }
+void dev_bsr_upload(const BlockCsr& h, int prec, DevBsr& d) {
+  if (!fast_block_size(uniform_block(h))) {
+    if (prec != kF64) throw Error(kErrArg, "device matrices of variable or untuned block sizes are kept in fp64");
+    dev_bsr_upload_var(h, d);
+    return;
+  }
+  dev_bsr_upload_parts({&h}, prec, d);
+}
 
 // Full storage of a symmetric matrix from its upper blocks: full block k is
 // upper block src[k] (src[k] >= 0) or the transpose of upper block ~src[k]
@@ -1148,8 +1401,14 @@ static void build_tri_parts(const std::vector<const BlockCsr*>& parts, bool forw
   t.ja.upload(ja_all.data(), ja_all.size() * 4);
 }
 
-void dev_tri_forward(const BlockCsr& f, int prec, DevTri& t) { build_tri_parts({&f}, true, prec, t); }
-void dev_tri_backward(const BlockCsr& f, int prec, DevTri& t) { build_tri_parts({&f}, false, prec, t); }
+void dev_tri_forward(const BlockCsr& f, int prec, DevTri& t) {
+  if (!fast_block_size(uniform_block(f))) build_tri_var(f, true, prec, t);
+  else build_tri_parts({&f}, true, prec, t);
+}
+void dev_tri_backward(const BlockCsr& f, int prec, DevTri& t) {
+  if (!fast_block_size(uniform_block(f))) build_tri_var(f, false, prec, t);
+  else build_tri_parts({&f}, false, prec, t);
+}
 void dev_tri_forward_parts(const std::vector<const BlockCsr*>& f, int prec, DevTri& t) {
   build_tri_parts(f, true, prec, t);
 }
@@ -1259,6 +1518,16 @@ static void spmv_nr(const DevBsr& A, const double* x, double* y, const double* b
 
 void launch_spmv(const DevBsr& A, const double* x, double* y, const double* bsub, int nrhs, double* dot_out,
                  const double* wdot, RedWork& rw, hipStream_t s, const int* skip, RowGate gate) {
+  if (A.b == 0) {
+    if (gate.active) throw Error(kErrArg, "row-gated products need a uniform tuned block size");
+    launch_spmv_var(A, x, y, bsub, nrhs, s, skip);
+    if (dot_out) {  // the fused dot of the first column, as the tuned kernels leave it
+      const double* xs[1] = {wdot};
+      const double* ys[1] = {y};
+      launch_multidot(A.n, 1, xs, ys, dot_out, rw, s, skip);
+    }
+    return;
+  }
   switch (A.b) {
     case 3: spmv_nr<3>(A, x, y, bsub, nrhs, dot_out, wdot, rw, s, skip, gate); break;
     case 6: spmv_nr<6>(A, x, y, bsub, nrhs, dot_out, wdot, rw, s, skip, gate); break;
@@ -1342,6 +1611,11 @@ static void sweep_nr(const DevTri& T, const void* x, void* y, int vec_prec, int 
 void launch_sweep(const DevTri& T, const void* x, void* y, int vec_prec, int nrhs, hipStream_t s,
                   const int* skip, SweepSync* sync, RowGate gate) {
   if (x == y) throw Error(kErrArg, "triangular sweep: input and output must not alias");
+  if (T.b == 0) {  // variable blocks: level by level whatever the schedule
+    if (gate.active) throw Error(kErrArg, "row-gated sweeps need a uniform tuned block size");
+    launch_sweep_var(T, x, y, vec_prec, nrhs, s, skip);
+    return;
+  }
   if (sync) sync->ensure(T.nsup, s);
   switch (T.b) {
     case 3: sweep_nr<3>(T, x, y, vec_prec, nrhs, s, skip, sync, gate); break;

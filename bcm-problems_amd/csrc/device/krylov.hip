@@ -332,8 +332,11 @@ void set_pending(Context& ctx, int blocks, int k) {
 // copied and expanded on the device; on the host only if a block lies below
 // the diagonal.
 static void upload_symmetric(const BlockCsr& upper, DevBsr& d) {
+  bool uniform = upper.nsup > 0;
+  for (int i = 0; i < upper.nsup && uniform; ++i) uniform = upper.bla(i) == upper.bla(0);
   std::vector<int64_t> full_blocks;
-  if (!dev_bsr_upload_symm_parts({&upper}, d, full_blocks)) dev_bsr_upload(symm_mtr(upper), kF64, d);
+  if (!(uniform && fast_block_size(upper.bla(0))) || !dev_bsr_upload_symm_parts({&upper}, d, full_blocks))
+    dev_bsr_upload(symm_mtr(upper), kF64, d);
 }
 
 void Matrix::to_device(bool with_transpose) {
@@ -778,7 +781,7 @@ void solve(Matrix& A, Factor& F, Context& ctx, const SolveOptions& opt, const do
            SolveStats& st) {
   auto t0 = std::chrono::steady_clock::now();
   st = SolveStats();
-  A.to_device(opt.kind == kLanczos);
+  A.to_device(opt.kind == kLanczos || blz_variant(opt.kind) >= 0);
   if (A.d.n != F.u.n()) throw Error(kErrArg, "matrix and factor sizes differ");
   auto run = [&] {
     switch (opt.kind) {
@@ -788,6 +791,9 @@ void solve(Matrix& A, Factor& F, Context& ctx, const SolveOptions& opt, const do
       case kFgmres: solve_fgmres(A, F, ctx, opt, b, x, st); break;
       case kLanczos: solve_lanczos(A, F, ctx, opt, b, x, st); break;
       case kBlockGmres: solve_block_gmres(A, F, ctx, opt, 1, b, x, st); break;
+      case kBlzCentral:
+      case kBlzRight:
+      case kBlzLeft: solve_block_lanczos(A, F, ctx, opt, blz_variant(opt.kind), 1, b, x, st); break;
       default: throw Error(kErrArg, "solver kind not available");
     }
     BCM_HIP(hipStreamSynchronize(ctx.stream));

@@ -120,7 +120,15 @@ enum SolverKind : int {
   kCgPar = 4,        // reference parallel-form Cg (cg.cpp:355-771)
   kBlockPcg = 5,     // block PCG over several right-hand sides
   kBlockGmres = 6,   // reference block GMRES (gmres.cpp:337-1038)
+  kBlzCentral = 7,   // reference block Lanczos, central preconditioning (lanczos.cpp:2467-3225)
+  kBlzRight = 8,     // right preconditioning (lanczos.cpp:932-1718)
+  kBlzLeft = 9,      // left preconditioning (lanczos.cpp:1719-2466)
 };
+
+// Block Lanczos variant of a solver kind: 0 Right, 1 Left, 2 Central, -1 none.
+inline int blz_variant(int kind) {
+  return kind == kBlzRight ? 0 : kind == kBlzLeft ? 1 : kind == kBlzCentral ? 2 : -1;
+}
 
 struct SolveOptions {
   int kind = kPcg;
@@ -131,6 +139,7 @@ struct SolveOptions {
   int restart = 30;      // FGMRES cycle length
   int prec = kF64;       // precision of the preconditioner application
   int max_restarts = 3;  // true-residual restarts when the recursion drifted
+  int ichk = 5;          // block Lanczos: iterations between residual tests (slvparam.h:174)
 };
 
 struct SolveStats {
@@ -231,6 +240,12 @@ BatchTune batch_tune(Batch& bt, Context& ctx, const SolveOptions& opt, int nrhs,
 // Two-sided Lanczos driver (lanczos.hip).
 void solve_lanczos(Matrix& A, Factor& F, Context& ctx, const SolveOptions& opt, const double* b, double* x,
                    SolveStats& st);
+
+// Block Lanczos driver (blanczos.hip), variant 0 Right, 1 Left, 2 Central,
+// over nrhs (1..8) right-hand sides (n x nrhs column-major, x holds x0);
+// resi (host, nrhs, optional) receives ||b_c - A x_c||.
+void solve_block_lanczos(Matrix& A, Factor& F, Context& ctx, const SolveOptions& opt, int variant, int nrhs,
+                         const double* b, double* x, SolveStats& st, double* resi = nullptr);
 
 // Block GMRES driver (bgmres.hip) over nrhs (1..8) right-hand sides: b and x
 // are n x nrhs column-major, x holds x0; opt.niter is the number of block

@@ -147,11 +147,14 @@ void pick_solver(bool symmetric, bool par, const bcm_params& prm, bcm::SolveOpti
     return;
   }
   // Solver(RS) (solvers.cpp:1710-1800): ittype 1 two-sided Lanczos per
-  // right-hand side; 2, 3, 4 the block Lanczos variants Central, Right, Left.
-  if (prm.ittype != 1)
-    throw bcm::Error(bcm::kErrArg,
-                     "AbstractSolver: only ittype 1 (two-sided Lanczos) is available; the block Lanczos variants "
-                     "of ittype 2-4 are not");
+  // right-hand side; 2, 3, 4 the block Lanczos variants Central, Right, Left
+  // (relative residual test every ichk iterations, lanczos.cpp:1536-1585).
+  if (prm.ittype >= 2 && prm.ittype <= 4) {
+    opt.kind = prm.ittype == 2 ? bcm::kBlzCentral : prm.ittype == 3 ? bcm::kBlzRight : bcm::kBlzLeft;
+    opt.ichk = std::max(1, prm.ichk);
+    return;
+  }
+  if (prm.ittype != 1) throw bcm::Error(bcm::kErrArg, "AbstractSolver: ittype must be 1 (Lanczos) or 2-4 (block Lanczos)");
   opt.kind = bcm::kLanczos;
   // Lanczos sttype (lanczos.cpp:263): 0 absolute, 1 relative to ||r0||, 2 max-norm.
   opt.ittype = prm.sttype == 1 ? 0 : prm.sttype == 2 ? 2 : 1;

@@ -104,6 +104,17 @@ void ord_vect(char type, const std::vector<int>& sprnd_old, const std::vector<in
 // Upper-triangular part (j >= i) of a full matrix.
 BlockCsr upper_part(const BlockCsr& a);
 
+// ---- binary files in the reference's format (binio.cpp) ----
+
+// CSMatrixRS::WriteMtrBin (smatrixrs.cpp:3210-3227) layout.
+void write_mtr_bin(const BlockCsr& a, const std::string& path);
+// Reads that layout back (square block matrices, identity row list, blocks in ja order).
+BlockCsr read_mtr_bin(const std::string& path);
+// CSVector::WriteVectBin (svector.cpp:537-546) of an n x nrhs vector (one part).
+void write_vect_bin(const std::string& path, int n, int nrhs, const double* v);
+// CSVector::ReadVectBin (svector.cpp:488-507): sizes, and the values when v != null.
+void read_vect_bin(const std::string& path, int* n, int* nrhs, std::vector<double>* v);
+
 // ---- small dense kernels (lapack30.cpp, dense_small.cpp) ----
 
 namespace lapack30 {

@@ -8,8 +8,17 @@ spherical inclusion with radius U[0.1, 0.45], stiffness contrast
 macroscopic-strain right-hand sides each, IC2 factors (tau1 = 0.1,
 tau2 = 0.01) computed on the host. The first 1024 cells are solved from
 x0 = 0 (batched block PCG, FP32 factors, FP64 outer recurrence) and train
-the initial-guess model (128 POD modes per right-hand side, MLP 4 -> 256 ->
-256 -> 768 trained on the GPU with BF16 MFMA kernels). Evaluation cells
+the initial-guess model (64 POD modes per right-hand side, MLP 4 -> 256 ->
+256 -> 384 trained on the GPU with BF16 MFMA kernels).
+
+Why IC2(0.1, 0.01) and not the reference's defaults (1e-2, 1e-4,
+slvparam.h:146-147): on a 16^3 cell the defaults cost 9 s of host
+factorization per cell instead of 1 s (2.5x the fill, 806 dependency levels
+per sweep instead of 25), so the 1536 cells of the default run would take
+about 15 minutes of host time. The run therefore also solves a sample of
+cells at the reference's thresholds (`--ref-tau-cells`, batched from x0 = 0
+against the plain FP64 path on the same cells) and reports it beside the
+headline. Evaluation cells
 are then solved in groups of `--group` cells (256 by default) x 6
 right-hand sides: MLP inference,
 POD reconstruction and one batched block-PCG launch sequence per group
@@ -227,6 +236,7 @@ def run_learned(args, B):
     evalg = [list(range(ntrain + k * group, ntrain + (k + 1) * group)) for k in range(args.warmup + args.steps)]
     th = threading.Thread(target=prefetch, args=(groups[0], fmask))
     th.start()
+    t_train0 = time.perf_counter()
     for gi, idx in enumerate(groups):
         bt, db, cells = take()
         nxt = groups[gi + 1] if gi + 1 < len(groups) else evalg[0]
@@ -253,7 +263,9 @@ def run_learned(args, B):
             log(f"training group {gi + 1}/{len(groups)}: {st.seconds * 1e3:.0f} ms, mean iterations {its.mean():.1f}, "
                 f"host cells ready {time.perf_counter() - t_setup0:.0f}s")
         del bt, db, dx, cells
+    t_train1 = time.perf_counter()
     rep = guess.fit(ctx, steps=args.fit_steps, lr=args.fit_lr)
+    t_fit1 = time.perf_counter()
     log(f"guess fit: POD energy kept {rep.pod_energy:.6f}, loss {rep.loss_first:.3e} -> {rep.loss_final:.3e}, "
         f"POD {rep.seconds_pod:.2f}s, MLP {rep.seconds_train:.2f}s")
 
@@ -261,6 +273,8 @@ def run_learned(args, B):
     timed_s, timed_pairs, its_learned, its_zero, relres_max, ok = 0.0, 0, [], [], 0.0, True
     prof = None
     plain_cells = []
+    guess_err = None
+    t_eval0 = time.perf_counter()
     for gi, idx in enumerate(evalg):
         bt, db, cells = take()
         bt.set_tune(**tune)
@@ -288,10 +302,14 @@ def run_learned(args, B):
             ctx.profile_read(reset=True)
             bt.set_tune(**dict(tune, graph=0))
             guess.predict(ctx, params[idx], dz)
+            ctx.sync()
+            x0h = dz.host()
             bt.solve(ctx, db, dz, opts, 6, True)
             ctx.sync()
             prof = ctx.profile_read()
             ctx.profile(False)
+            guess_err = guess_error(bt, x0h, dz.host())
+            del x0h
             plain_cells = cells[:args.plain_cells]
             log(f"group {gi}: zero guess {itz.mean():.1f} iterations ({stz.seconds * 1e3:.0f} ms), learned "
                 f"{its.mean():.1f} iterations ({st.seconds * 1e3:.0f} ms)")
@@ -304,7 +322,12 @@ def run_learned(args, B):
         for i in idx[args.plain_cells:] if gi == args.warmup else idx:
             fac.release(i)
         del bt, db, dx
+    t_eval1 = time.perf_counter()
+    eval_pairs = len(evalg) * group * 6
     fac.close()
+    ref_tau = reference_thresholds_sample(args, B, ctx, params[ntrain + len(evalg) * group - args.ref_tau_cells:
+                                                             ntrain + len(evalg) * group], tune) \
+        if args.ref_tau_cells > 0 else None
 
     # plain FP64 drop-in on a sample of the evaluation cells: one right-hand side at a time
     popts = B.default_opts(kind=B.PCG, eps=EPS, ittype=3, check_every=8, niter=5000)
@@ -341,8 +364,10 @@ def run_learned(args, B):
         "sweeps (each one launch per dependency level)"
     value = timed_pairs / timed_s
     cpu = None
+    t_cpu0 = time.perf_counter()
     if not args.no_cpu:
-        cpu = cpu_baseline_ref(B, plain_cells, np.mean(plain_its))
+        cpu = cpu_baseline_ref(B, plain_cells, np.mean(plain_its), budget_s=20.0, cores=host_threads())
+    t_cpu1 = time.perf_counter()
     line = {
         "metric": METRIC,
         "value": value,
@@ -372,6 +397,18 @@ def run_learned(args, B):
         "speedup_vs_plain": value / plain_rate,
         "iterations": {"learned_mean": float(np.mean(its_learned)), "zero_guess_mean": float(np.mean(its_zero)),
                        "training_mean": float(np.mean(train_iters))},
+        "guess_error": guess_err,
+        "end_to_end": {
+            "evaluation": {"value": eval_pairs / (t_eval1 - t_eval0), "unit": "solves/s",
+                           "what": f"{len(evalg)} evaluation groups ({eval_pairs} solves, warm-up included), wall "
+                                   "time from the first group's hand-over to the last solve: host cell generation, "
+                                   "IC2 and batch build of the later groups overlap the solves, except that host "
+                                   "preparation pauses while a solve is timed"},
+            "training": {"value": train_pairs / (t_train1 - t_train0), "unit": "solves/s",
+                         "what": "the training groups from x0 = 0, host preparation of the next group "
+                                 "overlapping each solve (autotuning of the first group included)"},
+        },
+        "reference_thresholds": ref_tau,
         "relres_true_max": relres_max,
         "batched_zero_guess_training": {"value": train_pairs / train_s, "unit": "solves/s"},
         "guess_fit": {"pod_energy": rep.pod_energy, "loss_first": rep.loss_first, "loss_final": rep.loss_final,
@@ -381,41 +418,143 @@ def run_learned(args, B):
         "kernels": kernels,
         "roofline": roof,
         "cpu_baseline": cpu,
-        "host_setup_seconds": time.perf_counter() - t_setup0,
+        "phase_seconds": {
+            "first_training_group_ready": t_train0 - t_setup0,
+            "training_groups": t_train1 - t_train0,
+            "guess_fit": t_fit1 - t_train1,
+            "evaluation_groups": t_eval1 - t_eval0,
+            "cpu_baseline": t_cpu1 - t_cpu0,
+            "whole_run": time.perf_counter() - t_setup0,
+        },
     }
     print(json.dumps(line), flush=True)
 
 
-def cpu_baseline_ref(B, cells, iters_plain, budget_s=10.0):
-    """The reference's serial Cg (Bslib sources compiled into oracle/_ref) on
-    one host core, over the load cases of the sample cells until about
-    budget_s of CPU work has been done."""
+def guess_error(bt, x0, x):
+    """||x0 - x|| / ||x|| per (cell, load case) pair of one group: the learned
+    initial guess against the converged solution."""
+    e0 = bt.elem0
+    errs = []
+    for c in range(bt.ncell):
+        a, b = x0[e0[c]:e0[c + 1], :], x[e0[c]:e0[c + 1], :]
+        nx = np.linalg.norm(b, axis=0)
+        errs.extend((np.linalg.norm(a - b, axis=0) / np.where(nx > 0, nx, 1.0)).tolist())
+    errs = np.asarray(errs)
+    return {"mean": float(errs.mean()), "median": float(np.median(errs)), "max": float(errs.max()),
+            "pairs": int(errs.size), "what": "||x0 - x|| / ||x|| of the learned x0 against the converged x, "
+                                             "every pair of the first timed group"}
+
+
+def reference_thresholds_sample(args, B, ctx, params, tune):
+    """A sample of evaluation-style cells factorized at the reference's IC2
+    defaults (tau1 = 1e-2, tau2 = 1e-4): one batched FP32 solve of all of them
+    from x0 = 0, and the plain FP64 drop-in on two of them, one load case at a
+    time. The learned guess is not trained here (its 1024 training cells at
+    9 s of host IC2 each are out of a benchmark's reach)."""
+    k = len(params)
+    t0 = time.perf_counter()
+    mat0, _ = B.gen_elastic_cell(CELL_M, 0.3, 10.0, 0.3, 0.0, TEMPLATE_SEED)
+    order = mat0.rcm_order()
+    fpar = B.default_params(tau1=1e-2, tau2=1e-4, memory=-1.0)
+
+    def build(i):
+        r, lc, nu, sh = params[i]
+        mat, rhs = B.gen_elastic_cell(CELL_M, r, 10 ** lc, nu, sh, TEMPLATE_SEED)
+        mo = mat.reorder(order)
+        return mo, B.Factor.ic2(mo, fpar), mo.reorder_vector(order, rhs)
+
+    with ThreadPoolExecutor(host_threads()) as ex:
+        cells = list(ex.map(build, range(k)))
+    host_s = time.perf_counter() - t0
+    bt = B.Batch([(mo, f) for mo, f, _ in cells], prec_mask=1 << B.F32)
+    db = B.DeviceArray.from_host(bt.gather([c[2] for c in cells]))
+    opts = B.default_opts(kind=B.BLOCKPCG, eps=EPS, precision=B.F32, niter=5000)
+    t = bt.autotune(ctx, db, opts, 6, allow_graph=not args.no_graph)
+    dx = B.DeviceArray(db.shape)
+    dx.zero()
+    ctx.sync()
+    t1 = time.perf_counter()
+    st, its, rr = bt.solve(ctx, db, dx, opts, 6)
+    bs = time.perf_counter() - t1
+    popts = B.default_opts(kind=B.PCG, eps=EPS, ittype=3, check_every=8, niter=5000)
+    pctx = B.Context()
+    for mo, f, b in cells[:1]:  # device images uploaded before timing
+        d1 = B.DeviceArray.from_host(np.ascontiguousarray(b[:, 0]))
+        x1 = B.DeviceArray(mo.n)
+        x1.zero()
+        pctx.solve(mo, f, d1, x1, popts)
+    nplain, plain_s, pits, pok = 0, 0.0, [], True
+    for mo, f, b in cells[:2]:
+        for q in range(6):
+            d1 = B.DeviceArray.from_host(np.ascontiguousarray(b[:, q]))
+            x1 = B.DeviceArray(mo.n)
+            x1.zero()
+            pctx.sync()
+            t2 = time.perf_counter()
+            stp = pctx.solve(mo, f, d1, x1, popts)
+            plain_s += time.perf_counter() - t2
+            nplain += 1
+            pits.append(stp.iterations)
+            pok = pok and stp.converged == 1
+    fi = cells[0][1].info(1)
+    return {"tau": [1e-2, 1e-4], "cells": k,
+            "batched_f32_zero_guess": {"value": st.pairs / bs, "unit": "solves/s", "mean_iterations": float(its.mean()),
+                                       "relres_true_max": float(rr.max()),
+                                       "valid": bool(st.converged == st.pairs and rr.max() <= EPS),
+                                       "schedule": {"sweep_mode": t.sweep_mode, "graph": t.graph}},
+            "plain_fp64_gpu": {"value": nplain / plain_s, "unit": "solves/s", "mean_iterations": float(np.mean(pits)),
+                               "valid": bool(pok), "sample": f"{nplain} solves (2 cells x 6 load cases)"},
+            "speedup_batched_vs_plain": (st.pairs / bs) / (nplain / plain_s),
+            "factor": {"blocks": fi["nzja"], "values": fi["nza"], "levels": fi["levels_fwd"]},
+            "host_ic2_seconds_per_cell": host_s * min(host_threads(), k) / k}
+
+
+def cpu_baseline_ref(B, cells, iters_plain, budget_s=10.0, cores=1):
+    """The reference's serial Cg (Bslib sources compiled into oracle/_ref),
+    one independent solve per host core (`cores` threads; the ctypes call
+    releases the interpreter lock, and the reference's Cg keeps no global
+    state), over the load cases of the sample cells until every thread has
+    done about budget_s of CPU work. value = solves / wall seconds."""
     import oracle as O
 
     use_ref = O.ref_available()
-    nsolve, nit_sum, t_sum = 0, 0, 0.0
+    tasks = []
     for mo, f, b in cells:
         A = O.Block(*mo.arrays())
         U = O.Block(A.sprnd, *f.arrays(1, A.nsup))
         for q in range(b.shape[1]):
-            t0 = time.perf_counter()
+            tasks.append((A, U, np.ascontiguousarray(b[:, q])))
+    lock = threading.Lock()
+    state = {"next": 0, "nsolve": 0, "nit": 0}
+
+    def worker(t_end):
+        while time.perf_counter() < t_end:
+            with lock:
+                k = state["next"]
+                state["next"] += 1
+            A, U, bq = tasks[k % len(tasks)]
             if use_ref:
-                _, nit, _ = O.ref_cg(A, U, b[:, q], eps=EPS, ittype=0, niter=5000)
+                _, nit, _ = O.ref_cg(A, U, bq, eps=EPS, ittype=0, niter=5000)
             else:
-                _, r = O.cg(A, U, b[:, q], eps=EPS, ittype=3, niter=5000)
+                _, r = O.cg(A, U, bq, eps=EPS, ittype=3, niter=5000)
                 nit = r.updates
-            t_sum += time.perf_counter() - t0
-            nsolve += 1
-            nit_sum += nit
-            if t_sum >= budget_s:
-                break
-        if t_sum >= budget_s:
-            break
+            with lock:
+                state["nsolve"] += 1
+                state["nit"] += nit
+
+    t0 = time.perf_counter()
+    threads = [threading.Thread(target=worker, args=(t0 + budget_s,)) for _ in range(cores)]
+    for t in threads:
+        t.start()
+    for t in threads:
+        t.join()
+    wall = time.perf_counter() - t0
+    nsolve = state["nsolve"]
     who = "the reference's serial Cg (compiled from its sources)" if use_ref else "the oracle's Cg"
-    return dict(value=nsolve / t_sum, unit="solves/s", cores=1, kind="reference" if use_ref else "port",
-                sample=f"{nsolve} load cases of the first evaluation cells, one at a time, by {who}: "
-                       f"{nit_sum / nsolve:.1f} iterations each to rel-res {EPS}, {t_sum:.1f}s on one core "
-                       f"of {cpu_model()}")
+    return dict(value=nsolve / wall, unit="solves/s", cores=cores, kind="reference" if use_ref else "port",
+                sample=f"{nsolve} load-case solves of the first evaluation cells, one independent solve per core "
+                       f"on {cores} host threads, by {who}: {state['nit'] / max(nsolve, 1):.1f} iterations each to "
+                       f"rel-res {EPS}, {wall:.1f}s wall on {cpu_model()}")
 
 
 # ---------------------------------------------------------------------------
@@ -751,7 +890,9 @@ def main():
     ap.add_argument("--train-cells", type=int, default=1024)
     ap.add_argument("--fit-steps", type=int, default=16000)
     ap.add_argument("--fit-lr", type=float, default=5e-3)
-    ap.add_argument("--pod-modes", type=int, default=128, help="POD modes per load case of the initial-guess model")
+    ap.add_argument("--pod-modes", type=int, default=64, help="POD modes per load case of the initial-guess model")
+    ap.add_argument("--ref-tau-cells", type=int, default=16,
+                    help="cells solved at the reference's IC2 thresholds (1e-2, 1e-4) beside the headline (0: none)")
     ap.add_argument("--mlp-hidden", type=int, default=256, help="width of the guess MLP's two hidden layers")
     ap.add_argument("--plain-cells", type=int, default=4)
     ap.add_argument("--factor-precision", choices=["f32", "bf16"], default="f32",

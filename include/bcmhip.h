@@ -33,6 +33,12 @@ extern "C" {
 #define BCM_CGPAR 4      /* CSVector::Cg parallel form with operator plug-ins, cg.cpp:355-771 */
 #define BCM_BLOCKPCG 5   /* batched PCG over many cells and right-hand sides (bcm_batch_solve) */
 #define BCM_BLOCKGMRES 6 /* CSVector::BlockGMRES, gmres.cpp:337-1038 (bcm_solve_block; bcm_solve: 1 column) */
+/* Block Lanczos (block Golub-Kahan bidiagonalization with block Givens
+ * rotations) of Solver(RS), solvers.cpp:1798-1800, by preconditioning side
+ * (bcm_solve_block; bcm_solve: 1 column): */
+#define BCM_BLZ_CENTRAL 7 /* CSVector::BlockLanczosCentral, lanczos.cpp:2467-3225 (ittype 2) */
+#define BCM_BLZ_RIGHT 8   /* CSVector::BlockLanczosRight, lanczos.cpp:932-1718 (ittype 3) */
+#define BCM_BLZ_LEFT 9    /* CSVector::BlockLanczosLeft, lanczos.cpp:1719-2466 (ittype 4) */
 
 #define BCM_F64 0
 #define BCM_F32 1
@@ -72,6 +78,7 @@ typedef struct bcm_solve_opts {
   int restart;       /* FGMRES cycle length */
   int precision;     /* preconditioner storage/application: BCM_F64, BCM_F32, BCM_BF16 */
   int max_restarts;  /* true-residual restarts */
+  int ichk;          /* block Lanczos: iterations between residual tests (CSlvParam::ichk, slvparam.h:174) */
 } bcm_solve_opts;
 
 typedef struct bcm_stats {
@@ -110,6 +117,22 @@ int bcm_rcm_order(bcm_matrix m, int* order);                       /* order[old]
 int bcm_matrix_reorder(bcm_matrix m, const int* order, bcm_matrix* out);
 /* CSVector::OrdVect (svector.cpp:418): type 'D' applies order, 'I' undoes it. */
 int bcm_vector_reorder(bcm_matrix original, const int* order, char type, double* v, int nrhs);
+
+/* ---- binary files (the reference's WriteMtrBin / WriteVectBin layout) ---- */
+/* CSMatrixRS::WriteMtrBin (smatrixrs.cpp:3210-3227): raw ints m, n, nsupr,
+ * nsupc, nlist, nzja, nza, blamx, sprndr, sprndc, list, ia, ja, bsa, then the
+ * doubles a. */
+int bcm_matrix_write_bin(bcm_matrix m, const char* path);
+/* Reads that layout (square, every block row listed, blocks in ja order);
+ * symmetric as in bcm_matrix_create. The reference writes this layout but has
+ * no block-matrix reader; CSMatrixR::ReadMtrBin (smatrixr.cpp:3852) is its
+ * point-matrix counterpart. */
+int bcm_matrix_read_bin(const char* path, int symmetric, bcm_matrix* out);
+/* CSVector::WriteVectBin (svector.cpp:537-546) of an n x nrhs host vector. */
+int bcm_vector_write_bin(const char* path, int n, int nrhs, const double* v);
+/* CSVector::ReadVectBin (svector.cpp:488-507): *n and *nrhs always, the values
+ * into v (n * nrhs doubles) when v is not NULL. */
+int bcm_vector_read_bin(const char* path, int* n, int* nrhs, double* v);
 
 /* ---- dense kernels of the factorizations ---- */
 /* dsyev_("V", "U") as the IC2 calls it (lapackd.cpp:865-878 -> LAPACK 3.0
@@ -202,7 +225,8 @@ int bcm_dot(bcm_context c, int n, const double* x, const double* y, double* resu
 /* ---- GPU solve: x (device) holds the initial guess on entry ---- */
 int bcm_solve(bcm_context c, bcm_matrix a, bcm_factor f, const bcm_solve_opts* o, const double* b, double* x,
               bcm_stats* st);
-/* Block GMRES over nrhs (1..8) right-hand sides, replacing
+/* Block GMRES (o->kind BCM_BLOCKGMRES) or block Lanczos (BCM_BLZ_CENTRAL /
+ * RIGHT / LEFT) over nrhs (1..8) right-hand sides. Block GMRES, replacing
  * CSVector::BlockGMRES (gmres.cpp:337-1038) with the operator pair of
  * CParMvmA::MvmA / CParSolveLU::SolveLU (gsmatrix.h:758-817, 1008-1071):
  * b and x (device) are n x nrhs column-major like CSVector's vect, x holds
@@ -210,7 +234,11 @@ int bcm_solve(bcm_context c, bcm_matrix a, bcm_factor f, const bcm_solve_opts* o
  * ||r_c|| < eps ||r0_c|| (sttype != 0), 1 tests ||r_c|| < eps (sttype 0),
  * 3 tests ||r_c|| < eps ||b_c||. Columns with r0 = 0 are left as they are.
  * resi (host, nrhs, may be NULL) receives ||b_c - A x_c||; hist (host,
- * niter x nrhs, may be NULL) the recursion's residual norms after each step. */
+ * niter x nrhs, may be NULL) the recursion's residual norms after each step.
+ * Block Lanczos (lanczos.cpp:932-3225): o->niter caps the iterations, every
+ * o->ichk iterations each column's true residual is tested against eps
+ * times its initial norm (Right: also ||Ahat^T r||), the solve ends when all
+ * pass; hist is not written. Columns with r0 = 0 are left as they are. */
 int bcm_solve_block(bcm_context c, bcm_matrix a, bcm_factor f, const bcm_solve_opts* o, int nrhs, const double* b,
                     double* x, bcm_stats* st, double* resi, double* hist);
 
@@ -308,8 +336,9 @@ typedef void (*bcm_cb_storesol)(void* obj, int isup, double* sol);
 int bcm_abstract_symm_solver(void* pgener, bcm_params* p, bcm_cb_nsup fnsup, bcm_cb_sprnds fsprnds,
                              bcm_cb_iaja fiaja, bcm_cb_blockrow fblockrow, bcm_cb_rhs frhs,
                              bcm_cb_iniguess finiguess, bcm_cb_storesol fstoresol);
-/* AbstractSolver (solvers.cpp:5591): assemble, RCM, ILU2, GPU Krylov
- * (ittype 1: two-sided Lanczos as the reference; otherwise BiCGStab). */
+/* AbstractSolver (solvers.cpp:5591): assemble, RCM, ILU2, GPU Krylov as
+ * Solver(RS) dispatches (solvers.cpp:1710-1800): ittype 1 two-sided Lanczos,
+ * 2 / 3 / 4 block Lanczos Central / Right / Left; other values BCM_E_ARG. */
 int bcm_abstract_solver(void* pgener, bcm_params* p, bcm_cb_nsup fnsup, bcm_cb_sprnds fsprnds, bcm_cb_iaja fiaja,
                         bcm_cb_blockrow fblockrow, bcm_cb_rhs frhs, bcm_cb_iniguess finiguess,
                         bcm_cb_storesol fstoresol);

@@ -707,3 +707,236 @@ orc_result orc_block_gmres(const orc_bsr* a, const orc_bsr* l, const orc_bsr* u,
   free(p); free(taup); free(giv); free(bloc); free(eloc); free(rm); free(r); free(y); free(w); free(tmp); free(yp);
   return res;
 }
+
+/* ---- block Lanczos (block Golub-Kahan bidiagonalization with block Givens
+ *      rotations), CSVector::BlockLanczosRight / Left / Central
+ *      (lanczos.cpp:932-1718, 1719-2466, 2467-3225) ----
+ *
+ * The three variants run the same recurrence on a preconditioned operator
+ * Ahat and its transpose: variant 0 (Right) Ahat = A U^-1 L^-1, 1 (Left)
+ * Ahat = U^-1 L^-1 A, 2 (Central) Ahat = L^-1 A U^-1, where L is the factor
+ * stored by columns and U by rows (SolveL / SolveU as in mvm.cpp; their
+ * transposes are U.SolveL and L.SolveU). Blocks are QR-factored by
+ * Householder reflectors (QrdBlk, globals.cpp:413-470; the Q block formed by
+ * MvmQBlk, globals.cpp:717-776; R by GetRPartQrd, globals.cpp:920-933); the
+ * block Givens rotation is the explicit Q of the 2nrhs x 2nrhs QR of
+ * [rhobar; beta] (lanczos.cpp:1244-1268). The reference inverts rho through
+ * an SVD (dgesvd, lanczos.cpp:1430-1462); this restatement inverts it by
+ * Gauss-Jordan elimination with partial pivoting (equal up to rounding).
+ * Convergence is tested every ichk iterations on the true residual
+ * b - A x_k of every column (relative to its initial norm), and for Right
+ * also on ||Ahat^T r_k|| (lanczos.cpp:1536-1585). Returns nitperf = the number
+ * of iterations done, updates = 1 if every column met the test, resfin = the
+ * largest ||b_c - A x_c|| / ||b_c - A x0_c||. nrhs <= 16. */
+static int orc_inv_small(int nb, const double* a, double* inv) {
+  double m[16 * 32];
+  for (int r = 0; r < nb; ++r)
+    for (int c = 0; c < 2 * nb; ++c) m[r * 2 * nb + c] = c < nb ? a[(size_t)c * nb + r] : (c - nb == r ? 1.0 : 0.0);
+  for (int c = 0; c < nb; ++c) {
+    int piv = c;
+    for (int r = c + 1; r < nb; ++r)
+      if (fabs(m[r * 2 * nb + c]) > fabs(m[piv * 2 * nb + c])) piv = r;
+    if (m[piv * 2 * nb + c] == 0.0) return 1;
+    if (piv != c)
+      for (int k = 0; k < 2 * nb; ++k) {
+        const double t = m[c * 2 * nb + k];
+        m[c * 2 * nb + k] = m[piv * 2 * nb + k];
+        m[piv * 2 * nb + k] = t;
+      }
+    const double d = m[c * 2 * nb + c];
+    for (int k = 0; k < 2 * nb; ++k) m[c * 2 * nb + k] /= d;
+    for (int r = 0; r < nb; ++r) {
+      if (r == c) continue;
+      const double f = m[r * 2 * nb + c];
+      if (f != 0.0)
+        for (int k = 0; k < 2 * nb; ++k) m[r * 2 * nb + k] -= f * m[c * 2 * nb + k];
+    }
+  }
+  for (int r = 0; r < nb; ++r)
+    for (int c = 0; c < nb; ++c) inv[(size_t)c * nb + r] = m[r * 2 * nb + nb + c];
+  return 0;
+}
+
+/* y = Ahat x (transpose = 0) or Ahat^T x, nb columns; t1, t2 scratch */
+static void orc_blz_op(int variant, int transpose, const orc_bsr* a, const orc_bsr* l, const orc_bsr* u, int nb,
+                       const double* x, double* y, double* t1, double* t2) {
+  if (!transpose) {
+    if (variant == 0) { orc_solvel(l, x, t1, nb); orc_solveu(u, t1, t2, nb); orc_mvma(a, t2, y, nb); }
+    else if (variant == 1) { orc_mvma(a, x, t1, nb); orc_solvel(l, t1, t2, nb); orc_solveu(u, t2, y, nb); }
+    else { orc_solveu(u, x, t1, nb); orc_mvma(a, t1, t2, nb); orc_solvel(l, t2, y, nb); }
+  } else {
+    if (variant == 0) { orc_mvmat(a, x, t1, nb); orc_solvel(u, t1, t2, nb); orc_solveu(l, t2, y, nb); }
+    else if (variant == 1) { orc_solvel(u, x, t1, nb); orc_solveu(l, t1, t2, nb); orc_mvmat(a, t2, y, nb); }
+    else { orc_solveu(l, x, t1, nb); orc_mvmat(a, t1, t2, nb); orc_solvel(u, t2, y, nb); }
+  }
+}
+
+/* y += s * X * c (DaxpySVectBlk, svector.cpp:328-343): X is n x nb, c nb x nb column-major */
+static void orc_daxpy_blk(int n, int nb, double s, const double* c, const double* x, double* y) {
+  for (int i = 0; i < nb; ++i)
+    for (int j = 0; j < n; ++j) {
+      double aux = 0.0;
+      for (int k = 0; k < nb; ++k) aux += x[(size_t)k * n + j] * c[i * nb + k];
+      y[(size_t)i * n + j] += s * aux;
+    }
+}
+
+/* Householder QR of the n x nb block w in place; q = its Q block, r = R (nb x nb). */
+static void orc_blz_qr(int n, int nb, double* w, double* tau, double* q, double* r) {
+  double ident[16 * 16];
+  orc_update_qrd(n, 0, nb, w, n, tau);
+  memset(ident, 0, sizeof(ident));
+  for (int i = 0; i < nb; ++i) ident[i * nb + i] = 1.0;
+  orc_mvmq(n, nb, nb, w, n, tau, ident, nb, q, n);
+  for (int c = 0; c < nb; ++c)
+    for (int i = 0; i < nb; ++i) r[c * nb + i] = i <= c ? w[(size_t)c * n + i] : 0.0;
+}
+
+static void orc_tr_small(int nb, double* a) {
+  double t[256];
+  for (int i = 0; i < nb; ++i)
+    for (int j = 0; j < nb; ++j) t[j * nb + i] = a[i * nb + j];
+  memcpy(a, t, sizeof(double) * nb * nb);
+}
+
+/* out = a b for nb x nb column-major matrices (lanczos.cpp:1380-1421) */
+static void orc_ab(int nb, const double* a, const double* b, double* out) {
+  for (int i = 0; i < nb; ++i)
+    for (int j = 0; j < nb; ++j) {
+      double aux = 0.0;
+      for (int k = 0; k < nb; ++k) aux += a[k * nb + i] * b[j * nb + k];
+      out[j * nb + i] = aux;
+    }
+}
+
+orc_result orc_block_lanczos(int variant, const orc_bsr* a, const orc_bsr* l, const orc_bsr* u, int nrhs,
+                             const double* x0, const double* b, int niter, double eps, int ichk, double* sol_out) {
+  const int n = a->sprnd[a->nsup], nb = nrhs, nb2 = nb * nb;
+  const size_t vb = sizeof(double) * (size_t)n * nb;
+  double *r = malloc(vb), *uu = malloc(vb), *v = malloc(vb), *w = malloc(vb), *qw = malloc(vb), *pw = malloc(vb);
+  double *t1 = malloc(vb), *t2 = malloc(vb), *soli = calloc((size_t)n * nb, 8), *xk = malloc(vb);
+  double alpha[256], beta[256], rhobar[256], phibar[256], rho[256], rinv[256], theta[256], phi[256], cc[256], ss[256],
+      ccbar[256], ssbar[256], coef[256], tau[32], giv[1024], gq[1024], gr[1024], ident[1024];
+  double resi0[16], resort0[16];
+  orc_result res = {0, 0, 0.0, 0};
+  memcpy(sol_out, x0, vb);
+  orc_mvma(a, x0, r, nb);
+  for (size_t i = 0; i < (size_t)n * nb; ++i) r[i] = b[i] - r[i];
+  if (variant != 0) {
+    for (int c = 0; c < nb; ++c) {
+      double s = 0.0;
+      for (int i = 0; i < n; ++i) s += r[(size_t)c * n + i] * r[(size_t)c * n + i];
+      resi0[c] = sqrt(s);
+    }
+    if (variant == 1) { orc_solvel(l, r, t1, nb); orc_solveu(u, t1, r, nb); }
+    else { orc_solvel(l, r, t1, nb); memcpy(r, t1, vb); }
+  }
+  orc_blz_qr(n, nb, r, tau, uu, beta);
+  if (variant == 0)
+    for (int c = 0; c < nb; ++c) {
+      double s = 0.0;
+      for (int i = 0; i < nb; ++i) s += beta[c * nb + i] * beta[c * nb + i];
+      resi0[c] = sqrt(s);
+    }
+  orc_blz_op(variant, 1, a, l, u, nb, uu, w, t1, t2);
+  orc_blz_qr(n, nb, w, tau, v, alpha);
+  orc_tr_small(nb, alpha);
+  if (variant == 0)
+    for (int c = 0; c < nb; ++c) {
+      double s = 0.0;
+      for (int i = 0; i < nb; ++i) s += alpha[c * nb + i] * alpha[c * nb + i];
+      resort0[c] = sqrt(s);
+    }
+  memcpy(w, v, vb);
+  memcpy(phibar, beta, sizeof(double) * nb2);
+  memcpy(rhobar, alpha, sizeof(double) * nb2);
+  int k = 0, conv = 0;
+  for (k = 0; k < niter; ++k) {
+    orc_blz_op(variant, 0, a, l, u, nb, v, qw, t1, t2);
+    orc_daxpy_blk(n, nb, -1.0, alpha, uu, qw);
+    orc_blz_qr(n, nb, qw, tau, uu, beta);
+    orc_blz_op(variant, 1, a, l, u, nb, uu, pw, t1, t2);
+    memcpy(coef, beta, sizeof(double) * nb2);
+    orc_tr_small(nb, coef);
+    orc_daxpy_blk(n, nb, -1.0, coef, v, pw);
+    orc_blz_qr(n, nb, pw, tau, v, alpha);
+    orc_tr_small(nb, alpha);
+    /* block Givens rotation: QR of [rhobar; beta] (2nb x 2nb, last nb columns zero) */
+    const int n2 = 2 * nb;
+    memset(giv, 0, sizeof(double) * n2 * n2);
+    for (int i = 0; i < nb; ++i)
+      for (int j = 0; j < nb; ++j) {
+        giv[j * n2 + i] = rhobar[j * nb + i];
+        giv[j * n2 + nb + i] = beta[j * nb + i];
+      }
+    orc_update_qrd(n2, 0, n2, giv, n2, tau);
+    memset(ident, 0, sizeof(double) * n2 * n2);
+    for (int i = 0; i < n2; ++i) ident[i * n2 + i] = 1.0;
+    orc_mvmq(n2, n2, n2, giv, n2, tau, ident, n2, gq, n2);
+    for (int c = 0; c < n2; ++c)
+      for (int i = 0; i < n2; ++i) gr[c * n2 + i] = i <= c ? giv[c * n2 + i] : 0.0;
+    for (int i = 0; i < nb; ++i)
+      for (int j = 0; j < nb; ++j) {
+        rho[j * nb + i] = gr[j * n2 + i];
+        cc[j * nb + i] = gq[i * n2 + j];
+        ss[j * nb + i] = gq[i * n2 + nb + j];
+        ssbar[j * nb + i] = -gq[(i + nb) * n2 + j];
+        ccbar[j * nb + i] = gq[(i + nb) * n2 + nb + j];
+      }
+    orc_ab(nb, ss, alpha, theta);     /* theta = ss alpha */
+    orc_ab(nb, ccbar, alpha, rhobar); /* rhobar = ccbar alpha */
+    orc_ab(nb, cc, phibar, phi);      /* phi = cc phibar */
+    orc_ab(nb, ssbar, phibar, coef);  /* phibar = -ssbar phibar */
+    for (int i = 0; i < nb2; ++i) phibar[i] = -coef[i];
+    if (orc_inv_small(nb, rho, rinv)) { res.breakdown = 1; break; }
+    for (int i = 0; i < nb; ++i)
+      for (int j = 0; j < nb; ++j) {
+        double aux = 0.0;
+        for (int q = 0; q < nb; ++q) aux += rinv[q * nb + i] * phi[j * nb + q];
+        coef[j * nb + i] = aux;
+      }
+    orc_daxpy_blk(n, nb, 1.0, coef, w, soli);
+    for (int i = 0; i < nb; ++i)
+      for (int j = 0; j < nb; ++j) {
+        double aux = 0.0;
+        for (int q = 0; q < nb; ++q) aux += rinv[q * nb + i] * theta[j * nb + q];
+        coef[j * nb + i] = aux;
+      }
+    memcpy(pw, v, vb);
+    orc_daxpy_blk(n, nb, -1.0, coef, w, pw);
+    memcpy(w, pw, vb);
+    if (k % ichk == 0) {
+      if (variant == 0) { orc_solvel(l, soli, t1, nb); orc_solveu(u, t1, xk, nb); }
+      else if (variant == 1) memcpy(xk, soli, vb);
+      else orc_solveu(u, soli, xk, nb);
+      for (size_t i = 0; i < (size_t)n * nb; ++i) xk[i] += x0[i];
+      orc_mvma(a, xk, r, nb);
+      for (size_t i = 0; i < (size_t)n * nb; ++i) r[i] = b[i] - r[i];
+      if (variant == 0) orc_blz_op(0, 1, a, l, u, nb, r, pw, t1, t2);
+      conv = 0;
+      double worst = 0.0;
+      for (int c = 0; c < nb; ++c) {
+        double s = 0.0, so = 0.0;
+        for (int i = 0; i < n; ++i) s += r[(size_t)c * n + i] * r[(size_t)c * n + i];
+        const double q = sqrt(s) / resi0[c];
+        if (q > worst) worst = q;
+        int ok = q < eps;
+        if (variant == 0) {
+          for (int i = 0; i < n; ++i) so += pw[(size_t)c * n + i] * pw[(size_t)c * n + i];
+          ok = ok || sqrt(so) / resort0[c] < eps;
+        }
+        conv += ok;
+      }
+      res.resfin = worst;
+      if (conv == nb) break;
+    }
+  }
+  res.nitperf = k < niter ? k + 1 : niter;
+  res.updates = conv == nb;
+  if (variant == 0) { orc_solvel(l, soli, t1, nb); orc_solveu(u, t1, xk, nb); }
+  else if (variant == 1) memcpy(xk, soli, vb);
+  else orc_solveu(u, soli, xk, nb);
+  for (size_t i = 0; i < (size_t)n * nb; ++i) sol_out[i] += xk[i];
+  free(r); free(uu); free(v); free(w); free(qw); free(pw); free(t1); free(t2); free(soli); free(xk);
+  return res;
+}

@@ -70,6 +70,9 @@ def orc() -> C.CDLL:
         l.orc_block_gmres.argtypes = [C.POINTER(Bsr)] * 3 + [C.c_int, C.c_int, _DP, _DP, C.c_int, C.c_double,
                                                             C.c_int, _DP, _DP, _DP]
         l.orc_fgmres.restype = Result
+        l.orc_block_lanczos.argtypes = [C.c_int] + [C.POINTER(Bsr)] * 3 + [C.c_int, _DP, _DP, C.c_int, C.c_double,
+                                                                           C.c_int, _DP]
+        l.orc_block_lanczos.restype = Result
         _orc = l
     return _orc
 
@@ -99,6 +102,11 @@ def ref() -> C.CDLL:
         l.ref_ich2.argtypes = mat + [C.c_double] * 4 + [C.c_char_p, C.c_int]
         l.ref_ilu2.argtypes = mat + [C.c_double] * 6 + [C.c_char_p, C.c_int]
         l.ref_fct_take.argtypes = [C.c_int, _IP, _IP, _IP, _IP, _DP]
+        l.ref_block_lanczos.argtypes = [C.c_int] + mat + [_IP, _IP, _DP, _IP, _IP, _DP, C.c_int, _DP, _DP,
+                                                           C.c_int, C.c_double, C.c_int, _DP]
+        l.ref_write_mtr_bin.argtypes = mat + [C.c_char_p]
+        l.ref_write_vect_bin.argtypes = [C.c_int, C.c_int, _DP, C.c_char_p]
+        l.ref_read_vect_bin.argtypes = [C.c_char_p, _IP, _IP, _DP]
         l.ref_dsyev.argtypes = [C.c_int, _DP, _DP, _DP, C.c_int, _IP]
         l.ref_dgesvd.argtypes = [C.c_int, _DP, _DP, _DP, _DP, _DP, C.c_int, _IP]
         _ref = l
@@ -200,6 +208,19 @@ def fgmres(a: Block, l: Block, u: Block, b, x0=None, m=30, niter=1000, eps=1e-10
     b = _d(b); x0 = np.zeros_like(b) if x0 is None else _d(x0); x = np.zeros_like(b)
     r = orc().orc_fgmres(C.byref(a.c), C.byref(l.c), C.byref(u.c), _dp(x0), _dp(b), m, niter, eps, int(symm),
                          _dp(x))
+    return x, r
+
+
+def block_lanczos(variant: int, a: Block, l: Block, u: Block, b, x0=None, niter=200, eps=1e-10, ichk=5):
+    """Block Lanczos of the reference (lanczos.cpp:932-3225): variant 0 Right,
+    1 Left, 2 Central preconditioning, on the n x nrhs block b. Returns (x, result)."""
+    b = np.asfortranarray(np.asarray(b, dtype=np.float64).reshape(a.n, -1, order="F"))
+    nrhs = b.shape[1]
+    x0 = np.zeros_like(b) if x0 is None else np.asfortranarray(np.asarray(x0, dtype=np.float64).reshape(a.n, -1,
+                                                                                                       order="F"))
+    x = np.zeros_like(b)
+    r = orc().orc_block_lanczos(variant, C.byref(a.c), C.byref(l.c), C.byref(u.c), nrhs, _dp(x0), _dp(b), niter,
+                                eps, ichk, _dp(x))
     return x, r
 
 
@@ -318,6 +339,41 @@ def ref_ilu2(m: Block, tau1=1e-2, tau2=1e-4, theta=0.1, sclmin=1e-8, pivmin=1e-8
     if ref().ref_ilu2(*m.args(), tau1, tau2, theta, sclmin, pivmin, memory, msg, 256):
         raise RefFactorError(msg.value.decode(errors="replace"))
     return _take(ref().ref_fct_take, 0, nsup=m.nsup), _take(ref().ref_fct_take, 1, nsup=m.nsup)
+
+
+def ref_block_lanczos(variant: int, a: Block, l: Block, u: Block, b, x0=None, niter=200, eps=1e-10, ichk=5):
+    """The reference's CSVector::BlockLanczosRight (0) / Left (1) / Central (2)
+    (lanczos.cpp:932, 1719, 2467) on the n x nrhs block b."""
+    b = np.asfortranarray(np.asarray(b, dtype=np.float64).reshape(a.n, -1, order="F"))
+    nrhs = b.shape[1]
+    x0 = np.zeros_like(b) if x0 is None else np.asfortranarray(np.asarray(x0, dtype=np.float64).reshape(a.n, -1,
+                                                                                                       order="F"))
+    sol = np.zeros_like(b)
+    ref().ref_block_lanczos(variant, *a.args(), l.ia.ctypes.data_as(_IP), l.ja.ctypes.data_as(_IP),
+                            l.a.ctypes.data_as(_DP), u.ia.ctypes.data_as(_IP), u.ja.ctypes.data_as(_IP),
+                            u.a.ctypes.data_as(_DP), nrhs, x0.ctypes.data_as(_DP), b.ctypes.data_as(_DP), niter, eps,
+                            ichk, sol.ctypes.data_as(_DP))
+    return sol
+
+
+def ref_write_mtr_bin(m: Block, path: str):
+    """CSMatrixRS::WriteMtrBin (smatrixrs.cpp:3210) of m to path."""
+    ref().ref_write_mtr_bin(*m.args(), path.encode())
+
+
+def ref_write_vect_bin(v, path: str):
+    """CSVector::WriteVectBin (svector.cpp:537) of an n x nrhs array."""
+    v = np.asfortranarray(np.asarray(v, dtype=np.float64).reshape(v.shape[0], -1, order="F"))
+    ref().ref_write_vect_bin(v.shape[0], v.shape[1], v.ctypes.data_as(_DP), path.encode())
+
+
+def ref_read_vect_bin(path: str):
+    """CSVector::ReadVectBin (svector.cpp:488): n x nrhs array."""
+    n, nrhs = C.c_int(), C.c_int()
+    ref().ref_read_vect_bin(path.encode(), C.byref(n), C.byref(nrhs), None)
+    v = np.zeros((n.value, nrhs.value), order="F")
+    ref().ref_read_vect_bin(path.encode(), C.byref(n), C.byref(nrhs), v.ctypes.data_as(_DP))
+    return v
 
 
 def ref_dsyev(a):

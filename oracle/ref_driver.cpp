@@ -143,6 +143,54 @@ int ref_ilu2(int nsup, const int* sprnd, const int* ia, const int* ja, const dou
   return 0;
 }
 
+// CSVector::BlockLanczosRight / Left / Central (lanczos.cpp:932, 1719, 2467)
+// with ILU2 factors l (by columns) and u (by rows), as Solver(RS) calls them
+// for ittype 3 / 4 / 2 (solvers.cpp:1798-1800); x0, b, sol are n x nrhs.
+void ref_block_lanczos(int variant, int nsup, const int* sprnd, const int* ia, const int* ja, const double* a,
+                       const int* lia, const int* lja, const double* la, const int* uia, const int* uja,
+                       const double* ua, int nrhs, const double* x0, const double* b, int niter, double eps, int ichk,
+                       double* sol) {
+  CSMatrixRS am = make(nsup, sprnd, ia, ja, a);
+  CSMatrixRS lm = make(nsup, sprnd, lia, lja, la);
+  CSMatrixRS um = make(nsup, sprnd, uia, uja, ua);
+  const int n = sprnd[nsup];
+  CSVector vx = vec(n, nrhs, x0), vb = vec(n, nrhs, b), out(n, 1, nrhs);
+  out.SetSVect(0.0);
+  CSlvParam p;
+  p.msglev = 0;
+  p.niter = niter;
+  p.eps = eps;
+  p.ichk = ichk;
+  std::ofstream fout("/dev/null");
+  if (variant == 0) out.BlockLanczosRight(fout, &am, &lm, &um, vx, vb, p);
+  else if (variant == 1) out.BlockLanczosLeft(fout, &am, &lm, &um, vx, vb, p);
+  else out.BlockLanczosCentral(fout, &am, &lm, &um, vx, vb, p);
+  std::memcpy(sol, out.GetVect(), sizeof(double) * n * nrhs);
+}
+
+// CSMatrixRS::WriteMtrBin (smatrixrs.cpp:3210) of the matrix to path.
+void ref_write_mtr_bin(int nsup, const int* sprnd, const int* ia, const int* ja, const double* a, const char* path) {
+  CSMatrixRS m = make(nsup, sprnd, ia, ja, a);
+  std::ofstream f(path, std::ios::binary);
+  m.WriteMtrBin(f);
+}
+
+// CSVector::WriteVectBin (svector.cpp:537) of an n x nrhs vector to path.
+void ref_write_vect_bin(int n, int nrhs, const double* v, const char* path) {
+  CSVector x = vec(n, nrhs, v);
+  std::ofstream f(path, std::ios::binary);
+  x.WriteVectBin(f);
+}
+
+// CSVector::ReadVectBin (svector.cpp:488) of path: sizes, then the values when v != null.
+void ref_read_vect_bin(const char* path, int* n, int* nrhs, double* v) {
+  std::ifstream f(path, std::ios::binary);
+  CSVector x = CSVector::ReadVectBin(f);
+  *n = x.GetNv();
+  *nrhs = x.GetNrhs();
+  if (v) std::memcpy(v, x.GetVect(), sizeof(double) * (size_t)x.GetNv() * x.GetNrhs());
+}
+
 // The last factor's part which (0 L, 1 U): sizes, then arrays when non-null.
 void ref_fct_take(int which, int* nzja, int* nza, int* oia, int* oja, double* oa) {
   take(g_fct[which], nzja, nza, oia, oja, oa);

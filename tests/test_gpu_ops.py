@@ -177,9 +177,12 @@ def test_single_block_row(B, O, ctx):
     assert rel(dy.host(), np.linalg.solve(a.reshape(8, 8).T, x)) < 1e-12
 
 
-def test_unsupported_block_size_is_an_error(B, ctx):
-    m, _ = B.gen_seepage3d(3, 2, 2, 5, 0.3, 1)
-    dx, dy = B.DeviceArray(m.n), B.DeviceArray(m.n)
+def test_block_rows_beyond_64_are_an_error(B, ctx):
+    """Blocks of up to 64 rows run on the device; larger ones are refused."""
+    b = 65
+    a = np.random.default_rng(10).standard_normal(b * b) + 20 * np.eye(b).reshape(-1)
+    m = B.Matrix.from_arrays([0, b], [0, 1], [0], a, symmetric=False)
+    dx, dy = B.DeviceArray(b), B.DeviceArray(b)
     with pytest.raises(B.BcmError) as e:
         ctx.mvm(m, dx, dy)
     assert e.value.code == B.E_ARG

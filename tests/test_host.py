@@ -410,3 +410,56 @@ def test_ilu2_bitwise_equals_reference(B, O, bla, skew, tau):
         ia, ja, a = f.arrays(which, A.nsup)
         assert np.array_equal(ia, ri) and np.array_equal(ja, rj), which
         assert np.array_equal(a, ra), which
+
+
+# ---- binary files in the reference's layout ----
+
+@pytest.mark.parametrize("kind", ["heat-upper", "seepage-full"])
+def test_matrix_binary_file_matches_reference(B, O, kind, tmp_path):
+    """bcm_matrix_write_bin writes the same bytes as the reference's
+    CSMatrixRS::WriteMtrBin (smatrixrs.cpp:3210-3227, compiled in
+    oracle/_ref) for the same matrix, and bcm_matrix_read_bin reads the
+    reference's file back to the same arrays."""
+    need_ref(O)
+    if kind == "heat-upper":
+        m, _ = B.gen_heat2d(7, 6, 9, 3)
+        symm = True
+    else:
+        m, _ = B.gen_seepage3d(4, 3, 3, 6, 0.3, 4)
+        symm = False
+    A = O.Block(*m.arrays())
+    ours, theirs = str(tmp_path / "ours.bin"), str(tmp_path / "ref.bin")
+    m.write_bin(ours)
+    O.ref_write_mtr_bin(A, theirs)
+    assert open(ours, "rb").read() == open(theirs, "rb").read()
+    back = B.Matrix.read_bin(theirs, symmetric=symm)
+    for u, v in zip(back.arrays(), m.arrays()):
+        assert np.array_equal(u, v)
+    assert back.info()["symmetric"] == int(symm)
+
+
+def test_vector_binary_file_matches_reference(B, O, tmp_path):
+    """bcm_vector_write_bin writes CSVector::WriteVectBin's bytes
+    (svector.cpp:537-546), the reference's ReadVectBin (svector.cpp:488-507)
+    reads them, and bcm_vector_read_bin reads the reference's file."""
+    need_ref(O)
+    v = np.asfortranarray(np.random.default_rng(9).standard_normal((123, 3)))
+    ours, theirs = str(tmp_path / "v_ours.bin"), str(tmp_path / "v_ref.bin")
+    B.vector_write_bin(ours, v)
+    O.ref_write_vect_bin(v, theirs)
+    assert open(ours, "rb").read() == open(theirs, "rb").read()
+    assert np.array_equal(O.ref_read_vect_bin(ours), v)
+    assert np.array_equal(B.vector_read_bin(theirs), v)
+
+
+def test_binary_files_reject_truncation(B, tmp_path):
+    m, _ = B.gen_seepage3d(3, 3, 2, 3, 0.3, 5)
+    p = str(tmp_path / "m.bin")
+    m.write_bin(p)
+    data = open(p, "rb").read()
+    open(p, "wb").write(data[: len(data) - 8])
+    with pytest.raises(B.BcmError) as e:
+        B.Matrix.read_bin(p)
+    assert e.value.code == B.E_ARG
+    with pytest.raises(B.BcmError):
+        B.vector_read_bin(str(tmp_path / "missing.bin"))

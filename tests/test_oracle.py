@@ -275,3 +275,94 @@ def test_block_gmres_oracle_pins(B, O):
     xe, re, he, fe = O.block_gmres(e["A"], e["L"], e["U"], Be, niter=10, eps=1e-10)
     assert re.nitperf == 1 and re.updates == 1
     assert np.linalg.norm(xe - np.linalg.solve(e["mo"].dense(), Be)) <= 1e-11 * np.linalg.norm(xe)
+
+
+# ---- pins of the restatements that have no reference counterpart ----
+
+def test_mlp_oracle_adam_matches_torch():
+    """mlp_oracle.adam (the rule the GPU Adam kernel is checked against)
+    against torch.optim.Adam in float64 on the CPU, over a sequence of steps
+    with changing gradients (bias correction by step count)."""
+    torch = pytest.importorskip("torch")
+    import mlp_oracle as M
+
+    rng = np.random.default_rng(7)
+    p = rng.standard_normal(257)
+    m = np.zeros_like(p)
+    v = np.zeros_like(p)
+    tp = torch.tensor(p.copy(), dtype=torch.float64, requires_grad=True)
+    opt = torch.optim.Adam([tp], lr=3e-3, betas=(0.9, 0.999), eps=1e-8)
+    for step in range(1, 26):
+        g = rng.standard_normal(p.size) * (1.0 + step)
+        p, m, v = M.adam(p, g, m, v, step, 3e-3)
+        opt.zero_grad()
+        tp.grad = torch.tensor(g, dtype=torch.float64)
+        opt.step()
+        np.testing.assert_allclose(p, tp.detach().numpy(), rtol=1e-12, atol=1e-15)
+
+
+def test_bicgstab_oracle_iterations_match_scipy(B, O, ns):
+    """orc_bicgstab (right-preconditioned BiCGStab, the nonsymmetric drop-in's
+    method) against scipy.sparse.linalg.bicgstab with the same explicit
+    preconditioner M^-1 = U^-1 L^-1 and the same absolute stopping level
+    eps ||b||: iteration counts within one (scipy counts a half-step exit as
+    an iteration the same way only when the callback sees it) and the same
+    solution to the tolerance."""
+    spla = pytest.importorskip("scipy.sparse.linalg")
+    A = ns["mo"].dense()
+    minv = lambda r: O.solveu(ns["U"], O.solvel(ns["L"], r))   # noqa: E731
+    M = spla.LinearOperator(A.shape, matvec=minv, dtype=np.float64)
+    b = ns["b"]
+    for eps in (1e-8, 1e-11):
+        count = [0]
+        xs, info = spla.bicgstab(A, b, rtol=0.0, atol=eps * np.linalg.norm(b), M=M, maxiter=500,
+                                 callback=lambda xk: count.__setitem__(0, count[0] + 1))
+        assert info == 0
+        xo, r = O.bicgstab(ns["A"], ns["L"], ns["U"], b, eps=eps)
+        assert abs(r.nitperf - count[0]) <= 1, (eps, r.nitperf, count[0])
+        assert rel(xo, xs) < 1e3 * eps
+
+
+def test_fgmres_oracle_is_the_minimal_residual_iterate(B, O, ns):
+    """orc_fgmres with a fixed preconditioner is right-preconditioned GMRES:
+    after k < m steps its iterate minimises ||b - A x|| over x0 + M^-1
+    K_k(A M^-1, r0). Checked against the dense least-squares minimiser over
+    that space for k = 1..8 (a dropped Hessenberg term, a wrong rotation sign
+    or a left-preconditioned basis fail it)."""
+    A = ns["mo"].dense()
+    n = A.shape[0]
+    minv = lambda r: O.solveu(ns["U"], O.solvel(ns["L"], r))   # noqa: E731
+    b = ns["b"]
+    x0 = np.random.default_rng(3).standard_normal(n) * 0.1
+    r0 = b - A @ x0
+    basis = [r0 / np.linalg.norm(r0)]
+    for k in range(1, 9):
+        Q, _ = np.linalg.qr(np.stack(basis, axis=1))
+        Z = np.stack([minv(Q[:, j]) for j in range(k)], axis=1)
+        y, *_ = np.linalg.lstsq(A @ Z, r0, rcond=None)
+        xbf = x0 + Z @ y
+        xo, r = O.fgmres(ns["A"], ns["L"], ns["U"], b, x0=x0, m=30, niter=k, eps=1e-300)
+        assert rel(xo, xbf) < 1e-9, k
+        basis.append(A @ minv(Q[:, k - 1]))
+
+
+@pytest.mark.parametrize("variant", [0, 1, 2], ids=["right", "left", "central"])
+@pytest.mark.parametrize("nrhs", [1, 3])
+def test_block_lanczos_oracle_matches_reference(B, O, variant, nrhs):
+    """orc_block_lanczos against the reference's own BlockLanczosRight / Left /
+    Central (lanczos.cpp:932-3225, compiled with its LAPACK into oracle/_ref)
+    on the same ILU2-preconditioned system: the same solution to 1e-12 (the
+    restatement inverts rho by elimination where the reference uses an SVD,
+    so the iterates agree to rounding, not bit for bit), at a loose and a
+    tight tolerance; and the solution solves the system."""
+    need_ref(O)
+    s = seepage(B, O, 5, 4, 3, 6, seed=21)
+    b = np.random.default_rng(5 + nrhs).standard_normal((s["A"].n, nrhs))
+    A = s["mo"].dense()
+    for eps in (1e-6, 1e-11):
+        xr = O.ref_block_lanczos(variant, s["A"], s["L"], s["U"], b, eps=eps)
+        xo, r = O.block_lanczos(variant, s["A"], s["L"], s["U"], b, eps=eps)
+        assert r.updates == 1 and r.breakdown == 0
+        assert rel(xo, xr) < 1e-12, (variant, eps)
+        res = max(np.linalg.norm(b[:, c] - A @ xo[:, c]) / np.linalg.norm(b[:, c]) for c in range(nrhs))
+        assert res < 10 * eps
