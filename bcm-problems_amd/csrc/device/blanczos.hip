@@ -400,9 +400,9 @@ LzResult blanczos_core(Matrix& A, Factor& F, Context& ctx, const SolveOptions& o
   const int ge = red_blocks(n);
   SweepSync* sync = ctx.sweep_sync();
 
-  auto sweep = [&](const DevTri& T, const double* in, double* out) {
+  auto sweep = [&](const DevTri& T, const double* in, double* out, const int* gate) {
     ProfScope ps(ctx, kProfSweep, sweep_bytes(T, nb));
-    launch_sweep(T, in, out, kF64, nb, s, done, sync);
+    launch_sweep(T, in, out, kF64, nb, s, gate, sync);
   };
   auto spmv = [&](const DevBsr& M, const double* in, double* out) {
     ProfScope ps(ctx, kProfSpmv, spmv_bytes(M, nb));
@@ -410,20 +410,21 @@ LzResult blanczos_core(Matrix& A, Factor& F, Context& ctx, const SolveOptions& o
   };
   // y = Ahat x (t1, t2 scratch; x, y distinct from them)
   auto aop = [&](const double* in, double* out) {
-    if (variant == 0) { sweep(im.fwd, in, t1); sweep(im.bwd, t1, t2); spmv(A.d, t2, out); }
-    else if (variant == 1) { spmv(A.d, in, t1); sweep(im.fwd, t1, t2); sweep(im.bwd, t2, out); }
-    else { sweep(im.bwd, in, t1); spmv(A.d, t1, t2); sweep(im.fwd, t2, out); }
+    if (variant == 0) { sweep(im.fwd, in, t1, done); sweep(im.bwd, t1, t2, done); spmv(A.d, t2, out); }
+    else if (variant == 1) { spmv(A.d, in, t1); sweep(im.fwd, t1, t2, done); sweep(im.bwd, t2, out, done); }
+    else { sweep(im.bwd, in, t1, done); spmv(A.d, t1, t2); sweep(im.fwd, t2, out, done); }
   };
   auto atop = [&](const double* in, double* out) {
-    if (variant == 0) { spmv(A.dt, in, t1); sweep(im.fwdt, t1, t2); sweep(im.bwdt, t2, out); }
-    else if (variant == 1) { sweep(im.fwdt, in, t1); sweep(im.bwdt, t1, t2); spmv(A.dt, t2, out); }
-    else { sweep(im.bwdt, in, t1); spmv(A.dt, t1, t2); sweep(im.fwdt, t2, out); }
+    if (variant == 0) { spmv(A.dt, in, t1); sweep(im.fwdt, t1, t2, done); sweep(im.bwdt, t2, out, done); }
+    else if (variant == 1) { sweep(im.fwdt, in, t1, done); sweep(im.bwdt, t1, t2, done); spmv(A.dt, t2, out); }
+    else { sweep(im.bwdt, in, t1, done); spmv(A.dt, t1, t2); sweep(im.fwdt, t2, out, done); }
   };
-  // out = s-map of the iterate: Right U^-1 L^-1 s, Left s, Central U^-1 s
-  auto smap = [&](const double* in, double* out) {
-    if (variant == 0) { sweep(im.fwd, in, t1); sweep(im.bwd, t1, out); }
+  // out = s-map of the iterate: Right U^-1 L^-1 s, Left s, Central U^-1 s.
+  // The final map runs ungated (gate null): the stop flag is set by then.
+  auto smap = [&](const double* in, double* out, const int* gate) {
+    if (variant == 0) { sweep(im.fwd, in, t1, gate); sweep(im.bwd, t1, out, gate); }
     else if (variant == 1) launch_copy((int64_t)n * nb, in, out, s);
-    else sweep(im.bwd, in, out);
+    else sweep(im.bwd, in, out, gate);
   };
   auto small = [&](int op, const double* p1, const double* p2, int target, int iter) {
     hipLaunchKernelGGL(k_lz_small, dim3(1), dim3(256), 0, s, op, nb, p1, nbr, p2, nbr, sm, ks, target, variant,
@@ -447,13 +448,14 @@ LzResult blanczos_core(Matrix& A, Factor& F, Context& ctx, const SolveOptions& o
   };
 
   hipLaunchKernelGGL(k_lz_clear, dim3(1), dim3(64), 0, s, ks);
+  launch_fill((int64_t)n * nb, 0.0, S, s);  // s_0 = 0 (the context's buffers keep earlier solves)
   // r0 = b - A x0, its norms, and the variant's starting block
   launch_spmv(A.d, x, r, b, nb, nullptr, nullptr, ctx.rw, s);
   if (variant != 0) {
     gram(r, part);
     small(kOpResi0, part, nullptr, 0, 0);
-    if (variant == 1) { sweep(im.fwd, r, t1); sweep(im.bwd, t1, t2); }
-    else sweep(im.fwd, r, t2);
+    if (variant == 1) { sweep(im.fwd, r, t1, done); sweep(im.bwd, t1, t2, done); }
+    else sweep(im.fwd, r, t2, done);
     launch_copy((int64_t)n * nb, t2, r, s);
   }
   qr(r, U, kSBeta);
@@ -476,7 +478,7 @@ LzResult blanczos_core(Matrix& A, Factor& F, Context& ctx, const SolveOptions& o
     small(kOpGivens, nullptr, nullptr, 0, k);
     LZ_DISPATCH(nb, hipLaunchKernelGGL(k_lz_wupd<NBC>, dim3(ge), dim3(256), 0, s, n, W, V, S, sm, done));
     if (k % ichk == 0) {
-      smap(S, t2);
+      smap(S, t2, done);
       launch_axpby((int64_t)n * nb, 1.0, x, 1.0, t2, s);  // t2 = x0 + map(s)
       launch_spmv(A.d, t2, r, b, nb, nullptr, nullptr, ctx.rw, s, done);
       gram(r, part);
@@ -496,7 +498,7 @@ LzResult blanczos_core(Matrix& A, Factor& F, Context& ctx, const SolveOptions& o
   res.converged = ctx.host_state->st[kLzConv];
   res.breakdown = ctx.host_state->st[kLzBreak];
   // x += map(s) (lanczos.cpp:1593-1596 / 2441 / 3196-3199), also after a breakdown
-  smap(S, t2);
+  smap(S, t2, nullptr);
   launch_axpby((int64_t)n * nb, 1.0, t2, 1.0, x, s);
   return res;
 }
