@@ -135,6 +135,7 @@ SIGNATURES = {
     "bcm_factor_info": (C.c_int, [_P, C.c_int, _IP, C.POINTER(C.c_longlong), _IP, _IP]),
     "bcm_factor_export": (C.c_int, [_P, C.c_int, _IP, _IP, _DP]),
     "bcm_device_count": (C.c_int, [_IP]),
+    "bcm_shutdown": (C.c_int, []),
     "bcm_dev_alloc": (C.c_int, [C.c_ulonglong, C.POINTER(C.c_void_p)]),
     "bcm_dev_free": (C.c_int, [C.c_void_p]),
     "bcm_copy_to_device": (C.c_int, [C.c_void_p, C.c_void_p, C.c_ulonglong]),
@@ -144,6 +145,8 @@ SIGNATURES = {
     "bcm_context_destroy": (C.c_int, [_P]),
     "bcm_context_sync": (C.c_int, [_P]),
     "bcm_context_set_sweep_mode": (C.c_int, [_P, C.c_int]),
+    "bcm_context_set_sweep_waves": (C.c_int, [_P, C.c_int]),
+    "bcm_context_raise_wait_flag": (C.c_int, [_P]),
     "bcm_context_profile": (C.c_int, [_P, C.c_int]),
     "bcm_context_profile_read": (C.c_int, [_P, C.POINTER(Profile), C.c_int]),
     "bcm_mvm": (C.c_int, [_P, _P, C.c_void_p, C.c_void_p, C.c_int]),
@@ -495,6 +498,15 @@ class Context:
         """0: one launch per dependency level; 1: one sync-free launch per sweep."""
         _check(lib().bcm_context_set_sweep_mode(self.h, mode))
 
+    def set_sweep_waves(self, waves_per_cu: int):
+        """Resident waves per CU of the sync-free sweeps (default 8)."""
+        _check(lib().bcm_context_set_sweep_waves(self.h, waves_per_cu))
+
+    def raise_wait_flag(self):
+        """Test hook: the next solve's sync-free waits exit at once and the
+        solve is repeated with cooperative launches."""
+        _check(lib().bcm_context_raise_wait_flag(self.h))
+
     def profile(self, enable: bool = True):
         _check(lib().bcm_context_profile(self.h, int(enable)))
 
@@ -559,6 +571,15 @@ def device_count() -> int:
     n = C.c_int()
     _check(lib().bcm_device_count(C.byref(n)))
     return n.value
+
+
+def shutdown():
+    """bcm_shutdown: wait for the device and free the library's pinned
+    staging buffers; call after every handle is gone, before exit."""
+    import gc
+
+    gc.collect()
+    _check(lib().bcm_shutdown())
 
 
 class Batch:

@@ -509,32 +509,48 @@ int bcm_device_count(int* count) {
   });
 }
 
+int bcm_shutdown(void) {
+  return guarded([&] {
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n < 1) return;
+    BCM_HIP(hipDeviceSynchronize());
+    bcm::release_staging();
+  });
+}
+
 int bcm_dev_alloc(unsigned long long bytes, void** out) {
   return guarded([&] {
     need(out != nullptr, "dev_alloc: out is null");
     need_device();
     *out = nullptr;
+    bcm::SetupGuard g;
     BCM_HIP(hipMalloc(out, bytes ? bytes : 1));
   });
 }
 
 int bcm_dev_free(void* p) {
   return guarded([&] {
-    if (p) BCM_HIP(hipFree(p));
+    if (!p) return;
+    bcm::SetupGuard g;
+    BCM_HIP(hipFree(p));
   });
 }
 
 int bcm_copy_to_device(void* dst, const void* src, unsigned long long bytes) {
   return guarded([&] {
     need(dst && src, "copy_to_device: null pointer");
-    if (bytes) BCM_HIP(hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice));
+    if (!bytes) return;
+    bcm::SetupGuard g;
+    BCM_HIP(hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice));
   });
 }
 
 int bcm_copy_to_host(void* dst, const void* src, unsigned long long bytes) {
   return guarded([&] {
     need(dst && src, "copy_to_host: null pointer");
-    if (bytes) BCM_HIP(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost));
+    if (!bytes) return;
+    bcm::SetupGuard g;
+    BCM_HIP(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost));
   });
 }
 
@@ -543,6 +559,7 @@ int bcm_dev_zero(void* p, unsigned long long bytes) {
     need(p != nullptr, "dev_zero: null pointer");
     if (bytes) {
       // the null stream does not order with the library's non-blocking streams: complete here
+      bcm::SetupGuard g;
       BCM_HIP(hipMemset(p, 0, bytes));
       BCM_HIP(hipDeviceSynchronize());
     }
@@ -575,6 +592,26 @@ int bcm_context_set_sweep_mode(bcm_context c, int mode) {
     BCM_HIP(hipStreamSynchronize(c->c.stream));
     c->c.sweep_mode = mode == 0 ? 0 : 1;
     c->c.sync.level_order = mode == 1;
+  });
+}
+
+int bcm_context_set_sweep_waves(bcm_context c, int waves_per_cu) {
+  return guarded([&] {
+    need(c != nullptr && waves_per_cu >= 1 && waves_per_cu <= 64, "context_set_sweep_waves: bad argument");
+    BCM_HIP(hipStreamSynchronize(c->c.stream));
+    c->c.sync.waves_per_cu = waves_per_cu;
+  });
+}
+
+int bcm_context_raise_wait_flag(bcm_context c) {
+  return guarded([&] {
+    need(c != nullptr, "context_raise_wait_flag: null handle");
+    bcm::Context& x = c->c;
+    x.sync.ensure(0, x.stream);
+    const int one = 1;
+    bcm::SetupGuard g;
+    BCM_HIP(hipMemcpyAsync(x.sync.ctl.as<int>() + 2, &one, sizeof(int), hipMemcpyHostToDevice, x.stream));
+    BCM_HIP(hipStreamSynchronize(x.stream));
   });
 }
 

@@ -633,6 +633,7 @@ void solve_nr(Batch& bt, Context& ctx, const SolveOptions& opt, const double* b,
         bt.graph.reset(new BatchGraph());
         g = bt.graph.get();
         ctx.sync.cooperative = false;
+        CaptureGuard capture;  // no set-up HIP calls of other threads until instantiation
         BCM_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
         try {
           for (int it = 0; it < every; ++it) iteration<NR>(bt, ctx, w, opt, im.fwd, im.bwd, flex);

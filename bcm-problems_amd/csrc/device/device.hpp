@@ -35,6 +35,31 @@ enum Prec : int { kF64 = 0, kF32 = 1, kBF16 = 2 };
 
 inline size_t prec_bytes(int p) { return p == kF64 ? 8 : (p == kF32 ? 4 : 2); }
 
+// Stream capture and device set-up on other host threads exclude each other.
+// A capture (begin to instantiation) holds the lock exclusively; the
+// library's set-up HIP calls (allocation, frees, pinned staging, copies,
+// helper streams) hold it shared, one call group at a time, so a build on a
+// worker thread never interleaves with a capture on the solving thread.
+// Waiting captures go first; set-up calls made by the capturing thread
+// itself do not take the lock.
+struct SetupGuard {
+  bool held;
+  SetupGuard();
+  ~SetupGuard();
+  SetupGuard(const SetupGuard&) = delete;
+  SetupGuard& operator=(const SetupGuard&) = delete;
+};
+struct CaptureGuard {
+  CaptureGuard();
+  ~CaptureGuard();
+  CaptureGuard(const CaptureGuard&) = delete;
+  CaptureGuard& operator=(const CaptureGuard&) = delete;
+};
+
+// Frees the pinned staging buffers kept for uploads (they are allocated again
+// on demand); called when no build is running.
+void release_staging();
+
 // Owning device buffer.
 struct DevBuf {
   void* p = nullptr;

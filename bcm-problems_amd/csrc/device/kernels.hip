@@ -22,6 +22,7 @@
 
 #include <algorithm>
 #include <atomic>
+#include <condition_variable>
 #include <cstring>
 #include <exception>
 #include <mutex>
@@ -643,6 +644,10 @@ __global__ __launch_bounds__(64) void k_sweep_free(int nsup, int n, int forward,
     for (int i = 0; i < PF; ++i) pf[i] = (act && g + i * G < ncol) ? (Acc)ldnt(lp + i * G * B) : Acc(0);
 #pragma unroll
     for (int i = 0; i < PD; ++i) pd[i] = (act && g + i * G < B) ? (Acc)ldnt(d + (g + i * G) * B + r) : Acc(0);
+    // the row's right-hand side is known up front too: off the wait's critical path
+    Acc xv[NR];
+#pragma unroll
+    for (int q = 0; q < NR; ++q) xv[q] = lane < B ? (Acc)x[(size_t)q * n + (size_t)row * B + lane] : Acc(0);
     Acc acc[NR];
 #pragma unroll
     for (int q = 0; q < NR; ++q) acc[q] = Acc(0);
@@ -668,7 +673,7 @@ __global__ __launch_bounds__(64) void k_sweep_free(int nsup, int n, int forward,
       Acc s = acc[q];
 #pragma unroll
       for (int h = 1; h < G; ++h) s += __shfl(acc[q], lane + h * B);
-      if (lane < B) v[q][lane] = (Acc)x[(size_t)q * n + (size_t)row * B + lane] - s;
+      if (lane < B) v[q][lane] = xv[q] - s;
     }
     __syncthreads();
     Acc z[NR];
@@ -758,23 +763,74 @@ int grid_for(int64_t n, int threads) {
 // ---------------------------------------------------------------------------
 // Buffers and host-side construction
 
+namespace {
+// writer-preferring shared lock (a waiting capture blocks new set-up calls)
+struct CaptureLock {
+  std::mutex mu;
+  std::condition_variable cv;
+  int readers = 0, writers_waiting = 0;
+  bool writer = false;
+};
+CaptureLock g_capture;
+thread_local int t_capturing = 0;
+thread_local int t_setup = 0;  // nesting depth: only the outermost guard takes the lock
+}  // namespace
+
+SetupGuard::SetupGuard() : held(t_capturing == 0 && t_setup == 0) {
+  ++t_setup;
+  if (!held) return;
+  std::unique_lock<std::mutex> l(g_capture.mu);
+  g_capture.cv.wait(l, [] { return !g_capture.writer && g_capture.writers_waiting == 0; });
+  ++g_capture.readers;
+}
+
+SetupGuard::~SetupGuard() {
+  --t_setup;
+  if (!held) return;
+  std::lock_guard<std::mutex> l(g_capture.mu);
+  if (--g_capture.readers == 0) g_capture.cv.notify_all();
+}
+
+CaptureGuard::CaptureGuard() {
+  if (t_setup > 0) throw Error(kErrInternal, "stream capture inside device set-up");
+  std::unique_lock<std::mutex> l(g_capture.mu);
+  ++g_capture.writers_waiting;
+  g_capture.cv.wait(l, [] { return !g_capture.writer && g_capture.readers == 0; });
+  --g_capture.writers_waiting;
+  g_capture.writer = true;
+  ++t_capturing;
+}
+
+CaptureGuard::~CaptureGuard() {
+  --t_capturing;
+  std::lock_guard<std::mutex> l(g_capture.mu);
+  g_capture.writer = false;
+  g_capture.cv.notify_all();
+}
+
 void DevBuf::alloc(size_t nbytes) {
   release();
   if (nbytes == 0) return;
+  SetupGuard g;
   BCM_HIP(hipMalloc(&p, nbytes));
   bytes = nbytes;
 }
 
 void DevBuf::release() {
-  if (p) (void)hipFree(p);
+  if (p) {
+    SetupGuard g;
+    (void)hipFree(p);
+  }
   p = nullptr;
   bytes = 0;
 }
 
 void DevBuf::upload(const void* src, size_t nbytes, hipStream_t s) {
   if (nbytes > bytes) alloc(nbytes);
-  if (nbytes) BCM_HIP(hipMemcpyAsync(p, src, nbytes, hipMemcpyHostToDevice, s));
-  if (nbytes) BCM_HIP(hipStreamSynchronize(s));
+  if (!nbytes) return;
+  SetupGuard g;
+  BCM_HIP(hipMemcpyAsync(p, src, nbytes, hipMemcpyHostToDevice, s));
+  BCM_HIP(hipStreamSynchronize(s));
 }
 
 int device_cu_count() {
@@ -847,6 +903,7 @@ void* stage_take() {
     }
   }
   void* p = nullptr;
+  SetupGuard g;
   BCM_HIP(hipHostMalloc(&p, kStageBytes, hipHostMallocDefault));
   return p;
 }
@@ -856,6 +913,16 @@ void stage_give(void* p) {
   g_stage_free.push_back(p);
 }
 }  // namespace
+
+void release_staging() {
+  std::vector<void*> bufs;
+  {
+    std::lock_guard<std::mutex> g(g_stage_mu);
+    bufs.swap(g_stage_free);
+  }
+  SetupGuard g;
+  for (void* p : bufs) (void)hipHostFree(p);
+}
 
 // Converts `count` fp64 values to `prec` and copies them to dst (device),
 // through a pinned staging buffer one chunk at a time.
@@ -875,6 +942,7 @@ static void upload_values_at(const double* v, size_t count, int prec, void* dst)
         uint16_t* h = static_cast<uint16_t*>(st);
         for (size_t i = 0; i < m; ++i) h[i] = to_bf16((float)v[i0 + i]);
       }
+      SetupGuard g;
       BCM_HIP(hipMemcpy(static_cast<char*>(dst) + i0 * es, st, m * es, hipMemcpyHostToDevice));
     }
   } catch (...) {
@@ -1282,6 +1350,7 @@ bool dev_bsr_upload_symm_parts(const std::vector<const BlockCsr*>& uppers, DevBs
   if (nfull > 0) {
     // a stream of its own, so that the build does not wait for work other
     // streams run meanwhile (e.g. the solve of the previous batch)
+    SetupGuard g;
     hipStream_t xs = nullptr;
     BCM_HIP(hipStreamCreateWithFlags(&xs, hipStreamNonBlocking));
     hipLaunchKernelGGL(k_expand_symm, dim3((unsigned)std::min<int64_t>(nfull, 1 << 16)), dim3(256), 0, xs, nfull,

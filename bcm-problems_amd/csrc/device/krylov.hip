@@ -378,9 +378,13 @@ const Factor::Image& Factor::to_device(int prec, bool with_transpose) {
 }
 
 Context::Context() {
-  BCM_HIP(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+  {
+    SetupGuard g;
+    BCM_HIP(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+  }
   own_stream = true;
   kstate.alloc(sizeof(KState));
+  SetupGuard g;
   BCM_HIP(hipMemsetAsync(kstate.p, 0, sizeof(KState), stream));
   BCM_HIP(hipStreamSynchronize(stream));
   BCM_HIP(hipHostMalloc((void**)&host_state, sizeof(KState), hipHostMallocDefault));
@@ -389,6 +393,7 @@ Context::Context() {
 }
 
 Context::~Context() {
+  SetupGuard g;
   if (stream) (void)hipStreamSynchronize(stream);
   for (auto& p : pending) { (void)hipEventDestroy(p.e0); (void)hipEventDestroy(p.e1); }
   for (auto e : ev_pool) (void)hipEventDestroy(e);

@@ -923,6 +923,8 @@ def main():
         run_heat(args, B)
     else:
         run_dropin(args, B)
+    # release what the library keeps for the process before the runtime's teardown
+    B.shutdown()
 
 
 if __name__ == "__main__":

@@ -167,6 +167,13 @@ int bcm_dev_free(void* p);
 int bcm_copy_to_device(void* dst, const void* src, unsigned long long bytes);
 int bcm_copy_to_host(void* dst, const void* src, unsigned long long bytes);
 int bcm_dev_zero(void* p, unsigned long long bytes);
+/* Waits for the device, then frees what the library keeps between calls for
+ * the life of the process (the pinned host staging buffers of matrix and
+ * factor uploads). Call it after destroying every handle and before the
+ * process exits, so that no HIP resource of the library is left to the
+ * runtime's own teardown; later calls allocate again on demand. Not while a
+ * build runs on another thread. No counterpart in the reference. */
+int bcm_shutdown(void);
 
 /* ---- device context ---- */
 int bcm_context_create(bcm_context* out);
@@ -179,6 +186,14 @@ int bcm_context_sync(bcm_context c);
  * natural sweep order. The schedules agree to rounding: per row they add
  * the same products, grouped differently across lanes for some block sizes. */
 int bcm_context_set_sweep_mode(bcm_context c, int mode);
+/* Resident waves per CU of the sync-free sweeps (1..64, default 8; fewer
+ * waves than block rows make each wave own several rows in turn). */
+int bcm_context_set_sweep_waves(bcm_context c, int waves_per_cu);
+/* Test hook: marks the sync-free sweeps' dependency wait as having reached its
+ * bound, so that every wait of the next solve exits at once and the solve
+ * takes its recovery path (repeated from x0 with cooperative launches). No
+ * counterpart in the reference. */
+int bcm_context_raise_wait_flag(bcm_context c);
 
 /* Per-kernel-class timing with HIP events on the context's stream (the
  * reference's clock() statistics per operator, cg.cpp:743-764). Times are

@@ -117,6 +117,48 @@ def test_batch_schedules_agree(B, ctx, batch, cells):
     batch.set_tune(sweep_mode=1, graph=0, check_every=8, waves_per_cu=8)
 
 
+def test_graph_capture_while_another_thread_builds(B, ctx, cells):
+    """Graph-replayed solves on this thread while another thread builds and
+    drops batches (device allocation and frees, pinned staging copies, the
+    symmetric expansion's helper stream): every capture excludes the other
+    thread's set-up calls, and each solve (a fresh capture, since setting the
+    schedule drops the graph) equals a quiet run bit for bit."""
+    import threading
+
+    mats = [(c["mo"], c["f"]) for c in cells]
+    bt = B.Batch(mats, prec_mask=1 << B.F32)
+    tune = dict(sweep_mode=0, graph=1, check_every=4, waves_per_cu=8)
+    bt.set_tune(**tune)
+    x_ref, st, its_ref, _ = run(B, ctx, bt, cells, eps=1e-10, precision=B.F32)
+    assert st.converged == st.pairs
+    x_ref = np.concatenate([v.ravel() for v in x_ref])
+    stop, built, err = threading.Event(), [0], []
+
+    def builder():
+        try:
+            while not stop.is_set():
+                B.Batch(mats, prec_mask=1 << B.F32)  # built on this thread and dropped here
+                built[0] += 1
+        except BaseException as e:  # reported by the main thread
+            err.append(e)
+
+    th = threading.Thread(target=builder)
+    th.start()
+    try:
+        for k in range(4):
+            print(f"concurrent build + capture, solve {k}", flush=True)
+            bt.set_tune(**tune)
+            xs, st, its, rr = run(B, ctx, bt, cells, eps=1e-10, precision=B.F32)
+            x = np.concatenate([v.ravel() for v in xs])
+            assert st.converged == st.pairs
+            assert np.array_equal(x, x_ref) and np.array_equal(its, its_ref), k
+    finally:
+        stop.set()
+        th.join()
+    assert not err, err
+    assert built[0] >= 1
+
+
 def test_batch_warm_start_fallback_and_zero_rhs(B, ctx, batch, cells, oracle_sol):
     good = batch.gather([xo * (1 + 1e-7) for xo, _ in oracle_sol])
     xs, st, its, rr = run(B, ctx, batch, cells, x0=good, fallback=True, eps=1e-10)
