@@ -120,6 +120,8 @@ SIGNATURES = {
     "bcm_gen_elastic_cell": (C.c_int, [C.c_int, C.c_double, C.c_double, C.c_double, C.c_double,
                                        C.c_ulonglong, C.POINTER(_P), _DP]),
     "bcm_rcm_order": (C.c_int, [_P, _IP]),
+    "bcm_part_binary_tree_nd": (C.c_int, [_P, C.c_int, _IP, _IP, _IP, _IP]),
+    "bcm_find_separator": (C.c_int, [C.c_int, _IP, _IP, _IP, _IP]),
     "bcm_matrix_reorder": (C.c_int, [_P, _IP, C.POINTER(_P)]),
     "bcm_combine_lu": (C.c_int, [_P, C.POINTER(_P), C.POINTER(_P)]),
     "bcm_dsyev": (C.c_int, [C.c_int, _DP, _DP]),
@@ -329,6 +331,16 @@ class Matrix:
         o = np.zeros(self.info()["nsup"], np.int32)
         _check(lib().bcm_rcm_order(self.h, _ip(o)))
         return o
+
+    def part_binary_tree_nd(self, nproc: int = 1):
+        """bcm_part_binary_tree_nd: (order, blks, blk2cpu) of the nested
+        dissection and its binary tree over nproc virtual ranks."""
+        o = np.zeros(self.info()["nsup"], np.int32)
+        nb = C.c_int()
+        blks = np.zeros(2 * nproc + 1, np.int32)
+        b2c = np.zeros(2 * nproc, np.int32)
+        _check(lib().bcm_part_binary_tree_nd(self.h, nproc, _ip(o), C.byref(nb), _ip(blks), _ip(b2c)))
+        return o, blks[:nb.value + 1].copy(), b2c[:nb.value].copy()
 
     def reorder(self, order) -> "Matrix":
         order = np.ascontiguousarray(order, dtype=np.int32)
@@ -571,6 +583,15 @@ def device_count() -> int:
     n = C.c_int()
     _check(lib().bcm_device_count(C.byref(n)))
     return n.value
+
+
+def find_separator(ia, ja):
+    """bcm_find_separator (CSMatrix::FindSeparator) on a symmetric pattern."""
+    ia = np.ascontiguousarray(ia, np.int32)
+    ja = np.ascontiguousarray(ja, np.int32)
+    i1, i2 = C.c_int(), C.c_int()
+    _check(lib().bcm_find_separator(len(ia) - 1, _ip(ia), _ip(ja), C.byref(i1), C.byref(i2)))
+    return i1.value, i2.value
 
 
 def shutdown():

@@ -332,6 +332,27 @@ int bcm_matrix_reorder(bcm_matrix m, const int* order, bcm_matrix* out) {
   });
 }
 
+int bcm_part_binary_tree_nd(bcm_matrix m, int nproc, int* order, int* nblks, int* blks, int* blk2cpu) {
+  return guarded([&] {
+    need(m && order && nblks && blks && blk2cpu, "part_binary_tree_nd: null argument");
+    need(nproc >= 1 && (nproc & (nproc - 1)) == 0, "part_binary_tree_nd: nproc must be a power of two");
+    const std::vector<int> o = bcm::nd_order(m->m.h);
+    const bcm::BlockCsr ordered = m->m.symmetric ? bcm::ord_mtr_symm(m->m.h, o) : bcm::ord_mtr(m->m.h, o);
+    const bcm::NdTree t = bcm::part_binary_tree_nd(ordered, nproc);
+    std::copy(o.begin(), o.end(), order);
+    *nblks = (int)t.blk2cpu.size();
+    std::copy(t.blks.begin(), t.blks.end(), blks);
+    std::copy(t.blk2cpu.begin(), t.blk2cpu.end(), blk2cpu);
+  });
+}
+
+int bcm_find_separator(int n, const int* ia, const int* ja, int* ind1, int* ind2) {
+  return guarded([&] {
+    need(n >= 0 && ia && ja && ind1 && ind2, "find_separator: bad argument");
+    bcm::find_separator(n, ia, ja, *ind1, *ind2);
+  });
+}
+
 int bcm_matrix_write_bin(bcm_matrix m, const char* path) {
   return guarded([&] {
     need(m && path, "matrix_write_bin: bad argument");

@@ -4,7 +4,8 @@
 // with the Krylov phase on the GPU.
 //
 // Steps kept from the reference: callback assembly (rows sorted by column,
-// blocks copied in sorted order), RCM profile ordering (ordtype > 0),
+// blocks copied in sorted order), RCM profile ordering (ordtype > 0; the
+// parallel solver's nested dissection, PartBinaryTreeND),
 // symmetric / general reordering, IC2 (symmetric) or CombineLU + ILU2
 // (general), ordering of the right-hand side and initial guess, the Krylov
 // solve, inverse ordering and CSTORESOL per block.
@@ -187,15 +188,19 @@ int run(bool symmetric, bool par, bcm_params* p, const Callbacks& cb) {
     gather_blocks(cb, a, cb.frhs, b);
     gather_blocks(cb, a, cb.finiguess, x);
 
-    // ordering: ordtype > 0 -> RCM profile ordering, 0 -> identity
+    // ordering: the parallel solver always dissects (PartBinaryTreeND,
+    // solvers.cpp:5890: METIS_NodeND's layout, here this library's own
+    // dissection); otherwise ordtype > 0 -> RCM profile ordering, 0 -> identity
     std::vector<int> order(a.nsup);
-    if (prm.ordtype != 0) order = bcm::rcm_order(a);
+    const bool reorder = par || prm.ordtype != 0;
+    if (par) order = bcm::nd_order(a);
+    else if (prm.ordtype != 0) order = bcm::rcm_order(a);
     else std::iota(order.begin(), order.end(), 0);
     bcm::Matrix mat;
     mat.symmetric = symmetric;
-    mat.h = prm.ordtype != 0 ? (symmetric ? bcm::ord_mtr_symm(a, order) : bcm::ord_mtr(a, order)) : a;
+    mat.h = reorder ? (symmetric ? bcm::ord_mtr_symm(a, order) : bcm::ord_mtr(a, order)) : a;
     std::vector<int> sp_new = mat.h.sprnd;
-    if (prm.ordtype != 0) {
+    if (reorder) {
       bcm::ord_vect('D', a.sprnd, sp_new, order, b.data(), a.n(), 1);
       bcm::ord_vect('D', a.sprnd, sp_new, order, x.data(), a.n(), 1);
     }
@@ -232,7 +237,7 @@ int run(bool symmetric, bool par, bcm_params* p, const Callbacks& cb) {
     // rules report the FP64 true relative residual here (the reference leaves 0)
     prm.resfin = opt.kind == bcm::kPcg && opt.ittype == 2 ? st.errmin : st.relres_true;
     prm.timeitr = st.seconds;
-    if (prm.ordtype != 0) bcm::ord_vect('I', a.sprnd, sp_new, order, x.data(), a.n(), 1);
+    if (reorder) bcm::ord_vect('I', a.sprnd, sp_new, order, x.data(), a.n(), 1);
     for (int i = 0; i < a.nsup; ++i) cb.fstoresol(cb.obj, i, &x[a.sprnd[i]]);
     if (st.breakdown) throw bcm::Error(bcm::kErrBreakdown, "Krylov breakdown (best iterate stored)");
     return BCM_OK;

@@ -12,6 +12,7 @@
 #include <cstdint>
 #include <stdexcept>
 #include <string>
+#include <utility>
 #include <vector>
 
 namespace bcm {
@@ -85,6 +86,27 @@ void symm_pattern(const BlockCsr& a, std::vector<int>& ias, std::vector<int>& ja
 void genrcm(int n, const std::vector<int>& xadj, const std::vector<int>& adj, std::vector<int>& perm);
 // Profile ordering as in OrderPrfMtr(1): order[old] = new.
 std::vector<int> rcm_order(const BlockCsr& a);
+
+// ---- nested dissection and the parallel solver's tree (nd.cpp) ----
+
+// Nested-dissection ordering of the symmetrized block pattern with the layout
+// METIS_NodeND gives OrderPrfMtr(-1) (ordmatr.cpp:28-85): each part ordered
+// recursively, then its separator. order[old] = new.
+std::vector<int> nd_order(const BlockCsr& a);
+// CSMatrix::FindSeparator (smatrix.cpp:4355-4488) on a symmetric pattern of n
+// rows: ind1 = first row of part 2, ind2 = first separator row; -1 if none.
+void find_separator(int n, const int* ia, const int* ja, int& ind1, int& ind2);
+// Blocks of the binary dissection tree over nproc (a power of two) virtual
+// ranks, as CSMatrix::PartBinaryTreeND (ordmatr.cpp:1204-1396, 1698-1805) and
+// CTree::PartitionTree (tree.cpp:2414-2455) derive them from an ND-ordered
+// matrix: blocks in matrix order (leaves and separators), each with the rank
+// owning it (a separator: the first rank of its subtree).
+struct NdTree {
+  int nproc = 1;
+  std::vector<int> blks, blk2cpu;
+  std::vector<std::pair<int, int>> leaf_rows, separators;  // [first, last + 1) block rows
+};
+NdTree part_binary_tree_nd(const BlockCsr& ordered, int nproc);
 // Permuted copy, order[old] = new, columns re-sorted ascending per row.
 BlockCsr ord_mtr(const BlockCsr& a, const std::vector<int>& order);
 // Full symmetric matrix from upper-triangular storage (lower blocks transposed).

@@ -113,6 +113,18 @@ int bcm_gen_elastic_cell(int m, double radius, double contrast, double poisson, 
 
 /* ---- ordering (ordmatr.cpp:28-227 OrderPrfMtr(1) -> rcm.cpp genrcm) ---- */
 int bcm_rcm_order(bcm_matrix m, int* order);                       /* order[old] = new */
+/* CSMatrix::PartBinaryTreeND (ordmatr.cpp:1698-1805) in one process: the
+ * nested-dissection ordering of the block pattern (this library's own
+ * dissection in place of METIS_NodeND, with its layout: parts, then their
+ * separator) and the blocks of the binary dissection tree over nproc virtual
+ * ranks (a power of two), as FindSeparator and CTree::PartitionTree derive
+ * them (ordmatr.cpp:1204-1396, tree.cpp:2414). order[old] = new (nsup
+ * entries); blks gets nblks + 1 entries (at most 2 nproc), blk2cpu nblks. */
+int bcm_part_binary_tree_nd(bcm_matrix m, int nproc, int* order, int* nblks, int* blks, int* blk2cpu);
+/* CSMatrix::FindSeparator (smatrix.cpp:4355-4488) on a symmetric pattern of
+ * n rows: ind1 = first row of the second part, ind2 = first separator row
+ * (-1, -1 when the pattern has no such split). */
+int bcm_find_separator(int n, const int* ia, const int* ja, int* ind1, int* ind2);
 /* OrdMtrSymm (smatrixrs.cpp:456) for symmetric, OrdMtr (smatrixrs.cpp:345) otherwise. */
 int bcm_matrix_reorder(bcm_matrix m, const int* order, bcm_matrix* out);
 /* CSVector::OrdVect (svector.cpp:418): type 'D' applies order, 'I' undoes it. */
@@ -358,8 +370,11 @@ int bcm_abstract_solver(void* pgener, bcm_params* p, bcm_cb_nsup fnsup, bcm_cb_s
                         bcm_cb_blockrow fblockrow, bcm_cb_rhs frhs, bcm_cb_iniguess finiguess,
                         bcm_cb_storesol fstoresol);
 
-/* AbstractParSolver (solvers.cpp:5799-6224) on one rank: assemble, RCM (the
- * reference's nested-dissection ordering needs METIS), ILU2, then the kind of
+/* AbstractParSolver (solvers.cpp:5799-6224) on one rank: assemble, nested
+ * dissection (PartBinaryTreeND, solvers.cpp:5890, with this library's own
+ * dissection in place of METIS_NodeND; on one rank the tree is a single
+ * node), ILU2 of the dissected matrix (what Ilu2Schur reduces to on one rank
+ * is not restated: the serial Ilu2 is used), then the kind of
  * Solver(tree) (solvers.cpp:1865-2041): ittype 1 parallel-form Cg, 2
  * two-sided Lanczos (relative test against ||b||, lanczos.cpp:797), otherwise
  * block GMRES. comm (the reference's CMPIComm*) is not used: the system lives

@@ -97,6 +97,7 @@ def ref() -> C.CDLL:
                                         _DP, _IP, _DP]
         l.ref_genrcm.argtypes = [C.c_int, _IP, _IP, _IP]
         l.ref_order_prf.argtypes = mat + [_IP]
+        l.ref_find_separator.argtypes = [C.c_int, _IP, _IP, _IP, _IP]
         l.ref_ordmtr.argtypes = [C.c_int] + mat + [_IP, _IP, _IP, _IP, _IP, _DP]
         l.ref_combinelu.argtypes = [C.c_int] + mat + [_IP, _IP, _IP, _IP, _DP]
         l.ref_ich2.argtypes = mat + [C.c_double] * 4 + [C.c_char_p, C.c_int]
@@ -300,6 +301,15 @@ def ref_order_prf(m: Block):
     order = np.zeros(m.nsup, np.int32)
     ref().ref_order_prf(*m.args(), order.ctypes.data_as(_IP))
     return order
+
+
+def ref_find_separator(ia, ja):
+    """The reference's CSMatrix::FindSeparator (smatrix.cpp:4355) on a pattern."""
+    ia = np.ascontiguousarray(ia, np.int32)
+    ja = np.ascontiguousarray(ja, np.int32)
+    i1, i2 = C.c_int(), C.c_int()
+    ref().ref_find_separator(len(ia) - 1, ia.ctypes.data_as(_IP), ja.ctypes.data_as(_IP), C.byref(i1), C.byref(i2))
+    return i1.value, i2.value
 
 
 def _take(fn, *args, nsup):

@@ -312,6 +312,20 @@ void ref_order_prf(int nsup, const int* sprnd, const int* ia, const int* ja, con
   m.OrderPrfMtr(1, order);
 }
 
+// CSMatrix::FindSeparator (smatrix.cpp:4355) on an n-row pattern (unit blocks).
+void ref_find_separator(int n, const int* ia, const int* ja, int* ind1, int* ind2) {
+  int* sp = new int[n + 1];
+  for (int i = 0; i <= n; ++i) sp[i] = i;
+  double* a = new double[ia[n] > 0 ? ia[n] : 1]();
+  CSMatrixRS m = make(n, sp, ia, ja, a);
+  int i1 = 0, i2 = 0;
+  m.FindSeparator(i1, i2);
+  *ind1 = i1;
+  *ind2 = i2;
+  delete[] sp;
+  delete[] a;
+}
+
 // OrdMtrSymm (smatrixrs.cpp:456) / OrdMtr (:345): two calls, sizes then arrays.
 void ref_ordmtr(int symm, int nsup, const int* sprnd, const int* ia, const int* ja, const double* a,
                 const int* order, int* nzja, int* nza, int* oia, int* oja, double* oa) {

@@ -211,3 +211,24 @@ def test_dropin_abstract_par_solver(B, O, ittype, skew):
     assert rel(x, xd) < 1e-8, ittype
     assert np.linalg.norm(rhs - Ad @ x) <= 1e-9 * np.linalg.norm(rhs)
     assert p.nitperf > 0 and p.timefct > 0
+
+
+@pytest.mark.parametrize("ittype,skew", [(1, 0.0), (0, 0.3)], ids=["cg-par", "block-gmres"])
+def test_dropin_par_solver_on_a_dissected_system(B, O, ittype, skew):
+    """720 block rows: the parallel solver's nested dissection splits the
+    pattern (several tree levels below the 64-row leaves); the solve on the
+    dissected, ILU2-factored system returns the solution in the caller's
+    ordering."""
+    m, rhs = B.gen_seepage3d(10, 9, 8, 6, skew, 34)
+    order, blks, _ = m.part_binary_tree_nd(8)
+    assert len(blks) > 8  # the tree did split
+    x = np.zeros(m.n)
+    cbs = _callbacks(B, m, rhs, x)
+    p = B.default_params(ittype=ittype, sttype=1, eps=1e-11, niter=400)
+    print(f"par solver ittype={ittype} n={m.n}", flush=True)
+    code = B.lib().bcm_abstract_par_solver(None, None, C.byref(p), *cbs)
+    assert code == 0, B.lib().bcm_last_error()
+    Ad = m.dense()
+    xd = np.linalg.solve(Ad, rhs)
+    assert rel(x, xd) < 1e-8, ittype
+    assert np.linalg.norm(rhs - Ad @ x) <= 1e-9 * np.linalg.norm(rhs)
