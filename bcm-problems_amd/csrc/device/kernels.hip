@@ -613,6 +613,34 @@ __device__ __forceinline__ void gather_acc(int t0, int ncol, int jb, const int* 
     for (int q = 0; q < NR; ++q) acc[q] += w[i] * (Acc)yv[i][q];
 }
 
+// Waits until every dependency row has published the last value of its last
+// right-hand side: lanes 0..ndep-1 each poll one coherent value, and a wave
+// vote ends the wait. A waiting wave thus adds one load per dependency row
+// and poll to the memory system, not one per value it gathers (with some
+// thousand waves waiting at once, per-value polling loads the memory system
+// enough to slow the rows that are not waiting). The gather that follows
+// still checks every value it reads. The loop exit is a wave vote (uniform);
+// a wait past its bound raises ctl[2], as the gather's does.
+template <int B, int NR, class TX>
+__device__ __forceinline__ void await_deps(int jb, int ndep, const int* __restrict__ ja, TX* y, int n, int* ctl) {
+  const int lane = threadIdx.x;
+  for (int k0 = 0; k0 < ndep; k0 += 64) {
+    const int k = k0 + lane;
+    const TX* pp = y + (size_t)(NR - 1) * n + (size_t)(k < ndep ? ja[jb + k] : 0) * B + (B - 1);
+    bool ok = k >= ndep || !is_sentinel(ld_coherent(pp));
+    int spins = 0;
+    while (!__all(ok)) {
+      __builtin_amdgcn_s_sleep(1);
+      if (!ok) ok = !is_sentinel(ld_coherent(pp));
+      ++spins;
+      if (__any(spins > kSpinLimit || ((spins & 1023) == 0 && ld_coherent(&ctl[2]) != 0))) {
+        if (lane == 0) st_coherent(&ctl[2], 1);
+        return;
+      }
+    }
+  }
+}
+
 template <int B, int NR, class TA, class TX>
 __global__ __launch_bounds__(64) void k_sweep_free(int nsup, int n, int forward, const int* __restrict__ order,
                                                    const int* __restrict__ ia,
@@ -651,6 +679,7 @@ __global__ __launch_bounds__(64) void k_sweep_free(int nsup, int n, int forward,
     Acc acc[NR];
 #pragma unroll
     for (int q = 0; q < NR; ++q) acc[q] = Acc(0);
+    await_deps<B, NR>(jb, je - jb, ja, y, n, ctl);
     if (act) {
       int spins = 0;
       // the prefetched columns, then the rest in chunks of CH columns per lane:
