@@ -532,16 +532,18 @@ __global__ __launch_bounds__(256) void k_sweep_level(int nrows, const int* __res
 // The output vector is its own completion record: before the sweep it is
 // filled with a sentinel NaN pattern that no arithmetic produces (a signaling
 // NaN with a fixed payload); a row's values are final once they differ from
-// it. A wave loads its row's factor blocks into registers, then loads the
-// dependency values it needs and re-polls (bounded) those still holding the
-// sentinel, computes, and stores its row. Values cross XCDs through
+// it. A wave puts its row's factor columns in flight, stages the values of
+// the rows it depends on through LDS (each value loaded once; the ones still
+// holding the sentinel re-polled, bounded, until a wave vote finds none),
+// computes, and stores its row. Values cross XCDs through
 // device-scope relaxed atomic loads/stores (coherent across the 8 L2s, the
 // protocol rocPRIM's look-back scan uses on gfx942/gfx950); there is no
 // separate flag and no fence on the critical path. A wait that exceeds its
 // bound raises ctl[2], which stops every other wait; the host reports it.
 //
-// The row loop is wave-uniform (blockIdx/gridDim scalars); lane-divergent
-// code inside holds no cross-lane operation.
+// The row loop is wave-uniform (blockIdx/gridDim scalars) and every wait
+// ends on a wave vote; lane-divergent code inside holds no cross-lane
+// operation.
 template <class T> struct Sentinel;
 template <> struct Sentinel<double> {
   using Bits = unsigned long long;
@@ -567,78 +569,55 @@ __global__ void k_sweep_fill(int64_t n, TX* __restrict__ y, const int* skip) {
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) yb[i] = Sentinel<TX>::bits;
 }
 
-// acc[q] += sum_i w[i] * y(column t0 + i*G of the row's block stream, rhs q)
-// for the CNT columns this lane owns from t0 on; waits (bounded) for values
-// still holding the sentinel. Columns are added in ascending order.
-template <int CNT, int NR, int B, int G, class Acc, class TX>
-__device__ __forceinline__ void gather_acc(int t0, int ncol, int jb, const int* __restrict__ ja, TX* y, int n,
-                                           const Acc (&w)[CNT], Acc (&acc)[NR], int* ctl, int& spins) {
-  int src[CNT];  // element offsets in y (-1: column beyond the row)
-  TX yv[CNT][NR];
+// Stages the values of dependency blocks kb0 .. kb0+nb-1 of a row (block
+// columns ja[jb + k], every right-hand side) into ys[((k - kb0) * NR + q) * B
+// + c]. Each value is loaded once, by one of the 64 lanes (the columns a lane
+// multiplies are shared by the B lanes of its column group, so loading them
+// per lane would repeat every load B times), and the lanes re-poll the values
+// still holding the sentinel until a wave vote finds none: one coherent round
+// trip once the dependency rows are done, and a waiting wave polls only the
+// values it still lacks. The loop exit is a wave vote (uniform); a wait past
+// its bound raises ctl[2], which ends every other wait.
+template <int B, int NR, int KD, class TX>
+__device__ __forceinline__ void stage_deps(int kb0, int nb, int jb, const int* __restrict__ ja, TX* y, int n,
+                                           TX* ys, int* ctl) {
+  constexpr int SV = KD * B * NR, SJ = (SV + 63) / 64;
+  const int lane = threadIdx.x;
+  const int nv = nb * B * NR;
+  int64_t src[SJ];  // element offsets in y (-1: beyond the chunk)
+  TX val[SJ];
+  bool ok = true;
 #pragma unroll
-  for (int i = 0; i < CNT; ++i) {
-    const int tt = t0 + i * G;
-    const int bk = tt / B, c = tt - bk * B;
-    src[i] = tt < ncol ? ja[jb + bk] * B + c : -1;
+  for (int j = 0; j < SJ; ++j) {
+    const int e = lane + 64 * j;
+    src[j] = -1;
+    val[j] = TX(0);
+    if (e < nv) {
+      const int k = e / (B * NR), rem = e - k * (B * NR), q = rem / B, c = rem - q * B;
+      src[j] = (int64_t)q * n + (int64_t)ja[jb + kb0 + k] * B + c;
+      val[j] = ld_coherent(y + src[j]);
+      ok = ok && !is_sentinel(val[j]);
+    }
   }
-#pragma unroll
-  for (int i = 0; i < CNT; ++i)
-#pragma unroll
-    for (int q = 0; q < NR; ++q) yv[i][q] = src[i] >= 0 ? ld_coherent(y + src[i] + (size_t)q * n) : TX(0);
-  bool miss = false;
-#pragma unroll
-  for (int i = 0; i < CNT; ++i)
-#pragma unroll
-    for (int q = 0; q < NR; ++q) miss |= is_sentinel(yv[i][q]);
-  while (miss) {  // dependencies not all published yet
+  int spins = 0;
+  while (!__all(ok)) {
     __builtin_amdgcn_s_sleep(1);
-    miss = false;
+    ok = true;
 #pragma unroll
-    for (int i = 0; i < CNT; ++i)
-#pragma unroll
-      for (int q = 0; q < NR; ++q)
-        if (is_sentinel(yv[i][q])) {
-          yv[i][q] = ld_coherent(y + src[i] + (size_t)q * n);
-          miss |= is_sentinel(yv[i][q]);
-        }
-    if (++spins > kSpinLimit ||
-        ((spins & 1023) == 0 && __hip_atomic_load(&ctl[2], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))) {
-      __hip_atomic_store(&ctl[2], 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    for (int j = 0; j < SJ; ++j)
+      if (src[j] >= 0 && is_sentinel(val[j])) {
+        val[j] = ld_coherent(y + src[j]);
+        ok = ok && !is_sentinel(val[j]);
+      }
+    ++spins;
+    if (__any(spins > kSpinLimit || ((spins & 1023) == 0 && ld_coherent(&ctl[2]) != 0))) {
+      if (lane == 0) st_coherent(&ctl[2], 1);
       break;
     }
   }
 #pragma unroll
-  for (int i = 0; i < CNT; ++i)
-#pragma unroll
-    for (int q = 0; q < NR; ++q) acc[q] += w[i] * (Acc)yv[i][q];
-}
-
-// Waits until every dependency row has published the last value of its last
-// right-hand side: lanes 0..ndep-1 each poll one coherent value, and a wave
-// vote ends the wait. A waiting wave thus adds one load per dependency row
-// and poll to the memory system, not one per value it gathers (with some
-// thousand waves waiting at once, per-value polling loads the memory system
-// enough to slow the rows that are not waiting). The gather that follows
-// still checks every value it reads. The loop exit is a wave vote (uniform);
-// a wait past its bound raises ctl[2], as the gather's does.
-template <int B, int NR, class TX>
-__device__ __forceinline__ void await_deps(int jb, int ndep, const int* __restrict__ ja, TX* y, int n, int* ctl) {
-  const int lane = threadIdx.x;
-  for (int k0 = 0; k0 < ndep; k0 += 64) {
-    const int k = k0 + lane;
-    const TX* pp = y + (size_t)(NR - 1) * n + (size_t)(k < ndep ? ja[jb + k] : 0) * B + (B - 1);
-    bool ok = k >= ndep || !is_sentinel(ld_coherent(pp));
-    int spins = 0;
-    while (!__all(ok)) {
-      __builtin_amdgcn_s_sleep(1);
-      if (!ok) ok = !is_sentinel(ld_coherent(pp));
-      ++spins;
-      if (__any(spins > kSpinLimit || ((spins & 1023) == 0 && ld_coherent(&ctl[2]) != 0))) {
-        if (lane == 0) st_coherent(&ctl[2], 1);
-        return;
-      }
-    }
-  }
+  for (int j = 0; j < SJ; ++j)
+    if (src[j] >= 0) ys[lane + 64 * j] = val[j];
 }
 
 template <int B, int NR, class TA, class TX>
@@ -650,52 +629,57 @@ __global__ __launch_bounds__(64) void k_sweep_free(int nsup, int n, int forward,
   if (skip && *skip) return;
   using Acc = typename AccOf<TX>::type;
   constexpr int G = 64 / B;
-  // off-diagonal values prefetched per lane: three blocks (one per extra right-hand side set)
-  constexpr int PF = ((NR == 1 ? 3 : 1) * B + G - 1) / G;
+  constexpr int KD = NR == 1 ? 3 : 1;       // dependency blocks per staged chunk
+  constexpr int CK = (KD * B + G - 1) / G;  // columns of a chunk per lane
   constexpr int PD = (B + G - 1) / G;       // diagonal-block columns per lane
-  constexpr int CH = NR == 1 ? 8 : 24 / NR;  // columns per lane and chunk after the prefetched ones
   __shared__ Acc v[NR][B];
+  __shared__ TX ys[KD * B * NR];
   const int lane = threadIdx.x;
   const int r = lane % B, g = lane / B;
   const bool act = g < G;
   for (int t = blockIdx.x; t < nsup; t += gridDim.x) {
     const int row = order ? order[t] : (forward ? t : nsup - 1 - t);
-    const int jb = ia[row], je = ia[row + 1];
-    const int ncol = (je - jb) * B;
+    const int jb = ia[row], ndep = ia[row + 1] - jb;
     const TA* blk0 = off + (size_t)jb * B * B;
     const TA* d = diag + (size_t)row * B * B;
-    // factor values of this row do not depend on the wait: load them first;
-    // column tt of the row's block stream (block tt/B, column tt%B) starts at blk0 + tt*B
-    Acc pf[PF], pd[PD];
-    const TA* lp = blk0 + g * B + r;
-#pragma unroll
-    for (int i = 0; i < PF; ++i) pf[i] = (act && g + i * G < ncol) ? (Acc)ldnt(lp + i * G * B) : Acc(0);
+    // the diagonal block and the right-hand side do not depend on the wait
+    Acc pd[PD];
 #pragma unroll
     for (int i = 0; i < PD; ++i) pd[i] = (act && g + i * G < B) ? (Acc)ldnt(d + (g + i * G) * B + r) : Acc(0);
-    // the row's right-hand side is known up front too: off the wait's critical path
     Acc xv[NR];
 #pragma unroll
     for (int q = 0; q < NR; ++q) xv[q] = lane < B ? (Acc)x[(size_t)q * n + (size_t)row * B + lane] : Acc(0);
     Acc acc[NR];
 #pragma unroll
     for (int q = 0; q < NR; ++q) acc[q] = Acc(0);
-    await_deps<B, NR>(jb, je - jb, ja, y, n, ctl);
-    if (act) {
-      int spins = 0;
-      // the prefetched columns, then the rest in chunks of CH columns per lane:
-      // every load of a chunk is issued at once and only the values still
-      // holding the sentinel are polled again (one round trip per chunk when
-      // the dependencies are done)
-      gather_acc<PF, NR, B, G>(g, ncol, jb, ja, y, n, pf, acc, ctl, spins);
-      for (int t0 = g + PF * G; t0 < ncol; t0 += CH * G) {
-        Acc av[CH];
+    // Chunks of KD dependency blocks. Column tt of the row's block stream
+    // (block tt / B, column tt % B) starts at blk0 + tt * B; the lane group g
+    // multiplies the columns tt = g (mod G), in ascending order.
+    for (int kb0 = 0; kb0 < ndep; kb0 += KD) {
+      const int nb = min(KD, ndep - kb0);
+      const int c0 = kb0 * B, c1 = (kb0 + nb) * B;
+      const int tf = c0 + ((g - c0 % G) % G + G) % G;
+      // the chunk's factor values do not depend on the wait: in flight first
+      Acc av[CK];
 #pragma unroll
-        for (int i = 0; i < CH; ++i) {
-          const int tt = t0 + i * G;
-          av[i] = tt < ncol ? (Acc)ldnt(blk0 + (size_t)tt * B + r) : Acc(0);
-        }
-        gather_acc<CH, NR, B, G>(t0, ncol, jb, ja, y, n, av, acc, ctl, spins);
+      for (int i = 0; i < CK; ++i) {
+        const int tt = tf + i * G;
+        av[i] = (act && tt < c1) ? (Acc)ldnt(blk0 + (size_t)tt * B + r) : Acc(0);
       }
+      stage_deps<B, NR, KD>(kb0, nb, jb, ja, y, n, ys, ctl);
+      __syncthreads();
+      if (act) {
+#pragma unroll
+        for (int i = 0; i < CK; ++i) {
+          const int tt = tf + i * G;
+          if (tt < c1) {
+            const int k = tt / B - kb0, c = tt % B;
+#pragma unroll
+            for (int q = 0; q < NR; ++q) acc[q] += av[i] * (Acc)ys[(k * NR + q) * B + c];
+          }
+        }
+      }
+      __syncthreads();
     }
 #pragma unroll
     for (int q = 0; q < NR; ++q) {
