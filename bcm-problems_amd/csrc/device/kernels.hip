@@ -629,7 +629,7 @@ __global__ __launch_bounds__(64) void k_sweep_free(int nsup, int n, int forward,
   if (skip && *skip) return;
   using Acc = typename AccOf<TX>::type;
   constexpr int G = 64 / B;
-  constexpr int KD = NR == 1 ? 3 : 1;       // dependency blocks per staged chunk
+  constexpr int KD = 3;                     // dependency blocks per staged chunk (one round trip for most rows)
   constexpr int CK = (KD * B + G - 1) / G;  // columns of a chunk per lane
   constexpr int PD = (B + G - 1) / G;       // diagonal-block columns per lane
   __shared__ Acc v[NR][B];

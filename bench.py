@@ -350,6 +350,18 @@ def run_learned(args, B):
             ok = ok and stp.converged == 1
     plain_s = time.perf_counter() - t0
     plain_rate = nplain / plain_s
+    # one more plain solve with per-kernel HIP events (outside the timed loop;
+    # polled every iteration so that no launch is queued past convergence)
+    mo, f, b = plain_cells[0]
+    d1 = B.DeviceArray.from_host(np.ascontiguousarray(b[:, 0]))
+    x1 = B.DeviceArray(mo.n)
+    x1.zero()
+    pctx.profile(True)
+    pctx.profile_read(reset=True)
+    pctx.solve(mo, f, d1, x1, B.default_opts(kind=B.PCG, eps=EPS, ittype=3, check_every=1, niter=5000))
+    pctx.sync()
+    plain_roof, plain_kernels = roofline_from(pctx.profile_read())
+    pctx.profile(False)
 
     roof, kernels = roofline_from(prof, sweep_single_launch=tune["sweep_mode"] != 0)
     traffic_file = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles",
@@ -393,7 +405,8 @@ def run_learned(args, B):
         "valid": bool(ok),
         "plain_fp64_gpu": {"value": plain_rate, "unit": "solves/s", "sample": f"{nplain} solves "
                            f"({len(plain_cells)} evaluation cells x 6 load cases, one at a time, FP64 IC2, x0 = 0)",
-                           "mean_iterations": float(np.mean(plain_its))},
+                           "mean_iterations": float(np.mean(plain_its)),
+                           "roofline": plain_roof, "kernels": plain_kernels},
         "speedup_vs_plain": value / plain_rate,
         "iterations": {"learned_mean": float(np.mean(its_learned)), "zero_guess_mean": float(np.mean(its_zero)),
                        "training_mean": float(np.mean(train_iters))},
@@ -807,7 +820,7 @@ def run_batch(args, B):
                            "size_adjusted": "the sample's time per unknown and load case applied to the batch's "
                                             "unknowns: solves/s the plain path would reach on this batch"},
         "speedup_vs_plain_size_adjusted": value / (6 * ncell * frac / plain_s),
-        "host_setup_seconds": setup_s, "kernels": kernels, "roofline": roof, "cpu_baseline": cpu,
+        "host_cell_generation_and_ic2_seconds": setup_s, "kernels": kernels, "roofline": roof, "cpu_baseline": cpu,
     }), flush=True)
 
 
