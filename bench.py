@@ -12,10 +12,10 @@ the initial-guess model (64 POD modes per right-hand side, MLP 4 -> 256 ->
 256 -> 384 trained on the GPU with BF16 MFMA kernels).
 
 Why IC2(0.1, 0.01) and not the reference's defaults (1e-2, 1e-4,
-slvparam.h:146-147): on a 16^3 cell the defaults cost 9 s of host
-factorization per cell instead of 1 s (2.5x the fill, 806 dependency levels
-per sweep instead of 25), so the 1536 cells of the default run would take
-about 15 minutes of host time. The run therefore also solves a sample of
+slvparam.h:146-147): on a 16^3 cell the defaults cost 6-9 s of host
+factorization per cell instead of about 1 s (2.5x the fill, 800-870
+dependency levels per sweep instead of 25), so the 1536 cells of the default
+run would take about 15 minutes of host time. The run therefore also solves a sample of
 cells at the reference's thresholds (`--ref-tau-cells`, batched from x0 = 0
 against the plain FP64 path on the same cells) and reports it beside the
 headline. Evaluation cells
@@ -31,8 +31,8 @@ in HBM (uploads happen between steps, outside the timed region). `value`
 is (cell, right-hand side) solves per second. The plain FP64 GPU drop-in
 (one right-hand side at a time, FP64 factors, x0 = 0) is timed on a sample
 of the same cells for the comparison, and the reference's own serial Cg
-(compiled from its sources into oracle/_ref) on one host core gives the
-CPU baseline.
+(compiled from its sources into oracle/_ref), one independent solve per
+host core on every core, gives the CPU baseline.
 
 `--workload drop-in` runs the plain FP64 nonsymmetric drop-in instead (32^3
 cells, block 24, ILU2 + BiCGStab); `--workload precision` the FP64
