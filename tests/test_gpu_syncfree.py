@@ -1,6 +1,7 @@
-"""Sync-free triangular sweeps (one launch per sweep, rows claimed from an
-atomic ticket, bounded waits on per-row completion epochs) against the CPU
-oracle, and against the level-scheduled sweeps."""
+"""Sync-free triangular sweeps (one launch per sweep; static row ownership;
+the output pre-filled with a sentinel that marks unfinished rows; each row's
+dependency values staged through LDS behind a bounded, wave-voted wait)
+against the CPU oracle, and against the level-scheduled sweeps."""
 
 import numpy as np
 import pytest
