@@ -77,6 +77,7 @@ class BatchStats(C.Structure):
     _fields_ = [
         ("pairs", C.c_int), ("converged", C.c_int), ("breakdowns", C.c_int), ("iters_max", C.c_int),
         ("restarts", C.c_int), ("iters_sum", C.c_longlong), ("relres_max", C.c_double), ("seconds", C.c_double),
+        ("recovered", C.c_int),
     ]
 
 

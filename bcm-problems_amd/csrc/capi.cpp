@@ -818,6 +818,7 @@ int bcm_batch_solve(bcm_context c, bcm_batch b, const bcm_solve_opts* o, int nrh
       st->iters_sum = s.iters_sum;
       st->relres_max = s.relres_max;
       st->seconds = s.seconds;
+      st->recovered = s.recovered;
     }
   });
 }

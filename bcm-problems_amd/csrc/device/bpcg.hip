@@ -709,14 +709,45 @@ void batch_solve(Batch& bt, Context& ctx, const SolveOptions& opt, int nrhs, con
   if (opt.prec < kF64 || opt.prec > kBF16) throw Error(kErrArg, "batch: unknown precision");
   if (opt.kind != kPcg && opt.kind != kBlockPcg) throw Error(kErrArg, "batch: block PCG only");
   if (bt.kind != 0) throw Error(kErrArg, "batch: block PCG needs IC2 factors");
-  switch (nrhs) {
-    case 1: solve_nr<1>(bt, ctx, opt, b, x, fallback, st, iters, relres); break;
-    case 3: solve_nr<3>(bt, ctx, opt, b, x, fallback, st, iters, relres); break;
-    case 6: solve_nr<6>(bt, ctx, opt, b, x, fallback, st, iters, relres); break;
-    default: throw Error(kErrArg, "batch: nrhs must be 1, 3 or 6");
+  if (nrhs != 1 && nrhs != 3 && nrhs != 6) throw Error(kErrArg, "batch: nrhs must be 1, 3 or 6");
+  auto run = [&] {
+    switch (nrhs) {
+      case 1: solve_nr<1>(bt, ctx, opt, b, x, fallback, st, iters, relres); break;
+      case 3: solve_nr<3>(bt, ctx, opt, b, x, fallback, st, iters, relres); break;
+      default: solve_nr<6>(bt, ctx, opt, b, x, fallback, st, iters, relres); break;
+    }
+    BCM_HIP(hipStreamSynchronize(ctx.stream));
+  };
+  // Sync-free sweeps rely on every wave being resident; a dependency wait
+  // that reaches its bound (part of the GPU held by another process) leaves
+  // its sweep unfinished and raises the error flag. The solve is then
+  // repeated from x0 with level-scheduled sweeps, which never wait.
+  const bool syncfree = bt.tune.sweep_mode != 0;
+  const int64_t nn = bt.n * nrhs;
+  double* x0 = nullptr;
+  if (syncfree) {
+    x0 = ctx.vec(88, (size_t)nn * 8);
+    launch_copy(nn, x, x0, ctx.stream);
   }
-  BCM_HIP(hipStreamSynchronize(ctx.stream));
-  ctx.check_sweep_error();
+  run();
+  if (syncfree && ctx.sync.error(ctx.stream)) {
+    ctx.sync.clear(ctx.stream);
+    launch_copy(nn, x0, x, ctx.stream);
+    const BatchTune saved = bt.tune;
+    bt.tune.sweep_mode = 0;
+    bt.drop_graph();
+    st = BatchStats();
+    try {
+      run();
+    } catch (...) {
+      bt.tune = saved;
+      bt.drop_graph();
+      throw;
+    }
+    bt.tune = saved;
+    bt.drop_graph();
+    st.recovered = 1;
+  }
   st.seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
 }
 

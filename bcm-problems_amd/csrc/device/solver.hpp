@@ -219,6 +219,7 @@ struct BatchStats {
   int pairs = 0, converged = 0, breakdowns = 0, iters_max = 0, restarts = 0;
   long long iters_sum = 0;
   double relres_max = 0.0, seconds = 0.0;
+  int recovered = 0;  // the solve was repeated with level-scheduled sweeps after a wait reached its bound
 };
 
 // Builds the device union of the cells' matrices and factors (all IC2 or all

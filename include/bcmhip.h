@@ -279,6 +279,8 @@ typedef struct bcm_batch_stats {
   int pairs, converged, breakdowns, iters_max, restarts;
   long long iters_sum;
   double relres_max, seconds;
+  int recovered; /* 1: a sync-free dependency wait reached its bound; the solve was repeated from x0 with
+                    level-scheduled sweeps (which never wait) and these are the repeat's results */
 } bcm_batch_stats;
 typedef struct bcm_batch_tune {
   int sweep_mode;    /* 0 level-scheduled sweeps, 1 sync-free sweeps */

@@ -117,6 +117,25 @@ def test_batch_schedules_agree(B, ctx, batch, cells):
     batch.set_tune(sweep_mode=1, graph=0, check_every=8, waves_per_cu=8)
 
 
+def test_syncfree_wait_bound_recovery(B, ctx, batch, cells):
+    """With the wait-bound flag raised from the host, every sync-free wait of
+    the batched solve exits at once; the solve notices, repeats itself from
+    x0 with level-scheduled sweeps and reports it: the result equals a
+    level-scheduled solve bit for bit. The next solve runs undisturbed."""
+    batch.set_tune(sweep_mode=0, graph=0, check_every=4, waves_per_cu=8)
+    x_ref, st_ref, its_ref, _ = run(B, ctx, batch, cells, eps=1e-10, precision=B.F32, niter=300)
+    assert st_ref.recovered == 0
+    batch.set_tune(sweep_mode=1, graph=0, check_every=4, waves_per_cu=8)
+    ctx.raise_wait_flag()
+    print("batched sync-free solve after a raised wait flag", flush=True)
+    x, st, its, rr = run(B, ctx, batch, cells, eps=1e-10, precision=B.F32, niter=300)
+    assert st.recovered == 1 and st.converged == st.pairs and rr.max() <= 1e-10
+    assert all(np.array_equal(a, b) for a, b in zip(x, x_ref)) and np.array_equal(its, its_ref)
+    x2, st2, _, rr2 = run(B, ctx, batch, cells, eps=1e-10, precision=B.F32, niter=300)
+    assert st2.recovered == 0 and st2.converged == st2.pairs and rr2.max() <= 1e-10
+    batch.set_tune(sweep_mode=1, graph=0, check_every=8, waves_per_cu=8)
+
+
 def test_graph_capture_while_another_thread_builds(B, ctx, cells):
     """Graph-replayed solves on this thread while another thread builds and
     drops batches (device allocation and frees, pinned staging copies, the
