@@ -121,6 +121,10 @@ SIGNATURES = {
     "bcm_gen_elastic_cell": (C.c_int, [C.c_int, C.c_double, C.c_double, C.c_double, C.c_double,
                                        C.c_ulonglong, C.POINTER(_P), _DP]),
     "bcm_rcm_order": (C.c_int, [_P, _IP]),
+    "bcm_cmatrix_create": (C.c_int, [C.c_int, _IP, _IP, _IP, _DP, C.POINTER(C.c_void_p)]),
+    "bcm_cfactor_create": (C.c_int, [C.c_int, _IP, _IP, _IP, _DP, _IP, _IP, _DP, C.POINTER(C.c_void_p)]),
+    "bcm_complex_to_real": (C.c_int, [C.c_int, _IP, _DP, _DP, C.c_int]),
+    "bcm_real_to_complex": (C.c_int, [C.c_int, _IP, _DP, _DP, C.c_int]),
     "bcm_part_binary_tree_nd": (C.c_int, [_P, C.c_int, _IP, _IP, _IP, _IP]),
     "bcm_find_separator": (C.c_int, [C.c_int, _IP, _IP, _IP, _IP]),
     "bcm_matrix_reorder": (C.c_int, [_P, _IP, C.POINTER(_P)]),
@@ -304,6 +308,18 @@ class Matrix:
                                        C.byref(out)))
         return cls(out.value)
 
+    @classmethod
+    def from_complex(cls, sprnd, ia, ja, a) -> "Matrix":
+        """bcm_cmatrix_create: a complex block matrix (a: complex values,
+        blocks column-major in ja order) as its real equivalent."""
+        sprnd = np.ascontiguousarray(sprnd, dtype=np.int32)
+        ia = np.ascontiguousarray(ia, dtype=np.int32)
+        ja = np.ascontiguousarray(ja, dtype=np.int32)
+        av = np.ascontiguousarray(np.asarray(a, dtype=np.complex128)).view(np.float64)
+        out = C.c_void_p()
+        _check(lib().bcm_cmatrix_create(len(sprnd) - 1, _ip(sprnd), _ip(ia), _ip(ja), _dp(av), C.byref(out)))
+        return cls(out.value)
+
     def __del__(self):
         if getattr(self, "h", None) is not None and self.h.value and _lib is not None:
             _lib.bcm_matrix_destroy(self.h)
@@ -427,6 +443,20 @@ class Factor:
         out = C.c_void_p()
         _check(lib().bcm_factor_ic2(upper.h, C.byref(p), C.byref(out)))
         return cls(out.value, 0, p)
+
+    @classmethod
+    def from_complex(cls, sprnd, lower, upper) -> "Factor":
+        """bcm_cfactor_create: complex ILU2 factors (ia, ja, complex values) in
+        the reference's layout, L by columns and U by rows."""
+        sprnd = np.ascontiguousarray(sprnd, dtype=np.int32)
+        arrs = []
+        for ia, ja, a in (lower, upper):
+            arrs += [np.ascontiguousarray(ia, dtype=np.int32), np.ascontiguousarray(ja, dtype=np.int32),
+                     np.ascontiguousarray(np.asarray(a, dtype=np.complex128)).view(np.float64)]
+        out = C.c_void_p()
+        _check(lib().bcm_cfactor_create(len(sprnd) - 1, _ip(sprnd), _ip(arrs[0]), _ip(arrs[1]), _dp(arrs[2]),
+                                        _ip(arrs[3]), _ip(arrs[4]), _dp(arrs[5]), C.byref(out)))
+        return cls(out.value, 1, default_params())
 
     @classmethod
     def ilu2(cls, a: Matrix, params: Optional[Params] = None) -> "Factor":
@@ -584,6 +614,30 @@ def device_count() -> int:
     n = C.c_int()
     _check(lib().bcm_device_count(C.byref(n)))
     return n.value
+
+
+def complex_to_real(sprnd, z):
+    """bcm_complex_to_real: complex n x nrhs (numpy complex) -> real-equivalent 2n x nrhs."""
+    sprnd = np.ascontiguousarray(sprnd, dtype=np.int32)
+    z = np.asarray(z, dtype=np.complex128)
+    nrhs = 1 if z.ndim == 1 else z.shape[1]
+    zf = np.asfortranarray(z.reshape(len(z), nrhs)).ravel(order="F").view(np.float64)
+    x = np.zeros(2 * len(z) * nrhs)
+    _check(lib().bcm_complex_to_real(len(sprnd) - 1, _ip(sprnd), _dp(np.ascontiguousarray(zf)), _dp(x), nrhs))
+    return x if z.ndim == 1 else x.reshape(2 * len(z), nrhs, order="F")
+
+
+def real_to_complex(sprnd, x):
+    """bcm_real_to_complex: real-equivalent 2n x nrhs -> complex n x nrhs."""
+    sprnd = np.ascontiguousarray(sprnd, dtype=np.int32)
+    x = np.asarray(x, dtype=np.float64)
+    nrhs = 1 if x.ndim == 1 else x.shape[1]
+    n = x.shape[0] // 2
+    xf = np.ascontiguousarray(x.reshape(2 * n, nrhs).ravel(order="F"))
+    z = np.zeros(2 * n * nrhs)
+    _check(lib().bcm_real_to_complex(len(sprnd) - 1, _ip(sprnd), _dp(xf), _dp(z), nrhs))
+    zc = z.view(np.complex128)
+    return zc if x.ndim == 1 else zc.reshape(n, nrhs, order="F")
 
 
 def find_separator(ia, ja):

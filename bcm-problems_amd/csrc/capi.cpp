@@ -219,6 +219,58 @@ int bcm_matrix_create(int nsup, const int* sprnd, const int* ia, const int* ja, 
   });
 }
 
+int bcm_cmatrix_create(int nsup, const int* sprnd, const int* ia, const int* ja, const double* a,
+                       bcm_matrix* out) {
+  return guarded([&] {
+    need(out && sprnd && ia && ja && a && nsup >= 0, "cmatrix_create: bad argument");
+    auto* h = new bcm_matrix_s;
+    try {
+      h->m.h = bcm::real_equivalent(nsup, sprnd, ia, ja, a, false);
+      h->m.symmetric = false;
+      h->m.h.validate(false);
+    } catch (...) {
+      delete h;
+      throw;
+    }
+    *out = h;
+  });
+}
+
+int bcm_cfactor_create(int nsup, const int* sprnd, const int* lia, const int* lja, const double* la,
+                       const int* uia, const int* uja, const double* ua, bcm_factor* out) {
+  return guarded([&] {
+    need(out && sprnd && lia && lja && la && uia && uja && ua && nsup >= 0, "cfactor_create: bad argument");
+    auto* h = new bcm_factor_s;
+    try {
+      h->f.kind = 1;
+      h->f.l = bcm::real_equivalent(nsup, sprnd, lia, lja, la, true);
+      h->f.u = bcm::real_equivalent(nsup, sprnd, uia, uja, ua, false);
+      for (int i = 0; i < nsup; ++i) {
+        need(h->f.l.ia[i] < h->f.l.ia[i + 1] && h->f.l.ja[h->f.l.ia[i]] == i, "cfactor_create: L row without its diagonal block first");
+        need(h->f.u.ia[i] < h->f.u.ia[i + 1] && h->f.u.ja[h->f.u.ia[i]] == i, "cfactor_create: U row without its diagonal block first");
+      }
+    } catch (...) {
+      delete h;
+      throw;
+    }
+    *out = h;
+  });
+}
+
+int bcm_complex_to_real(int nsup, const int* sprnd, const double* z, double* x, int nrhs) {
+  return guarded([&] {
+    need(sprnd && z && x && nsup >= 0 && nrhs >= 1, "complex_to_real: bad argument");
+    bcm::complex_to_real(nsup, sprnd, z, x, nrhs);
+  });
+}
+
+int bcm_real_to_complex(int nsup, const int* sprnd, const double* x, double* z, int nrhs) {
+  return guarded([&] {
+    need(sprnd && z && x && nsup >= 0 && nrhs >= 1, "real_to_complex: bad argument");
+    bcm::real_to_complex(nsup, sprnd, x, z, nrhs);
+  });
+}
+
 int bcm_matrix_destroy(bcm_matrix m) {
   return guarded([&] { delete m; });
 }

@@ -87,6 +87,19 @@ void genrcm(int n, const std::vector<int>& xadj, const std::vector<int>& adj, st
 // Profile ordering as in OrderPrfMtr(1): order[old] = new.
 std::vector<int> rcm_order(const BlockCsr& a);
 
+// ---- complex matrices as real equivalents (complex.cpp) ----
+
+// Real equivalent of a complex block matrix (interleaved re/im values, blocks
+// column-major in ja order): block rows of 2 b_i, each complex block C as
+// [[Re C, -Im C], [Im C, Re C]], or, for blocks stored transposed (L of an
+// ILU2 factor, by columns), [[Re C, Im C], [-Im C, Re C]].
+BlockCsr real_equivalent(int nsup, const int* sprnd, const int* ia, const int* ja, const double* a,
+                         bool transposed_blocks);
+// Complex vectors (interleaved, n x nrhs) to and from the real-equivalent
+// layout (per block row [Re z; Im z], 2n x nrhs).
+void complex_to_real(int nsup, const int* sprnd, const double* z, double* x, int nrhs);
+void real_to_complex(int nsup, const int* sprnd, const double* x, double* z, int nrhs);
+
 // ---- nested dissection and the parallel solver's tree (nd.cpp) ----
 
 // Nested-dissection ordering of the symmetrized block pattern with the layout

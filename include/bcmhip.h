@@ -101,6 +101,27 @@ void bcm_solve_opts_default(bcm_solve_opts* o);
 int bcm_matrix_create(int nsup, const int* sprnd, const int* ia, const int* ja, const double* a, int symmetric,
                       bcm_matrix* out);
 int bcm_matrix_destroy(bcm_matrix m);
+/* Complex block matrix (CSMatrixCS, smatrix.h:1499; its products MvmA,
+ * mvmc.cpp:160) held as its real equivalent: each complex block C of
+ * b_i x b_j becomes the real block [[Re C, -Im C], [Im C, Re C]] of
+ * 2b_i x 2b_j, so every entry point taking a bcm_matrix (products, solvers)
+ * applies the complex operator to vectors in the real-equivalent layout
+ * (bcm_complex_to_real). a: nza complex values interleaved (re, im) -- the
+ * reference's dcmplx array -- blocks column-major in ja order. Products agree
+ * with the reference's complex arithmetic to rounding. */
+int bcm_cmatrix_create(int nsup, const int* sprnd, const int* ia, const int* ja, const double* a, bcm_matrix* out);
+/* Complex ILU2 factors as CSMatrixCS::Ilu2 returns them (ilu2cs.cpp; L by
+ * columns with transposed blocks, U by rows, inverted diagonal blocks first in
+ * every row; values interleaved re/im), held as their real equivalents for the
+ * sweeps of SolveL / SolveU (mvmc.cpp:1723, 2506) and every preconditioned
+ * solver. The complex factorization itself is the caller's (this library
+ * factors real matrices only). */
+int bcm_cfactor_create(int nsup, const int* sprnd, const int* lia, const int* lja, const double* la,
+                       const int* uia, const int* uja, const double* ua, bcm_factor* out);
+/* Complex vectors (interleaved re/im, n x nrhs, CSVectorC's layout) to and
+ * from the real-equivalent layout (per block row [Re z; Im z], 2n x nrhs). */
+int bcm_complex_to_real(int nsup, const int* sprnd, const double* z, double* x, int nrhs);
+int bcm_real_to_complex(int nsup, const int* sprnd, const double* x, double* z, int nrhs);
 int bcm_matrix_info(bcm_matrix m, int* nsup, int* n, int* nzja, long long* nza, int* symmetric);
 int bcm_matrix_export(bcm_matrix m, int* sprnd, int* ia, int* ja, double* a);
 

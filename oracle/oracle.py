@@ -98,6 +98,10 @@ def ref() -> C.CDLL:
         l.ref_genrcm.argtypes = [C.c_int, _IP, _IP, _IP]
         l.ref_order_prf.argtypes = mat + [_IP]
         l.ref_find_separator.argtypes = [C.c_int, _IP, _IP, _IP, _IP]
+        l.ref_cmvma.argtypes = mat + [_DP, _DP, C.c_int]
+        l.ref_csolve.argtypes = [C.c_int] + mat + [_DP, _DP, C.c_int]
+        l.ref_cilu2.argtypes = mat + [C.c_double] * 6 + [C.c_char_p, C.c_int]
+        l.ref_cfct_take.argtypes = [C.c_int, _IP, _IP, _IP, _IP, _DP]
         l.ref_ordmtr.argtypes = [C.c_int] + mat + [_IP, _IP, _IP, _IP, _IP, _DP]
         l.ref_combinelu.argtypes = [C.c_int] + mat + [_IP, _IP, _IP, _IP, _DP]
         l.ref_ich2.argtypes = mat + [C.c_double] * 4 + [C.c_char_p, C.c_int]
@@ -301,6 +305,60 @@ def ref_order_prf(m: Block):
     order = np.zeros(m.nsup, np.int32)
     ref().ref_order_prf(*m.args(), order.ctypes.data_as(_IP))
     return order
+
+
+def _cargs(sprnd, ia, ja, a):
+    sprnd = np.ascontiguousarray(sprnd, np.int32)
+    ia = np.ascontiguousarray(ia, np.int32)
+    ja = np.ascontiguousarray(ja, np.int32)
+    av = np.ascontiguousarray(np.asarray(a, np.complex128)).view(np.float64)
+    return (len(sprnd) - 1, sprnd.ctypes.data_as(_IP), ia.ctypes.data_as(_IP), ja.ctypes.data_as(_IP),
+            av.ctypes.data_as(_DP)), (sprnd, ia, ja, av)
+
+
+def _cvec(x, nrhs):
+    return np.ascontiguousarray(np.asfortranarray(np.asarray(x, np.complex128).reshape(-1, nrhs)).ravel(order="F"))
+
+
+def ref_cmvma(sprnd, ia, ja, a, x, nrhs=1):
+    """The reference's complex CSMatrixCS::MvmA (mvmc.cpp:160): y = A x."""
+    args, keep = _cargs(sprnd, ia, ja, a)
+    xv = _cvec(x, nrhs)
+    y = np.zeros_like(xv)
+    ref().ref_cmvma(*args, xv.view(np.float64).ctypes.data_as(_DP), y.view(np.float64).ctypes.data_as(_DP), nrhs)
+    return y.reshape(np.asarray(x).shape, order="F")
+
+
+def ref_csolve(which, sprnd, ia, ja, a, x, nrhs=1):
+    """The reference's complex SolveL (which 0, L by columns, mvmc.cpp:1723)
+    or SolveU (which 1, U by rows, mvmc.cpp:2506)."""
+    args, keep = _cargs(sprnd, ia, ja, a)
+    xv = _cvec(x, nrhs)
+    y = np.zeros_like(xv)
+    ref().ref_csolve(which, *args, xv.view(np.float64).ctypes.data_as(_DP), y.view(np.float64).ctypes.data_as(_DP),
+                     nrhs)
+    return y.reshape(np.asarray(x).shape, order="F")
+
+
+def ref_cilu2(sprnd, ia, ja, a, tau1=1e-2, tau2=1e-4, theta=0.1, sclmin=1e-8, pivmin=1e-8, memory=10.0):
+    """The reference's CombineLU + complex CSMatrixCS::Ilu2 (ilu2cs.cpp):
+    ((lia, lja, la), (uia, uja, ua)) with complex values, L by columns."""
+    args, keep = _cargs(sprnd, ia, ja, a)
+    msg = C.create_string_buffer(256)
+    if ref().ref_cilu2(*args, tau1, tau2, theta, sclmin, pivmin, memory, msg, 256):
+        raise RefFactorError(msg.value.decode(errors="replace"))
+    nsup = len(keep[0]) - 1
+    out = []
+    for which in (0, 1):
+        nzja, nza = C.c_int(), C.c_int()
+        ref().ref_cfct_take(which, C.byref(nzja), C.byref(nza), None, None, None)
+        fia = np.zeros(nsup + 1, np.int32)
+        fja = np.zeros(nzja.value, np.int32)
+        fa = np.zeros(nza.value, np.complex128)
+        ref().ref_cfct_take(which, C.byref(nzja), C.byref(nza), fia.ctypes.data_as(_IP), fja.ctypes.data_as(_IP),
+                            fa.view(np.float64).ctypes.data_as(_DP))
+        out.append((fia, fja, fa))
+    return tuple(out)
 
 
 def ref_find_separator(ia, ja):

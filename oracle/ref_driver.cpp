@@ -80,6 +80,69 @@ void take(const CSMatrixRS& m, int* nzja, int* nza, int* ia, int* ja, double* a)
   }
 }
 
+// The same for the complex CSMatrixCS (smatrix.h:1499): a holds nza complex
+// values interleaved (re, im), blocks in ja order.
+CSMatrixCS make_cs(int nsup, const int* sprnd, const int* ia, const int* ja, const double* a) {
+  const int nzja = ia[nsup];
+  int nza = 0, blamx = 0;
+  for (int i = 0; i < nsup; ++i) {
+    const int bi = sprnd[i + 1] - sprnd[i];
+    if (bi > blamx) blamx = bi;
+    for (int j = ia[i]; j < ia[i + 1]; ++j) nza += bi * (sprnd[ja[j] + 1] - sprnd[ja[j]]);
+  }
+  CSMatrixCS m(nsup, nzja, nza);
+  m.SetM(sprnd[nsup]);
+  m.SetN(sprnd[nsup]);
+  m.SetNlist(nsup);
+  m.SetNsupr(nsup);
+  m.SetBlamx(blamx);
+  int *psr, *psc, *pbsa, *plist, *pia, *pja;
+  m.GetPartition(psr, psc, pbsa);
+  m.GetSparsity(plist, pia, pja);
+  for (int i = 0; i <= nsup; ++i) psr[i] = psc[i] = sprnd[i];
+  for (int i = 0; i < nsup; ++i) plist[i] = i;
+  for (int i = 0; i <= nsup; ++i) pia[i] = ia[i];
+  int off = 0;
+  for (int i = 0; i < nsup; ++i) {
+    const int bi = sprnd[i + 1] - sprnd[i];
+    for (int j = ia[i]; j < ia[i + 1]; ++j) {
+      pja[j] = ja[j];
+      pbsa[j] = off;
+      off += bi * (sprnd[ja[j] + 1] - sprnd[ja[j]]);
+    }
+  }
+  std::memcpy(m.GetA(), a, sizeof(double) * 2 * (size_t)nza);
+  m.SetNsupc(nsup);
+  m.SetNzja(nzja);
+  m.SetNza(nza);
+  m.SetNzatot(nza);
+  return m;
+}
+
+void take_cs(const CSMatrixCS& m, int* nzja, int* nza, int* ia, int* ja, double* a) {
+  int *psr, *psc, *pbsa, *plist, *pia, *pja;
+  m.GetPartition(psr, psc, pbsa);
+  m.GetSparsity(plist, pia, pja);
+  const int nsup = m.GetNsupr();
+  if (nzja) *nzja = pia[nsup];
+  if (nza) *nza = m.GetNza();
+  if (ia) std::memcpy(ia, pia, sizeof(int) * (nsup + 1));
+  if (ja) std::memcpy(ja, pja, sizeof(int) * pia[nsup]);
+  if (a) {
+    int off = 0;
+    for (int i = 0; i < nsup; ++i) {
+      const int bi = psr[i + 1] - psr[i];
+      for (int j = pia[i]; j < pia[i + 1]; ++j) {
+        const int sz = bi * (psr[pja[j] + 1] - psr[pja[j]]);
+        std::memcpy(a + 2 * (size_t)off, m.GetA() + pbsa[j], sizeof(double) * 2 * sz);
+        off += sz;
+      }
+    }
+  }
+}
+
+CSMatrixCS g_cfct[2];  // last complex factor: [0] L by columns, [1] U by rows
+
 CSVector vec(int n, int nrhs, const double* x) {
   CSVector v(n, 1, nrhs);
   if (x) std::memcpy(v.GetVect(), x, sizeof(double) * n * nrhs);
@@ -194,6 +257,53 @@ void ref_read_vect_bin(const char* path, int* n, int* nrhs, double* v) {
 // The last factor's part which (0 L, 1 U): sizes, then arrays when non-null.
 void ref_fct_take(int which, int* nzja, int* nza, int* oia, int* oja, double* oa) {
   take(g_fct[which], nzja, nza, oia, oja, oa);
+}
+
+// Complex solve phase (mvmc.cpp:160, 1723, 2506): y = A x, y = L^-1 x,
+// y = U^-1 x for CSMatrixCS, n x nrhs complex vectors interleaved (re, im).
+void ref_cmvma(int nsup, const int* sprnd, const int* ia, const int* ja, const double* a, const double* x, double* y,
+               int nrhs) {
+  CSMatrixCS m = make_cs(nsup, sprnd, ia, ja, a);
+  const int n = sprnd[nsup];
+  CSVectorC vx(n, 1, nrhs), vy(n, 1, nrhs);
+  std::memcpy(vx.GetVect(), x, sizeof(double) * 2 * (size_t)n * nrhs);
+  m.MvmA(vx, vy);
+  std::memcpy(y, vy.GetVect(), sizeof(double) * 2 * (size_t)n * nrhs);
+}
+
+void ref_csolve(int which, int nsup, const int* sprnd, const int* ia, const int* ja, const double* a,
+                const double* x, double* y, int nrhs) {
+  CSMatrixCS m = make_cs(nsup, sprnd, ia, ja, a);
+  const int n = sprnd[nsup];
+  CSVectorC vx(n, 1, nrhs), vy(n, 1, nrhs);
+  std::memcpy(vx.GetVect(), x, sizeof(double) * 2 * (size_t)n * nrhs);
+  if (which == 0) m.SolveL(vx, vy);
+  else m.SolveU(vx, vy);
+  std::memcpy(y, vy.GetVect(), sizeof(double) * 2 * (size_t)n * nrhs);
+}
+
+// CombineLU + CSMatrixCS::Ilu2 (ilu2cs.cpp) of a complex matrix, the factors
+// kept for ref_cfct_take. Returns 0, or 1 when the reference threw.
+int ref_cilu2(int nsup, const int* sprnd, const int* ia, const int* ja, const double* a, double tau1, double tau2,
+              double theta, double sclmin, double pivmin, double memory, char* msg, int msglen) {
+  CSMatrixCS am = make_cs(nsup, sprnd, ia, ja, a);
+  CSMatrixCS al, au, dummy;
+  am.CombineLU(al, au);
+  CSlvParam p;
+  set_fct_params(p, tau1, tau2, theta, sclmin, pivmin, memory);
+  std::ofstream fout("/dev/null");
+  try {
+    dummy.Ilu2(fout, p, al, au, g_cfct[0], g_cfct[1]);
+  } catch (const char* e) {
+    std::strncpy(msg, e, msglen - 1);
+    msg[msglen - 1] = 0;
+    return 1;
+  }
+  return 0;
+}
+
+void ref_cfct_take(int which, int* nzja, int* nza, int* oia, int* oja, double* oa) {
+  take_cs(g_cfct[which], nzja, nza, oia, oja, oa);
 }
 
 // dsyev_ / dgesvd_ as the factorizations call them (lapackd.cpp:865-905), for
