@@ -242,7 +242,7 @@ class DeviceArray:
     """A HIP allocation of `shape` float64 (or `dtype`) values, column-major."""
 
     def __init__(self, shape, dtype=np.float64):
-        self.shape = (shape,) if isinstance(shape, int) else tuple(shape)
+        self.shape = (int(shape),) if isinstance(shape, (int, np.integer)) else tuple(int(s) for s in shape)
         self.dtype = np.dtype(dtype)
         self.nbytes = int(np.prod(self.shape)) * self.dtype.itemsize
         p = C.c_void_p()
