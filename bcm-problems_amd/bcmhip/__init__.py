@@ -46,6 +46,7 @@ class Params(C.Structure):
         ("eps", C.c_double),
         ("nitperf", C.c_int), ("resfin", C.c_double), ("timefct", C.c_double),
         ("timeitr", C.c_double), ("density", C.c_double), ("density2", C.c_double),
+        ("fcttyp", C.c_int),
     ]
 
 
@@ -123,6 +124,7 @@ SIGNATURES = {
     "bcm_rcm_order": (C.c_int, [_P, _IP]),
     "bcm_cmatrix_create": (C.c_int, [C.c_int, _IP, _IP, _IP, _DP, C.POINTER(C.c_void_p)]),
     "bcm_cfactor_create": (C.c_int, [C.c_int, _IP, _IP, _IP, _DP, _IP, _IP, _DP, C.POINTER(C.c_void_p)]),
+    "bcm_cfactor_ilu2": (C.c_int, [C.c_int, _IP, _IP, _IP, _DP, C.POINTER(Params), C.POINTER(C.c_void_p)]),
     "bcm_complex_to_real": (C.c_int, [C.c_int, _IP, _DP, _DP, C.c_int]),
     "bcm_real_to_complex": (C.c_int, [C.c_int, _IP, _DP, _DP, C.c_int]),
     "bcm_part_binary_tree_nd": (C.c_int, [_P, C.c_int, _IP, _IP, _IP, _IP]),
@@ -135,6 +137,7 @@ SIGNATURES = {
     "bcm_vector_write_bin": (C.c_int, [C.c_char_p, C.c_int, C.c_int, _DP]),
     "bcm_vector_read_bin": (C.c_int, [C.c_char_p, _IP, _IP, _DP]),
     "bcm_dgesvd": (C.c_int, [C.c_int, _DP, _DP, _DP, _DP]),
+    "bcm_zgesvd": (C.c_int, [C.c_int, _DP, _DP, _DP, _DP]),
     "bcm_vector_reorder": (C.c_int, [_P, _IP, C.c_char, _DP, C.c_int]),
     "bcm_factor_ic2": (C.c_int, [_P, C.POINTER(Params), C.POINTER(_P)]),
     "bcm_factor_ilu2": (C.c_int, [_P, C.POINTER(Params), C.POINTER(_P)]),
@@ -197,6 +200,8 @@ SIGNATURES = {
                                       _CB_BLOCK, _CB_BLOCK, _CB_BLOCK, _CB_BLOCK]),
     "bcm_abstract_par_solver": (C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(Params), _CB_NSUP, _CB_SPRNDS,
                                           _CB_IAJA, _CB_BLOCK, _CB_BLOCK, _CB_BLOCK, _CB_BLOCK]),
+    "bcm_abstract_solver_c": (C.c_int, [C.c_void_p, C.POINTER(Params), _CB_NSUP, _CB_SPRNDS, _CB_IAJA,
+                                        _CB_BLOCK, _CB_BLOCK, _CB_BLOCK, _CB_BLOCK]),
     "bcm_alloc_ints": (_IP, [C.c_int]),
 }
 
@@ -459,6 +464,21 @@ class Factor:
         return cls(out.value, 1, default_params())
 
     @classmethod
+    def ilu2_complex(cls, sprnd, ia, ja, a, params: Optional[Params] = None) -> "Factor":
+        """bcm_cfactor_ilu2: CombineLU + complex ILU2 of a complex block matrix
+        (a: complex values, blocks column-major in ja order), held as its real
+        equivalent."""
+        p = params or default_params()
+        sprnd = np.ascontiguousarray(sprnd, dtype=np.int32)
+        ia = np.ascontiguousarray(ia, dtype=np.int32)
+        ja = np.ascontiguousarray(ja, dtype=np.int32)
+        av = np.ascontiguousarray(np.asarray(a, dtype=np.complex128)).view(np.float64)
+        out = C.c_void_p()
+        _check(lib().bcm_cfactor_ilu2(len(sprnd) - 1, _ip(sprnd), _ip(ia), _ip(ja), _dp(av), C.byref(p),
+                                      C.byref(out)))
+        return cls(out.value, 1, p)
+
+    @classmethod
     def ilu2(cls, a: Matrix, params: Optional[Params] = None) -> "Factor":
         p = params or default_params()
         out = C.c_void_p()
@@ -508,6 +528,17 @@ def dsyev(a: np.ndarray):
     w = np.zeros(n)
     _check(lib().bcm_dsyev(n, _dp(z), _dp(w)))
     return w, z
+
+
+def zgesvd(a: np.ndarray):
+    """zgesvd_("A", "A") (LAPACK 3.0 ZGESVD) of a square complex matrix, n >= 2: (s, U, VT)."""
+    n = a.shape[0]
+    x = np.array(a, dtype=np.complex128, order="F", copy=True)
+    s = np.zeros(n)
+    u = np.zeros((n, n), np.complex128, order="F")
+    vt = np.zeros((n, n), np.complex128, order="F")
+    _check(lib().bcm_zgesvd(n, x.ctypes.data_as(_DP), _dp(s), u.ctypes.data_as(_DP), vt.ctypes.data_as(_DP)))
+    return s, u, vt
 
 
 def dgesvd(a: np.ndarray):

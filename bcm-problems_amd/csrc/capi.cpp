@@ -86,6 +86,7 @@ bcm::SlvParam to_slv(const bcm_params* p) {
   s.niter = p->niter;
   s.ichk = p->ichk;
   s.eps = p->eps;
+  s.fcttyp = p->fcttyp;
   return s;
 }
 
@@ -183,6 +184,7 @@ void bcm_params_default(bcm_params* p) {
   p->niter = s.niter;
   p->ichk = s.ichk;
   p->eps = s.eps;
+  p->fcttyp = s.fcttyp;
 }
 
 void bcm_solve_opts_default(bcm_solve_opts* o) {
@@ -252,6 +254,52 @@ int bcm_cfactor_create(int nsup, const int* sprnd, const int* lia, const int* lj
     } catch (...) {
       delete h;
       throw;
+    }
+    *out = h;
+  });
+}
+
+int bcm_cfactor_ilu2(int nsup, const int* sprnd, const int* ia, const int* ja, const double* a, bcm_params* p,
+                     bcm_factor* out) {
+  return guarded([&] {
+    need(out && sprnd && ia && ja && a && nsup > 0, "cfactor_ilu2: bad argument");
+    bcm::BlockCsr c;
+    c.nsup = nsup;
+    c.sprnd.assign(sprnd, sprnd + nsup + 1);
+    c.ia.assign(ia, ia + nsup + 1);
+    need(c.ia[0] == 0, "cfactor_ilu2: ia[0] must be 0");
+    c.ja.assign(ja, ja + c.ia[nsup]);
+    c.bsa.resize(c.ia[nsup]);
+    int64_t nz = 0;
+    for (int i = 0; i < nsup; ++i) {
+      need(c.bla(i) > 0, "cfactor_ilu2: empty block row");
+      for (int j = c.ia[i]; j < c.ia[i + 1]; ++j) {
+        need(c.ja[j] >= 0 && c.ja[j] < nsup, "cfactor_ilu2: block column out of range");
+        need(j == c.ia[i] || c.ja[j] > c.ja[j - 1], "cfactor_ilu2: block columns must ascend within a row");
+        c.bsa[j] = nz;
+        nz += (int64_t)c.bla(i) * c.bla(c.ja[j]);
+      }
+    }
+    c.a.assign(a, a + 2 * nz);
+    c.set_blamx();
+    bcm::BlockCsr al, au, l, u;
+    bcm::combine_lu_complex(c, al, au);
+    bcm::FactorStats stats;
+    bcm::ilu2_complex(al, au, to_slv(p), l, u, &stats);
+    auto* h = new bcm_factor_s;
+    try {
+      h->f.kind = 1;
+      h->f.l = bcm::real_equivalent(nsup, sprnd, l.ia.data(), l.ja.data(), l.a.data(), true);
+      h->f.u = bcm::real_equivalent(nsup, sprnd, u.ia.data(), u.ja.data(), u.a.data(), false);
+      h->f.stats = stats;
+    } catch (...) {
+      delete h;
+      throw;
+    }
+    if (p) {
+      p->timefct = stats.seconds;
+      p->density = stats.densu;
+      p->density2 = stats.densr;
     }
     *out = h;
   });
@@ -457,6 +505,18 @@ int bcm_dgesvd(int n, double* a, double* s, double* u, double* vt) {
     std::vector<double> work((size_t)10 * n + 1);
     const int info = bcm::lapack30::dgesvd_aa(n, a, n, s, u, n, vt, n, work.data(), 10 * n);
     if (info != 0) throw bcm::Error(bcm::kErrInternal, "Error in the Lapack routine DGESVD");
+  });
+}
+
+int bcm_zgesvd(int n, double* a, double* s, double* u, double* vt) {
+  return guarded([&] {
+    need(n >= 2 && a && s && u && vt, "zgesvd: bad argument (n >= 2)");
+    std::vector<bcm::Zc> work((size_t)10 * n + 1);
+    std::vector<double> rwork((size_t)10 * n + 1);
+    const int info = bcm::lapack30::zgesvd_aa(n, reinterpret_cast<bcm::Zc*>(a), n, s, reinterpret_cast<bcm::Zc*>(u),
+                                              n, reinterpret_cast<bcm::Zc*>(vt), n, work.data(), 10 * n,
+                                              rwork.data());
+    if (info != 0) throw bcm::Error(bcm::kErrInternal, "Error in the Lapack routine ZGESVD");
   });
 }
 

@@ -1,7 +1,8 @@
 // Drop-in entry points: the reference's AbstractSymmSolver / AbstractSolver
-// (solvers.cpp:5383-5586, :5591-5794) and the dispatchers they call
-// (CSVector::SolveSymm solvers.cpp:4452-4610, CSVector::Solver :1568-1860),
-// with the Krylov phase on the GPU.
+// (solvers.cpp:5383-5586, :5591-5794), the complex AbstractSolver
+// (:7103-7398) and the dispatchers they call (CSVector::SolveSymm
+// solvers.cpp:4452-4610, CSVector::Solver :1568-1860, CSVectorC::Solver
+// :2588-2830), with the Krylov phase on the GPU.
 //
 // Steps kept from the reference: callback assembly (rows sorted by column,
 // blocks copied in sorted order), RCM profile ordering (ordtype > 0; the
@@ -36,18 +37,35 @@ struct Callbacks {
   bcm_cb_storesol fstoresol;
 };
 
+struct CCallbacks {
+  void* obj;
+  bcm_cb_nsup fnsup;
+  bcm_cb_sprnds fsprnds;
+  bcm_cb_iaja fiaja;
+  bcm_cb_blockrow_c fblockrow;
+  bcm_cb_rhs_c frhs;
+  bcm_cb_iniguess_c finiguess;
+  bcm_cb_storesol_c fstoresol;
+};
+
 // Assembles the block matrix exactly as AbstractSolver does: the callback
 // returns each row's blocks in its own ja order; rows are re-sorted by column
-// and blocks copied in the sorted order (solvers.cpp:5683-5736).
-bcm::BlockCsr assemble(const Callbacks& cb) {
+// and blocks copied in the sorted order (solvers.cpp:5683-5736; the complex
+// solver the same way, :7196-7268). `width` doubles per entry: 1 real, 2
+// complex (interleaved re/im; bsa counts entries).
+template <class RowFn>
+bcm::BlockCsr assemble_rows(void* obj, bcm_cb_nsup fnsup, bcm_cb_sprnds fsprnds, bcm_cb_iaja fiaja, RowFn&& row,
+                            int width) {
   int nsup = 0;
-  cb.fnsup(cb.obj, &nsup);
+  fnsup(obj, &nsup);
   if (nsup <= 0) throw bcm::Error(bcm::kErrArg, "callback reported no block rows");
   std::vector<int> sprnd(nsup + 1);
-  cb.fsprnds(cb.obj, sprnd.data());
+  fsprnds(obj, sprnd.data());
+  for (int i = 0; i < nsup; ++i)
+    if (sprnd[i + 1] <= sprnd[i]) throw bcm::Error(bcm::kErrArg, "callback block sizes must be positive");
   int* ia = nullptr;
   int* ja = nullptr;
-  cb.fiaja(cb.obj, &ia, &ja);
+  fiaja(obj, &ia, &ja);
   if (!ia || !ja) throw bcm::Error(bcm::kErrArg, "callback returned no sparsity");
   bcm::BlockCsr m;
   try {
@@ -66,8 +84,8 @@ bcm::BlockCsr assemble(const Callbacks& cb) {
       rowmax = std::max(rowmax, row);
       total += row;
     }
-    m.a.resize(total);
-    std::vector<double> arow(std::max<size_t>(rowmax, 1));
+    m.a.resize(total * width);
+    std::vector<double> arow(std::max<size_t>(rowmax * width, 1));
     std::vector<size_t> off;
     std::vector<int> idx;
     size_t nz = 0;
@@ -80,7 +98,7 @@ bcm::BlockCsr assemble(const Callbacks& cb) {
         off[k] = pos;
         pos += (size_t)bi * m.bla(ja[ia[i] + k]);
       }
-      cb.fblockrow(cb.obj, i, arow.data());
+      row(i, arow.data());
       idx.resize(nzj);
       std::iota(idx.begin(), idx.end(), 0);
       std::sort(idx.begin(), idx.end(), [&](int x, int y) { return ja[ia[i] + x] < ja[ia[i] + y]; });
@@ -90,7 +108,7 @@ bcm::BlockCsr assemble(const Callbacks& cb) {
         const size_t sz = (size_t)bi * m.bla(col);
         m.ja[ia[i] + k] = col;
         m.bsa[ia[i] + k] = (int64_t)nz;
-        std::memcpy(&m.a[nz], &arow[off[src]], sz * sizeof(double));
+        std::memcpy(&m.a[nz * width], &arow[off[src] * width], sz * width * sizeof(double));
         nz += sz;
       }
     }
@@ -103,6 +121,11 @@ bcm::BlockCsr assemble(const Callbacks& cb) {
   delete[] ja;
   m.set_blamx();
   return m;
+}
+
+bcm::BlockCsr assemble(const Callbacks& cb) {
+  return assemble_rows(cb.obj, cb.fnsup, cb.fsprnds, cb.fiaja,
+                       [&](int i, double* r) { cb.fblockrow(cb.obj, i, r); }, 1);
 }
 
 void gather_blocks(const Callbacks& cb, const bcm::BlockCsr& m, bcm_cb_rhs f, std::vector<double>& v) {
@@ -250,6 +273,137 @@ int run(bool symmetric, bool par, bcm_params* p, const Callbacks& cb) {
   }
 }
 
+// The complex AbstractSolver (solvers.cpp:7103-7398) and CSVectorC::Solver
+// (solvers.cpp:2588-2830): the complex matrix, factors and vectors are held
+// as their real equivalents (complex.cpp), and the complex block GMRES of one
+// right-hand side b is the real block GMRES of the two columns [b, i b] from
+// [x0, i x0]: the real span of the complex Krylov space of b is the block
+// Krylov space of those columns, so both minimize over the same set and the
+// first column's iterate is the complex one after every step.
+int run_complex(bcm_params* p, const CCallbacks& cb) {
+  try {
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0)
+      throw bcm::Error(bcm::kErrNoDevice, "no HIP device: the solve phase runs on the GPU only");
+    bcm_params defaults;
+    bcm_params_default(&defaults);
+    bcm_params& prm = p ? *p : defaults;
+    // ordtype = nparts of PartOrdMtr (solvers.cpp:2633, 2651-2657): one part is the
+    // RCM profile ordering; more parts (METIS) and ordtype < 0 (OrderBrdMtr) are not restated
+    if (prm.ordtype < 0 || prm.ordtype > 1)
+      throw bcm::Error(bcm::kErrArg, "complex AbstractSolver: ordtype must be 0 (none) or 1 (RCM profile)");
+    bcm::SlvParam slv;
+    slv.ordtype = prm.ordtype;
+    slv.memory = prm.memory;
+    slv.tau1 = prm.tau1;
+    slv.tau2 = prm.tau2;
+    slv.theta = prm.theta;
+    slv.sclmin = prm.sclmin;
+    slv.pivmin = prm.pivmin;
+    slv.fcttyp = prm.fcttyp;
+
+    bcm::BlockCsr ac = assemble_rows(cb.obj, cb.fnsup, cb.fsprnds, cb.fiaja,
+                                     [&](int i, double* r) { cb.fblockrow(cb.obj, i, r); }, 2);
+    const int nsup = ac.nsup;
+    const int n = ac.n();
+    std::vector<double> bz(2 * (size_t)n, 0.0), xz(2 * (size_t)n, 0.0);
+    for (int i = 0; i < nsup; ++i) {
+      cb.frhs(cb.obj, i, &bz[2 * (size_t)ac.sprnd[i]]);
+      cb.finiguess(cb.obj, i, &xz[2 * (size_t)ac.sprnd[i]]);
+    }
+    auto store = [&] {
+      for (int i = 0; i < nsup; ++i) cb.fstoresol(cb.obj, i, &xz[2 * (size_t)ac.sprnd[i]]);
+    };
+    double bnorm2 = 0.0;
+    for (double v : bz) bnorm2 += v * v;
+    if (bnorm2 == 0.0) {  // no nonzero right-hand side: the guess is stored as it came (solvers.cpp:2725-2790)
+      prm.nitperf = 0;
+      prm.resfin = 0.0;
+      store();
+      return BCM_OK;
+    }
+
+    bcm::BlockCsr re, im;
+    bcm::split_complex(ac, re, im);
+    re.validate(false);
+    std::vector<int> order(nsup);
+    const bool reorder = prm.ordtype > 0;
+    if (reorder) order = bcm::rcm_order(re);  // PartOrdMtr with one part: OrderPrfMtr(1) (ordmatr.cpp:1009-1017)
+    else std::iota(order.begin(), order.end(), 0);
+    bcm::BlockCsr co = reorder ? bcm::merge_complex(bcm::ord_mtr(re, order), bcm::ord_mtr(im, order)) : ac;
+    const std::vector<int>& sp_new = co.sprnd;
+    std::vector<int> sp2_old(nsup + 1), sp2_new(nsup + 1);  // block offsets in doubles of a complex vector
+    for (int i = 0; i <= nsup; ++i) {
+      sp2_old[i] = 2 * ac.sprnd[i];
+      sp2_new[i] = 2 * sp_new[i];
+    }
+    if (reorder) {
+      bcm::ord_vect('D', sp2_old, sp2_new, order, bz.data(), 2 * n, 1);
+      bcm::ord_vect('D', sp2_old, sp2_new, order, xz.data(), 2 * n, 1);
+    }
+    bcm::BlockCsr al, au, l, u;
+    bcm::combine_lu_complex(co, al, au);
+    bcm::Factor f;
+    f.kind = 1;
+    bcm::ilu2_complex(al, au, slv, l, u, &f.stats);
+    f.l = bcm::real_equivalent(nsup, sp_new.data(), l.ia.data(), l.ja.data(), l.a.data(), true);
+    f.u = bcm::real_equivalent(nsup, sp_new.data(), u.ia.data(), u.ja.data(), u.a.data(), false);
+    bcm::Matrix mat;
+    mat.symmetric = false;
+    mat.h = bcm::real_equivalent(nsup, sp_new.data(), co.ia.data(), co.ja.data(), co.a.data(), false);
+    prm.timefct = f.stats.seconds;
+    prm.density = f.stats.densu;
+    prm.density2 = f.stats.densr;
+
+    // columns [b, i b] and [x0, i x0] in the real-equivalent layout
+    const size_t n2 = 2 * (size_t)n;
+    std::vector<double> rot(n2), breal(2 * n2), xreal(2 * n2);
+    auto times_i = [&](const std::vector<double>& z) {
+      for (size_t k = 0; k < (size_t)n; ++k) {
+        rot[2 * k] = -z[2 * k + 1];
+        rot[2 * k + 1] = z[2 * k];
+      }
+      return rot.data();
+    };
+    bcm::complex_to_real(nsup, sp_new.data(), bz.data(), breal.data(), 1);
+    bcm::complex_to_real(nsup, sp_new.data(), times_i(bz), breal.data() + n2, 1);
+    bcm::complex_to_real(nsup, sp_new.data(), xz.data(), xreal.data(), 1);
+    bcm::complex_to_real(nsup, sp_new.data(), times_i(xz), xreal.data() + n2, 1);
+
+    bcm::Context ctx;
+    DevVec db(2 * n2), dx(2 * n2);
+    BCM_HIP(hipMemcpy(db.p, breal.data(), 2 * n2 * 8, hipMemcpyHostToDevice));
+    BCM_HIP(hipMemcpy(dx.p, xreal.data(), 2 * n2 * 8, hipMemcpyHostToDevice));
+    bcm::SolveOptions opt;
+    opt.kind = bcm::kBlockGmres;
+    opt.niter = prm.niter;
+    opt.eps = prm.eps;
+    opt.ittype = 0;  // gmres.cpp:1308-1313: every column's ||r|| < eps ||r0||
+    opt.max_restarts = 0;
+    bcm::SolveStats st;
+    double resi[2] = {0.0, 0.0};
+    auto t0 = std::chrono::steady_clock::now();
+    bcm::solve_block_gmres(mat, f, ctx, opt, 2, db.p, dx.p, st, resi, nullptr);
+    BCM_HIP(hipStreamSynchronize(ctx.stream));
+    ctx.check_sweep_error();
+    prm.timeitr = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    BCM_HIP(hipMemcpy(xreal.data(), dx.p, n2 * 8, hipMemcpyDeviceToHost));
+    bcm::real_to_complex(nsup, sp_new.data(), xreal.data(), xz.data(), 1);
+    prm.nitperf = st.iterations;
+    prm.resfin = resi[0] / std::sqrt(bnorm2);
+    if (reorder) bcm::ord_vect('I', sp2_old, sp2_new, order, xz.data(), 2 * n, 1);
+    store();
+    if (st.breakdown) throw bcm::Error(bcm::kErrBreakdown, "Krylov breakdown (best iterate stored)");
+    return BCM_OK;
+  } catch (const bcm::Error& e) {
+    bcm::set_last_error(e.what());
+    return e.code;
+  } catch (const std::exception& e) {
+    bcm::set_last_error(e.what());
+    return BCM_E_INTERNAL;
+  }
+}
+
 }  // namespace
 
 extern "C" {
@@ -274,6 +428,13 @@ int bcm_abstract_par_solver(void* pgener, void* comm, bcm_params* p, bcm_cb_nsup
   (void)comm;  // one rank: the whole system lives on this process's GPU
   if (!fnsup || !fsprnds || !fiaja || !fblockrow || !frhs || !finiguess || !fstoresol) return BCM_E_ARG;
   return run(false, true, p, Callbacks{pgener, fnsup, fsprnds, fiaja, fblockrow, frhs, finiguess, fstoresol});
+}
+
+int bcm_abstract_solver_c(void* pgener, bcm_params* p, bcm_cb_nsup fnsup, bcm_cb_sprnds fsprnds,
+                          bcm_cb_iaja fiaja, bcm_cb_blockrow_c fblockrow, bcm_cb_rhs_c frhs,
+                          bcm_cb_iniguess_c finiguess, bcm_cb_storesol_c fstoresol) {
+  if (!fnsup || !fsprnds || !fiaja || !fblockrow || !frhs || !finiguess || !fstoresol) return BCM_E_ARG;
+  return run_complex(p, CCallbacks{pgener, fnsup, fsprnds, fiaja, fblockrow, frhs, finiguess, fstoresol});
 }
 
 }  // extern "C"

@@ -55,9 +55,16 @@ struct BlockCsr {
   void validate(bool require_diag_first) const;
 };
 
+// One complex value as the reference stores it (dcmplx, complex.h:17-19):
+// real part, then imaginary part; arrays of them are interleaved doubles.
+struct Zc {
+  double re, im;
+};
+
 struct SlvParam {
   int msglev = 0;
   int ordtype = 1;       // 1: RCM profile ordering, 0: identity
+  int fcttyp = 1;        // complex ILU2 only: 0 keeps the pattern of A, else thresholds (slvparam.h:33)
   double memory = -1.0;  // workspace bound in units of nza (<=0: unbounded)
   double tau1 = 1.0e-2;  // first-order threshold (stored in the factor)
   double tau2 = 1.0e-4;  // second-order threshold (kept for updates only)
@@ -162,6 +169,11 @@ int dsyev_vu(int n, double* a, int lda, double* w, double* work, int lwork);
 // doubles. Returns INFO (> 0: DBDSQR did not converge, < 0: bad argument).
 int dgesvd_aa(int n, double* a, int lda, double* s, double* u, int ldu, double* vt, int ldvt, double* work,
               int lwork);
+// ZGESVD with JOBU = JOBVT = 'A' on a square n x n complex matrix (LAPACK
+// 3.0): a = U diag(s) VT with VT = V^H, s descending; a is overwritten. work
+// holds lwork >= 3n complex values, rwork 5n doubles. Returns INFO.
+int zgesvd_aa(int n, Zc* a, int lda, double* s, Zc* u, int ldu, Zc* vt, int ldvt, Zc* work, int lwork,
+              double* rwork);
 }  // namespace lapack30
 
 // Symmetric eigen-decomposition of an n x n column-major matrix: eigenvalues
@@ -171,6 +183,9 @@ void sym_eig(int n, double* a, double* w);
 // Singular value decomposition a = U diag(s) V^T (square n x n, column-major):
 // s descending; u = U, vt = V^T (both column-major): dgesvd_aa on a copy of a.
 void svd_square(int n, const double* a, double* s, double* u, double* vt);
+// Complex a = U diag(s) V^H (square n x n, column-major): s descending, u = U,
+// vt = V^H: zgesvd_aa on a copy of a.
+void svd_square_complex(int n, const Zc* a, double* s, Zc* u, Zc* vt);
 
 // ---- factorizations (ic2.cpp, ilu2.cpp) ----
 
@@ -191,6 +206,24 @@ BlockCsr ich2(const BlockCsr& upper, const SlvParam& p, FactorStats* st = nullpt
 // diagonal blocks, A ~ L U.
 void ilu2(const BlockCsr& al, const BlockCsr& au, const SlvParam& p, BlockCsr& l, BlockCsr& u,
           FactorStats* st = nullptr);
+
+// Complex block matrices on the host: the pattern of a BlockCsr whose `a`
+// holds interleaved (re, im) doubles, two per entry, with `bsa` counting
+// complex entries (the value of entry k sits at a[2k], a[2k + 1]).
+// Real and imaginary parts as two real matrices of one pattern, and back.
+void split_complex(const BlockCsr& c, BlockCsr& re, BlockCsr& im);
+BlockCsr merge_complex(const BlockCsr& re, const BlockCsr& im);
+// CSMatrixCS::CombineLU (smatrixcs.cpp:1185-1345): the L part transposed
+// without conjugation (TranspMtrNoConj) and both parts padded to the union
+// pattern; every step moves values only, so it is combine_lu on the real and
+// imaginary parts.
+void combine_lu_complex(const BlockCsr& a, BlockCsr& l, BlockCsr& u);
+// Second-order incomplete LU of a complex matrix (CSMatrixCS::Ilu2,
+// ilu2cs.cpp:1213-1567) from complex L-by-columns / U-by-rows parts with one
+// pattern (combine_lu_complex). Returns complex L by columns and U by rows
+// with inverted diagonal blocks, A ~ L U.
+void ilu2_complex(const BlockCsr& al, const BlockCsr& au, const SlvParam& p, BlockCsr& l, BlockCsr& u,
+                  FactorStats* st = nullptr);
 
 // ---- synthetic systems (generate.cpp) ----
 

@@ -82,6 +82,38 @@ BlockCsr real_equivalent(int nsup, const int* sprnd, const int* ia, const int* j
   return r;
 }
 
+void split_complex(const BlockCsr& c, BlockCsr& re, BlockCsr& im) {
+  if (c.a.size() % 2) throw Error(kErrArg, "complex matrix: odd number of values");
+  re = c;
+  re.a.resize(c.a.size() / 2);
+  im = re;
+  for (size_t k = 0; k < re.a.size(); ++k) {
+    re.a[k] = c.a[2 * k];
+    im.a[k] = c.a[2 * k + 1];
+  }
+}
+
+BlockCsr merge_complex(const BlockCsr& re, const BlockCsr& im) {
+  if (re.ja != im.ja || re.ia != im.ia || re.bsa != im.bsa || re.a.size() != im.a.size())
+    throw Error(kErrArg, "complex matrix: real and imaginary parts differ in pattern");
+  BlockCsr c = re;
+  c.a.resize(2 * re.a.size());
+  for (size_t k = 0; k < re.a.size(); ++k) {
+    c.a[2 * k] = re.a[k];
+    c.a[2 * k + 1] = im.a[k];
+  }
+  return c;
+}
+
+void combine_lu_complex(const BlockCsr& a, BlockCsr& l, BlockCsr& u) {
+  BlockCsr re, im, lre, lim, ure, uim;
+  split_complex(a, re, im);
+  combine_lu(re, lre, ure);
+  combine_lu(im, lim, uim);
+  l = merge_complex(lre, lim);
+  u = merge_complex(ure, uim);
+}
+
 void complex_to_real(int nsup, const int* sprnd, const double* z, double* x, int nrhs) {
   const int64_t n = sprnd[nsup];
   for (int q = 0; q < nrhs; ++q)

@@ -2,7 +2,8 @@
 // dense blocks, restating the LAPACK 3.0 routines the reference's IC2 and
 // ILU2 call (lapackd.cpp:865-905 -> lapackdm.f -> lapacksm/dsyev.f,
 // dgesvd.f): DSYEV with JOBZ = 'V', UPLO = 'U' and DGESVD with JOBU =
-// JOBVT = 'A' on a square matrix.
+// JOBVT = 'A' on a square matrix; and ZGESVD with JOBU = JOBVT = 'A'
+// (lapacksm/zgesvd.f), which the complex ILU2 calls (ilu2cs.cpp:132, 725).
 //
 // Factor values depend on every rounding of these decompositions, so each
 // routine follows its published algorithm operation by operation: the same
@@ -874,17 +875,96 @@ void dorgbr_p_square(int n, double* a, int lda, const double* tau, double* work)
   if (n > 1) dorgl2(n - 1, n - 1, n - 1, &A(1, 1), lda, tau, work);
 }
 
+// ---- the vector updates of the bidiagonal QR, on real or complex vectors ----
+//
+// ZBDSQR is DBDSQR with its rotations, sign flips and swaps applied to complex
+// vectors (ZDROT, ZLASR, ZDSCAL, ZSWAP); the bidiagonal itself stays real.
+
+inline void rot_v(int n, double* x, int incx, double* y, int incy, double c, double s) { drot(n, x, incx, y, incy, c, s); }
+inline void lasr_v(bool left, bool forward, int m, int n, const double* c, const double* s, double* a, int lda) {
+  dlasr(left, forward, m, n, c, s, a, lda);
+}
+inline void scal_v(int n, double a, double* x, int incx) { dscal(n, a, x, incx); }
+inline void swap_v(int n, double* x, int incx, double* y, int incy) { dswap(n, x, incx, y, incy); }
+
+// Complex arithmetic as the compiled Fortran evaluates it: a product of two
+// complex values is (ac - bd, ad + bc); a real times a complex value scales
+// both parts. The two forms differ at most in the sign of a zero part.
+inline Zc zmul(Zc a, Zc b) { return {a.re * b.re - a.im * b.im, a.re * b.im + a.im * b.re}; }
+inline Zc zadd(Zc a, Zc b) { return {a.re + b.re, a.im + b.im}; }
+inline Zc zsub(Zc a, Zc b) { return {a.re - b.re, a.im - b.im}; }
+inline Zc zconj(Zc a) { return {a.re, -a.im}; }
+inline Zc zneg(Zc a) { return {-a.re, -a.im}; }
+inline Zc zrs(double c, Zc a) { return {c * a.re, c * a.im}; }
+inline bool znonzero(Zc a) { return a.re != 0.0 || a.im != 0.0; }
+constexpr Zc kZero{0.0, 0.0};
+constexpr Zc kOne{1.0, 0.0};
+
+// ZDROT: x, y <- c x + s y, c y - s x.
+void rot_v(int n, Zc* x, int incx, Zc* y, int incy, double c, double s) {
+  for (int i = 0; i < n; ++i) {
+    Zc& xi = x[(size_t)i * incx];
+    Zc& yi = y[(size_t)i * incy];
+    const Zc t = zadd(zrs(c, xi), zrs(s, yi));
+    yi = zsub(zrs(c, yi), zrs(s, xi));
+    xi = t;
+  }
+}
+
+// ZLASR: dlasr on complex entries.
+void lasr_v(bool left, bool forward, int m, int n, const double* c, const double* s, Zc* a, int lda) {
+  if (m <= 0 || n <= 0) return;
+  auto A = [&](int i, int j) -> Zc& { return a[i + (size_t)j * lda]; };
+  auto rotate = [&](Zc& lo, Zc& hi, double ct, double st) {
+    const Zc t = hi;
+    hi = zsub(zrs(ct, t), zrs(st, lo));
+    lo = zadd(zrs(st, t), zrs(ct, lo));
+  };
+  if (left) {
+    auto apply = [&](int j) {
+      const double ct = c[j], st = s[j];
+      if (ct != 1.0 || st != 0.0)
+        for (int i = 0; i < n; ++i) rotate(A(j, i), A(j + 1, i), ct, st);
+    };
+    if (forward)
+      for (int j = 0; j < m - 1; ++j) apply(j);
+    else
+      for (int j = m - 2; j >= 0; --j) apply(j);
+  } else {
+    auto apply = [&](int j) {
+      const double ct = c[j], st = s[j];
+      if (ct != 1.0 || st != 0.0)
+        for (int i = 0; i < m; ++i) rotate(A(i, j), A(i, j + 1), ct, st);
+    };
+    if (forward)
+      for (int j = 0; j < n - 1; ++j) apply(j);
+    else
+      for (int j = n - 2; j >= 0; --j) apply(j);
+  }
+}
+
+// ZDSCAL: x <- dcmplx(a, 0) x, a complex product.
+void scal_v(int n, double a, Zc* x, int incx) {
+  if (n <= 0) return;
+  for (int i = 0; i < n; ++i) x[(size_t)i * incx] = zmul(Zc{a, 0.0}, x[(size_t)i * incx]);
+}
+
+void swap_v(int n, Zc* x, int incx, Zc* y, int incy) {
+  for (int i = 0; i < n; ++i) std::swap(x[(size_t)i * incx], y[(size_t)i * incy]);
+}
+
 // SVD of an upper bidiagonal (d, e) by implicit zero-shift / shifted QR,
 // rotating the rows of vt (n x ncvt) and the columns of u (nru x n) (DBDSQR
-// 'U' with NCC = 0). Returns INFO.
-int dbdsqr_u(int n, int ncvt, int nru, double* d, double* e, double* vt, int ldvt, double* u, int ldu,
+// 'U' with NCC = 0 for real T, ZBDSQR for complex T). Returns INFO.
+template <class T>
+int bdsqr_u(int n, int ncvt, int nru, double* d, double* e, T* vt, int ldvt, T* u, int ldu,
              double* work) {
   if (n == 0) return 0;
   auto D = [&](int i) -> double& { return d[i - 1]; };
   auto E = [&](int i) -> double& { return e[i - 1]; };
   auto W = [&](int i) -> double& { return work[i - 1]; };
-  auto VT = [&](int i, int j) -> double& { return vt[(i - 1) + (size_t)(j - 1) * ldvt]; };
-  auto U = [&](int i, int j) -> double& { return u[(i - 1) + (size_t)(j - 1) * ldu]; };
+  auto VT = [&](int i, int j) -> T& { return vt[(i - 1) + (size_t)(j - 1) * ldvt]; };
+  auto U = [&](int i, int j) -> T& { return u[(i - 1) + (size_t)(j - 1) * ldu]; };
   const double hndrth = 0.01, ten = 10.0, hndrd = 100.0, meigth = -0.125;
   const int maxitr = 6;
   int info = 0;
@@ -957,8 +1037,8 @@ int dbdsqr_u(int n, int ncvt, int nru, double* d, double* e, double* vt, int ldv
         D(m - 1) = sigmx;
         E(m - 1) = 0.0;
         D(m) = sigmn;
-        if (ncvt > 0) drot(ncvt, &VT(m - 1, 1), ldvt, &VT(m, 1), ldvt, cosr, sinr);
-        if (nru > 0) drot(nru, &U(1, m - 1), 1, &U(1, m), 1, cosl, sinl);
+        if (ncvt > 0) rot_v(ncvt, &VT(m - 1, 1), ldvt, &VT(m, 1), ldvt, cosr, sinr);
+        if (nru > 0) rot_v(nru, &U(1, m - 1), 1, &U(1, m), 1, cosl, sinl);
         m = m - 2;
         continue;
       }
@@ -1042,8 +1122,8 @@ int dbdsqr_u(int n, int ncvt, int nru, double* d, double* e, double* vt, int ldv
           const double h = D(m) * cs;
           D(m) = h * oldcs;
           E(m - 1) = h * oldsn;
-          if (ncvt > 0) dlasr(true, true, m - ll + 1, ncvt, &W(1), &W(n), &VT(ll, 1), ldvt);
-          if (nru > 0) dlasr(false, true, nru, m - ll + 1, &W(nm12 + 1), &W(nm13 + 1), &U(1, ll), ldu);
+          if (ncvt > 0) lasr_v(true, true, m - ll + 1, ncvt, &W(1), &W(n), &VT(ll, 1), ldvt);
+          if (nru > 0) lasr_v(false, true, nru, m - ll + 1, &W(nm12 + 1), &W(nm13 + 1), &U(1, ll), ldu);
           if (std::fabs(E(m - 1)) <= thresh) E(m - 1) = 0.0;
         } else {
           double cs = 1.0, oldcs = 1.0, sn = 0.0, oldsn = 0.0;
@@ -1059,8 +1139,8 @@ int dbdsqr_u(int n, int ncvt, int nru, double* d, double* e, double* vt, int ldv
           const double h = D(ll) * cs;
           D(ll) = h * oldcs;
           E(ll) = h * oldsn;
-          if (ncvt > 0) dlasr(true, false, m - ll + 1, ncvt, &W(nm12 + 1), &W(nm13 + 1), &VT(ll, 1), ldvt);
-          if (nru > 0) dlasr(false, false, nru, m - ll + 1, &W(1), &W(n), &U(1, ll), ldu);
+          if (ncvt > 0) lasr_v(true, false, m - ll + 1, ncvt, &W(nm12 + 1), &W(nm13 + 1), &VT(ll, 1), ldvt);
+          if (nru > 0) lasr_v(false, false, nru, m - ll + 1, &W(1), &W(n), &U(1, ll), ldu);
           if (std::fabs(E(ll)) <= thresh) E(ll) = 0.0;
         }
       } else {
@@ -1089,8 +1169,8 @@ int dbdsqr_u(int n, int ncvt, int nru, double* d, double* e, double* vt, int ldv
             W(i - ll + 1 + nm13) = sinl;
           }
           E(m - 1) = f;
-          if (ncvt > 0) dlasr(true, true, m - ll + 1, ncvt, &W(1), &W(n), &VT(ll, 1), ldvt);
-          if (nru > 0) dlasr(false, true, nru, m - ll + 1, &W(nm12 + 1), &W(nm13 + 1), &U(1, ll), ldu);
+          if (ncvt > 0) lasr_v(true, true, m - ll + 1, ncvt, &W(1), &W(n), &VT(ll, 1), ldvt);
+          if (nru > 0) lasr_v(false, true, nru, m - ll + 1, &W(nm12 + 1), &W(nm13 + 1), &U(1, ll), ldu);
           if (std::fabs(E(m - 1)) <= thresh) E(m - 1) = 0.0;
         } else {
           double f = (std::fabs(D(m)) - shift) * (fsign(1.0, D(m)) + shift / D(m));
@@ -1118,8 +1198,8 @@ int dbdsqr_u(int n, int ncvt, int nru, double* d, double* e, double* vt, int ldv
           }
           E(ll) = f;
           if (std::fabs(E(ll)) <= thresh) E(ll) = 0.0;
-          if (ncvt > 0) dlasr(true, false, m - ll + 1, ncvt, &W(nm12 + 1), &W(nm13 + 1), &VT(ll, 1), ldvt);
-          if (nru > 0) dlasr(false, false, nru, m - ll + 1, &W(1), &W(n), &U(1, ll), ldu);
+          if (ncvt > 0) lasr_v(true, false, m - ll + 1, ncvt, &W(nm12 + 1), &W(nm13 + 1), &VT(ll, 1), ldvt);
+          if (nru > 0) lasr_v(false, false, nru, m - ll + 1, &W(1), &W(n), &U(1, ll), ldu);
         }
       }
     }
@@ -1128,7 +1208,7 @@ int dbdsqr_u(int n, int ncvt, int nru, double* d, double* e, double* vt, int ldv
   for (int i = 1; i <= n; ++i) {
     if (D(i) < 0.0) {
       D(i) = -D(i);
-      if (ncvt > 0) dscal(ncvt, -1.0, &VT(i, 1), ldvt);
+      if (ncvt > 0) scal_v(ncvt, -1.0, &VT(i, 1), ldvt);
     }
   }
   for (int i = 1; i <= n - 1; ++i) {
@@ -1142,11 +1222,241 @@ int dbdsqr_u(int n, int ncvt, int nru, double* d, double* e, double* vt, int ldv
     if (isub != n + 1 - i) {
       D(isub) = D(n + 1 - i);
       D(n + 1 - i) = smin;
-      if (ncvt > 0) dswap(ncvt, &VT(isub, 1), ldvt, &VT(n + 1 - i, 1), ldvt);
-      if (nru > 0) dswap(nru, &U(1, isub), 1, &U(1, n + 1 - i), 1);
+      if (ncvt > 0) swap_v(ncvt, &VT(isub, 1), ldvt, &VT(n + 1 - i, 1), ldvt);
+      if (nru > 0) swap_v(nru, &U(1, isub), 1, &U(1, n + 1 - i), 1);
     }
   }
   return info;
+}
+
+// ---- ZGESVD call tree (square, JOBU = JOBVT = 'A') ----
+//
+// Under the same blocking argument as the real tree, ZGEBRD, ZUNGQR and
+// ZUNGLQ take their unblocked paths: ZGEBD2, ZUNG2R, ZUNGL2.
+
+// DZNRM2: the scaled sum of squares over the real and the imaginary part of
+// each entry in turn.
+double dznrm2(int n, const Zc* x, int incx) {
+  if (n < 1 || incx < 1) return 0.0;
+  double scale = 0.0, ssq = 1.0;
+  auto add = [&](double v) {
+    if (v != 0.0) {
+      const double temp = std::fabs(v);
+      if (scale < temp) {
+        const double q = scale / temp;
+        ssq = 1.0 + ssq * (q * q);
+        scale = temp;
+      } else {
+        const double q = temp / scale;
+        ssq = ssq + q * q;
+      }
+    }
+  };
+  for (int i = 0; i < n; ++i) {
+    add(x[(size_t)i * incx].re);
+    add(x[(size_t)i * incx].im);
+  }
+  return scale * std::sqrt(ssq);
+}
+
+// sqrt(x^2 + y^2 + z^2) scaled by the largest magnitude (DLAPY3).
+double dlapy3(double x, double y, double z) {
+  const double xa = std::fabs(x), ya = std::fabs(y), za = std::fabs(z);
+  const double w = std::max(std::max(xa, ya), za);
+  if (w == 0.0) return 0.0;
+  const double qx = xa / w, qy = ya / w, qz = za / w;
+  return w * std::sqrt(qx * qx + qy * qy + qz * qz);
+}
+
+// (a + ib) / (c + id) by Smith's scaling (DLADIV; ZLADIV calls it).
+void dladiv(double a, double b, double c, double d, double& p, double& q) {
+  if (std::fabs(d) < std::fabs(c)) {
+    const double e = d / c, f = c + d * e;
+    p = (a + b * e) / f;
+    q = (b - a * e) / f;
+  } else {
+    const double e = c / d, f = d + c * e;
+    p = (b + a * e) / f;
+    q = (-a + b * e) / f;
+  }
+}
+
+// ZSCAL: x <- a x.
+void zscal(int n, Zc a, Zc* x, int incx) {
+  for (int i = 0; i < n; ++i) x[(size_t)i * incx] = zmul(a, x[(size_t)i * incx]);
+}
+
+// ZLACGV: conjugates n entries.
+void zlacgv(int n, Zc* x, int incx) {
+  for (int i = 0; i < n; ++i) x[(size_t)i * incx].im = -x[(size_t)i * incx].im;
+}
+
+// Elementary reflector H = I - tau v v^H with H^H (alpha, x) = (beta, 0),
+// beta real (ZLARFG): unlike the real reflector, it is formed for n = 1 as
+// well whenever alpha is not real.
+void zlarfg(int n, Zc& alpha, Zc* x, int incx, Zc& tau) {
+  if (n <= 0) {
+    tau = kZero;
+    return;
+  }
+  double xnorm = dznrm2(n - 1, x, incx);
+  double alphr = alpha.re, alphi = alpha.im;
+  if (xnorm == 0.0 && alphi == 0.0) {
+    tau = kZero;
+    return;
+  }
+  double beta = -fsign(dlapy3(alphr, alphi, xnorm), alphr);
+  const double safmin = kSafmin / kEps, rsafmn = 1.0 / safmin;
+  int knt = 0;
+  if (std::fabs(beta) < safmin) {
+    do {
+      ++knt;
+      scal_v(n - 1, rsafmn, x, incx);
+      beta = beta * rsafmn;
+      alphi = alphi * rsafmn;
+      alphr = alphr * rsafmn;
+    } while (std::fabs(beta) < safmin);
+    xnorm = dznrm2(n - 1, x, incx);
+    alpha = Zc{alphr, alphi};
+    beta = -fsign(dlapy3(alphr, alphi, xnorm), alphr);
+  }
+  tau = Zc{(beta - alphr) / beta, -alphi / beta};
+  Zc inv;
+  dladiv(1.0, 0.0, alpha.re - beta, alpha.im, inv.re, inv.im);  // 1 / (alpha - beta)
+  zscal(n - 1, inv, x, incx);
+  alpha = Zc{beta, 0.0};
+  for (int j = 0; j < knt; ++j) alpha = zrs(safmin, alpha);
+}
+
+// y = A^H x (ZGEMV 'C' with alpha = 1, beta = 0): one column sum at a time.
+void zgemv_c(int m, int n, const Zc* a, int lda, const Zc* x, int incx, Zc* y) {
+  if (m == 0 || n == 0) return;
+  for (int j = 0; j < n; ++j) y[j] = kZero;
+  for (int j = 0; j < n; ++j) {
+    Zc t = kZero;
+    for (int i = 0; i < m; ++i) t = zadd(t, zmul(zconj(a[i + (size_t)j * lda]), x[(size_t)i * incx]));
+    y[j] = zadd(y[j], zmul(kOne, t));
+  }
+}
+
+// y = A x (ZGEMV 'N' with alpha = 1, beta = 0): columns whose x entry is zero skipped.
+void zgemv_n(int m, int n, const Zc* a, int lda, const Zc* x, int incx, Zc* y) {
+  if (m == 0 || n == 0) return;
+  for (int i = 0; i < m; ++i) y[i] = kZero;
+  for (int j = 0; j < n; ++j) {
+    const Zc xj = x[(size_t)j * incx];
+    if (znonzero(xj)) {
+      const Zc t = zmul(kOne, xj);
+      for (int i = 0; i < m; ++i) y[i] = zadd(y[i], zmul(t, a[i + (size_t)j * lda]));
+    }
+  }
+}
+
+// A += alpha x y^H (ZGERC): columns whose y entry is zero skipped.
+void zgerc(int m, int n, Zc alpha, const Zc* x, int incx, const Zc* y, int incy, Zc* a, int lda) {
+  if (m == 0 || n == 0 || !znonzero(alpha)) return;
+  for (int j = 0; j < n; ++j) {
+    const Zc yj = y[(size_t)j * incy];
+    if (znonzero(yj)) {
+      const Zc t = zmul(alpha, zconj(yj));
+      for (int i = 0; i < m; ++i) a[i + (size_t)j * lda] = zadd(a[i + (size_t)j * lda], zmul(x[(size_t)i * incx], t));
+    }
+  }
+}
+
+// Applies H = I - tau v v^H from the left (C = H C) or the right (C = C H) (ZLARF).
+void zlarf(bool left, int m, int n, const Zc* v, int incv, Zc tau, Zc* c, int ldc, Zc* work) {
+  if (!znonzero(tau)) return;
+  if (left) {
+    zgemv_c(m, n, c, ldc, v, incv, work);
+    zgerc(m, n, zneg(tau), v, incv, work, 1, c, ldc);
+  } else {
+    zgemv_n(m, n, c, ldc, v, incv, work);
+    zgerc(m, n, zneg(tau), work, 1, v, incv, c, ldc);
+  }
+}
+
+// Bidiagonal reduction Q^H A P = B, real upper bidiagonal for m >= n (ZGEBD2).
+void zgebd2(int m, int n, Zc* a, int lda, double* d, double* e, Zc* tauq, Zc* taup, Zc* work) {
+  auto A = [&](int i, int j) -> Zc& { return a[i + (size_t)j * lda]; };
+  for (int i = 0; i < n; ++i) {
+    Zc alpha = A(i, i);
+    zlarfg(m - i, alpha, &A(std::min(i + 1, m - 1), i), 1, tauq[i]);
+    d[i] = alpha.re;
+    A(i, i) = kOne;
+    zlarf(true, m - i, n - i - 1, &A(i, i), 1, zconj(tauq[i]), &A(i, i + 1 < n ? i + 1 : i), lda, work);
+    A(i, i) = Zc{d[i], 0.0};
+    if (i < n - 1) {
+      zlacgv(n - i - 1, &A(i, i + 1), lda);
+      alpha = A(i, i + 1);
+      zlarfg(n - i - 1, alpha, &A(i, std::min(i + 2, n - 1)), lda, taup[i]);
+      e[i] = alpha.re;
+      A(i, i + 1) = kOne;
+      zlarf(false, m - i - 1, n - i - 1, &A(i, i + 1), lda, taup[i], &A(i + 1 < m ? i + 1 : i, i + 1), lda, work);
+      zlacgv(n - i - 1, &A(i, i + 1), lda);
+      A(i, i + 1) = Zc{e[i], 0.0};
+    } else {
+      taup[i] = kZero;
+    }
+  }
+}
+
+// Q with orthonormal columns from k reflectors stored in the columns (ZUNG2R).
+void zung2r(int m, int n, int k, Zc* a, int lda, const Zc* tau, Zc* work) {
+  if (n <= 0) return;
+  auto A = [&](int i, int j) -> Zc& { return a[i + (size_t)j * lda]; };
+  for (int j = k; j < n; ++j) {
+    for (int l = 0; l < m; ++l) A(l, j) = kZero;
+    A(j, j) = kOne;
+  }
+  for (int i = k - 1; i >= 0; --i) {
+    if (i < n - 1) {
+      A(i, i) = kOne;
+      zlarf(true, m - i, n - i - 1, &A(i, i), 1, tau[i], &A(i, i + 1), lda, work);
+    }
+    if (i < m - 1) zscal(m - i - 1, zneg(tau[i]), &A(i + 1, i), 1);
+    A(i, i) = zsub(kOne, tau[i]);
+    for (int l = 0; l < i; ++l) A(l, i) = kZero;
+  }
+}
+
+// Q with orthonormal rows from k reflectors stored in the rows (ZUNGL2).
+void zungl2(int m, int n, int k, Zc* a, int lda, const Zc* tau, Zc* work) {
+  if (m <= 0) return;
+  auto A = [&](int i, int j) -> Zc& { return a[i + (size_t)j * lda]; };
+  if (k < m) {
+    for (int j = 0; j < n; ++j) {
+      for (int l = k; l < m; ++l) A(l, j) = kZero;
+      if (j >= k && j < m) A(j, j) = kOne;
+    }
+  }
+  for (int i = k - 1; i >= 0; --i) {
+    if (i < n - 1) {
+      zlacgv(n - i - 1, &A(i, i + 1), lda);
+      if (i < m - 1) {
+        A(i, i) = kOne;
+        zlarf(false, m - i - 1, n - i, &A(i, i), lda, zconj(tau[i]), &A(i + 1, i), lda, work);
+      }
+      zscal(n - i - 1, zneg(tau[i]), &A(i, i + 1), lda);
+      zlacgv(n - i - 1, &A(i, i + 1), lda);
+    }
+    A(i, i) = zsub(kOne, zconj(tau[i]));
+    for (int l = 0; l < i; ++l) A(i, l) = kZero;
+  }
+}
+
+// P^H of the bidiagonal reduction of a square n x n matrix (ZUNGBR 'P' with
+// k >= n: shift the reflectors one row down, then ZUNGLQ -> ZUNGL2).
+void zungbr_p_square(int n, Zc* a, int lda, const Zc* tau, Zc* work) {
+  if (n == 0) return;
+  auto A = [&](int i, int j) -> Zc& { return a[i + (size_t)j * lda]; };
+  A(0, 0) = kOne;
+  for (int i = 1; i < n; ++i) A(i, 0) = kZero;
+  for (int j = 1; j < n; ++j) {
+    for (int i = j - 1; i >= 1; --i) A(i, j) = A(i - 1, j);
+    A(0, j) = kZero;
+  }
+  if (n > 1) zungl2(n - 1, n - 1, n - 1, &A(1, 1), lda, tau, work);
 }
 
 }  // namespace
@@ -1229,7 +1539,54 @@ int dgesvd_aa(int n, double* a, int lda, double* s, double* u, int ldu, double* 
   for (int j = 0; j < n; ++j)  // DLACPY 'U'
     for (int i = 0; i <= j; ++i) vt[i + (size_t)j * ldvt] = a[i + (size_t)j * lda];
   dorgbr_p_square(n, vt, ldvt, taup, wrk);
-  const int info = dbdsqr_u(n, n, n, s, e, vt, ldvt, u, ldu, work + n);
+  const int info = bdsqr_u(n, n, n, s, e, vt, ldvt, u, ldu, work + n);
+  if (iscl == 1) {
+    if (anrm > bignum) dlascl(0, bignum, anrm, n, 1, s, n);
+    if (anrm < smlnum) dlascl(0, smlnum, anrm, n, 1, s, n);
+  }
+  return info;
+}
+
+int zgesvd_aa(int n, Zc* a, int lda, double* s, Zc* u, int ldu, Zc* vt, int ldvt, Zc* work, int lwork,
+              double* rwork) {
+  if (n < 0 || lda < std::max(1, n) || ldu < std::max(1, n) || ldvt < std::max(1, n)) return -1;
+  if (lwork < std::max(1, 3 * n)) return -13;
+  if (n == 0) return 0;
+  // n = 1 takes the QR path (m >= int(1.6 n)), which is not restated: the
+  // reference's complex factorizations treat 1 x 1 blocks themselves
+  // (ilu2cs.cpp:137-146, 730-739).
+  if (n == 1) return -1;
+  const double eps = kPrec;
+  const double smlnum = std::sqrt(kSafmin) / eps, bignum = 1.0 / smlnum;
+  double anrm = 0.0;  // ZLANGE 'M': the largest modulus
+  for (int j = 0; j < n; ++j)
+    for (int i = 0; i < n; ++i) {
+      const Zc v = a[i + (size_t)j * lda];
+      anrm = std::max(anrm, std::hypot(v.re, v.im));
+    }
+  // ZLASCL 'G' scales both parts of every entry by the same real factors:
+  // DLASCL over the interleaved doubles
+  double* ad = reinterpret_cast<double*>(a);
+  int iscl = 0;
+  if (anrm > 0.0 && anrm < smlnum) {
+    iscl = 1;
+    dlascl(0, anrm, smlnum, 2 * n, n, ad, 2 * lda);
+  } else if (anrm > bignum) {
+    iscl = 1;
+    dlascl(0, anrm, bignum, 2 * n, n, ad, 2 * lda);
+  }
+  double* e = rwork;
+  Zc* tauq = work;
+  Zc* taup = work + n;
+  Zc* wrk = work + 2 * n;
+  zgebd2(n, n, a, lda, s, e, tauq, taup, wrk);
+  for (int j = 0; j < n; ++j)  // ZLACPY 'L'
+    for (int i = j; i < n; ++i) u[i + (size_t)j * ldu] = a[i + (size_t)j * lda];
+  zung2r(n, n, n, u, ldu, tauq, wrk);  // ZUNGBR 'Q' with m >= k -> ZUNGQR
+  for (int j = 0; j < n; ++j)  // ZLACPY 'U'
+    for (int i = 0; i <= j; ++i) vt[i + (size_t)j * ldvt] = a[i + (size_t)j * lda];
+  zungbr_p_square(n, vt, ldvt, taup, wrk);
+  const int info = bdsqr_u(n, n, n, s, e, vt, ldvt, u, ldu, rwork + n);
   if (iscl == 1) {
     if (anrm > bignum) dlascl(0, bignum, anrm, n, 1, s, n);
     if (anrm < smlnum) dlascl(0, smlnum, anrm, n, 1, s, n);

@@ -66,6 +66,8 @@ typedef struct bcm_params {
   double timefct; /* out: host factorization seconds */
   double timeitr; /* out: solve seconds */
   double density, density2; /* out: factor fill (first / second order) per nza(A) */
+  int fcttyp;     /* complex ILU2 only (ilu2cs.cpp:613-651, 1014-1049): 0 keeps exactly the blocks of
+                     the pattern of A, otherwise tau1 / tau2 thresholds (slvparam.h:33; default 1) */
 } bcm_params;
 
 typedef struct bcm_solve_opts {
@@ -114,10 +116,17 @@ int bcm_cmatrix_create(int nsup, const int* sprnd, const int* ia, const int* ja,
  * columns with transposed blocks, U by rows, inverted diagonal blocks first in
  * every row; values interleaved re/im), held as their real equivalents for the
  * sweeps of SolveL / SolveU (mvmc.cpp:1723, 2506) and every preconditioned
- * solver. The complex factorization itself is the caller's (this library
- * factors real matrices only). */
+ * solver; bcm_cfactor_ilu2 computes them. */
 int bcm_cfactor_create(int nsup, const int* sprnd, const int* lia, const int* lja, const double* la,
                        const int* uia, const int* uja, const double* ua, bcm_factor* out);
+/* CSMatrixCS::CombineLU (smatrixcs.cpp:1185) + CSMatrixCS::Ilu2
+ * (ilu2cs.cpp:1213-1567) of a complex matrix in the given order (arrays as
+ * bcm_cmatrix_create; the diagonal block first in every row), bit for bit
+ * the reference's factors (its ZGESVD restated from LAPACK 3.0), held as
+ * their real equivalents like bcm_cfactor_create's. Reads tau1, tau2, theta,
+ * sclmin, pivmin, memory and fcttyp; writes timefct, density, density2. */
+int bcm_cfactor_ilu2(int nsup, const int* sprnd, const int* ia, const int* ja, const double* a, bcm_params* p,
+                     bcm_factor* out);
 /* Complex vectors (interleaved re/im, n x nrhs, CSVectorC's layout) to and
  * from the real-equivalent layout (per block row [Re z; Im z], 2n x nrhs). */
 int bcm_complex_to_real(int nsup, const int* sprnd, const double* z, double* x, int nrhs);
@@ -176,6 +185,11 @@ int bcm_dsyev(int n, double* a, double* w);
  * -> LAPACK 3.0 DGESVD, ilu2rs.cpp:126): a = U diag(s) VT, s descending; a is
  * overwritten. */
 int bcm_dgesvd(int n, double* a, double* s, double* u, double* vt);
+/* zgesvd_("A", "A") of a square complex matrix as the complex ILU2 calls it
+ * (ilu2cs.cpp:132, 725; LAPACK 3.0 ZGESVD): a = U diag(s) VT with VT = V^H,
+ * complex arrays interleaved re/im, column-major; a is overwritten; n >= 2
+ * (the factorization treats 1 x 1 blocks itself). */
+int bcm_zgesvd(int n, double* a, double* s, double* u, double* vt);
 
 /* CSMatrixRS::CombineLU (smatrixrs.cpp:1645-1803): splits a full-storage
  * matrix into L stored by columns (as the transpose of its strict lower part,
@@ -380,6 +394,12 @@ typedef void (*bcm_cb_blockrow)(void* obj, int isup, double* arow);
 typedef void (*bcm_cb_rhs)(void* obj, int isup, double* rhs);
 typedef void (*bcm_cb_iniguess)(void* obj, int isup, double* xini);
 typedef void (*bcm_cb_storesol)(void* obj, int isup, double* sol);
+/* CBLOCKROWC / CRHSC / CINIGUESSC / CSTORESOLC (smatrix.h:21-24): complex
+ * data, interleaved (re, im) doubles, behind a void pointer. */
+typedef void (*bcm_cb_blockrow_c)(void* obj, int isup, void* arow);
+typedef void (*bcm_cb_rhs_c)(void* obj, int isup, void* rhs);
+typedef void (*bcm_cb_iniguess_c)(void* obj, int isup, void* xini);
+typedef void (*bcm_cb_storesol_c)(void* obj, int isup, void* sol);
 
 /* AbstractSymmSolver (solvers.cpp:5383): assemble upper storage, RCM, IC2,
  * GPU PCG, return the solution through storesol. */
@@ -405,6 +425,24 @@ int bcm_abstract_solver(void* pgener, bcm_params* p, bcm_cb_nsup fnsup, bcm_cb_s
 int bcm_abstract_par_solver(void* pgener, void* comm, bcm_params* p, bcm_cb_nsup fnsup, bcm_cb_sprnds fsprnds,
                             bcm_cb_iaja fiaja, bcm_cb_blockrow fblockrow, bcm_cb_rhs frhs,
                             bcm_cb_iniguess finiguess, bcm_cb_storesol fstoresol);
+
+/* The complex AbstractSolver (solvers.cpp:7103-7398) and the dispatcher it
+ * calls, CSVectorC::Solver (solvers.cpp:2588-2830): assemble the complex
+ * block rows, RCM profile ordering when ordtype = 1 (PartOrdMtr with one
+ * part is OrderPrfMtr(1)), none when 0 (ordtype > 1, METIS parts, and
+ * ordtype < 0, OrderBrdMtr: BCM_E_ARG),
+ * CombineLU + complex ILU2 (bcm_cfactor_ilu2), then block GMRES
+ * (CSVectorC::BlockGMRES, gmres.cpp:1044-1466: one cycle of niter steps,
+ * stopped when ||r|| < eps ||r0||) on the GPU, and the solution through
+ * fstoresol. A zero right-hand side stores the initial guess unchanged, as
+ * the reference's solver does (it solves only nonzero columns). The
+ * complex Krylov space over the real equivalent: the real block GMRES of
+ * the columns [b, i b] from [x0, i x0] spans it step for step, so the
+ * iterate after k steps is the complex block GMRES iterate. Writes nitperf,
+ * resfin (FP64 true relative residual), timefct, timeitr, density, density2. */
+int bcm_abstract_solver_c(void* pgener, bcm_params* p, bcm_cb_nsup fnsup, bcm_cb_sprnds fsprnds,
+                          bcm_cb_iaja fiaja, bcm_cb_blockrow_c fblockrow, bcm_cb_rhs_c frhs,
+                          bcm_cb_iniguess_c finiguess, bcm_cb_storesol_c fstoresol);
 
 /* Callback memory for bcm_cb_iaja (operator new[] of nint ints, as new_struct in Bcmlib/utils.h:29-35). */
 int* bcm_alloc_ints(int nint);

@@ -24,6 +24,12 @@ REF_SO = os.path.join(HERE, "_ref", "libbslib_ref.so")
 
 _IP = C.POINTER(C.c_int)
 _DP = C.POINTER(C.c_double)
+# Callback types of the reference's abstract solvers (smatrix.h:14-24): CNSUP
+# and CSPRNDS fill ints, CIAJA returns two int arrays, the complex row /
+# right-hand side / guess / solution callbacks pass interleaved re/im doubles.
+CB_INTS = C.CFUNCTYPE(None, C.c_void_p, _IP)
+CB_IAJA = C.CFUNCTYPE(None, C.c_void_p, C.POINTER(_IP), C.POINTER(_IP))
+CB_CROW = C.CFUNCTYPE(None, C.c_void_p, C.c_int, _DP)
 _LP = C.POINTER(C.c_longlong)
 
 
@@ -100,7 +106,13 @@ def ref() -> C.CDLL:
         l.ref_find_separator.argtypes = [C.c_int, _IP, _IP, _IP, _IP]
         l.ref_cmvma.argtypes = mat + [_DP, _DP, C.c_int]
         l.ref_csolve.argtypes = [C.c_int] + mat + [_DP, _DP, C.c_int]
-        l.ref_cilu2.argtypes = mat + [C.c_double] * 6 + [C.c_char_p, C.c_int]
+        l.ref_cilu2.argtypes = mat + [C.c_double] * 6 + [C.c_int, C.c_char_p, C.c_int]
+        l.ref_zgesvd.argtypes = [C.c_int, _DP, _DP, _DP, _DP, _DP, C.c_int, _DP, _IP]
+        l.ref_alloc_ints.argtypes = [C.c_int]
+        l.ref_alloc_ints.restype = _IP
+        l.ref_abstract_solver_c.argtypes = ([C.c_void_p, C.c_int, C.c_int] + [C.c_double] * 6 + [C.c_int, C.c_double]
+                                            + [CB_INTS, CB_INTS, CB_IAJA, CB_CROW, CB_CROW, CB_CROW, CB_CROW]
+                                            + [C.c_char_p, C.c_int])
         l.ref_cfct_take.argtypes = [C.c_int, _IP, _IP, _IP, _IP, _DP]
         l.ref_ordmtr.argtypes = [C.c_int] + mat + [_IP, _IP, _IP, _IP, _IP, _DP]
         l.ref_combinelu.argtypes = [C.c_int] + mat + [_IP, _IP, _IP, _IP, _DP]
@@ -340,12 +352,12 @@ def ref_csolve(which, sprnd, ia, ja, a, x, nrhs=1):
     return y.reshape(np.asarray(x).shape, order="F")
 
 
-def ref_cilu2(sprnd, ia, ja, a, tau1=1e-2, tau2=1e-4, theta=0.1, sclmin=1e-8, pivmin=1e-8, memory=10.0):
+def ref_cilu2(sprnd, ia, ja, a, tau1=1e-2, tau2=1e-4, theta=0.1, sclmin=1e-8, pivmin=1e-8, memory=10.0, fcttyp=1):
     """The reference's CombineLU + complex CSMatrixCS::Ilu2 (ilu2cs.cpp):
     ((lia, lja, la), (uia, uja, ua)) with complex values, L by columns."""
     args, keep = _cargs(sprnd, ia, ja, a)
     msg = C.create_string_buffer(256)
-    if ref().ref_cilu2(*args, tau1, tau2, theta, sclmin, pivmin, memory, msg, 256):
+    if ref().ref_cilu2(*args, tau1, tau2, theta, sclmin, pivmin, memory, fcttyp, msg, 256):
         raise RefFactorError(msg.value.decode(errors="replace"))
     nsup = len(keep[0]) - 1
     out = []
@@ -359,6 +371,42 @@ def ref_cilu2(sprnd, ia, ja, a, tau1=1e-2, tau2=1e-4, theta=0.1, sclmin=1e-8, pi
                             fa.view(np.float64).ctypes.data_as(_DP))
         out.append((fia, fja, fa))
     return tuple(out)
+
+
+def ref_zgesvd(a):
+    """zgesvd_("A", "A") through lapackd.cpp -> lapackdm.f -> ZGESVD_SM: (s, U, VT), complex column-major."""
+    n = a.shape[0]
+    A = np.array(a, dtype=np.complex128, order="F", copy=True)
+    s = np.zeros(n)
+    U = np.zeros((n, n), np.complex128, order="F")
+    VT = np.zeros((n, n), np.complex128, order="F")
+    lw = 10 * n
+    work = np.zeros(lw, np.complex128)
+    rwork = np.zeros(lw)
+    info = C.c_int()
+    ref().ref_zgesvd(n, A.ctypes.data_as(_DP), s.ctypes.data_as(_DP), U.ctypes.data_as(_DP),
+                     VT.ctypes.data_as(_DP), work.ctypes.data_as(_DP), lw, rwork.ctypes.data_as(_DP), C.byref(info))
+    return s, U, VT, info.value
+
+
+def ref_alloc_ints(n):
+    """int[n] from the reference library's operator new[], for a CIAJA callback."""
+    return ref().ref_alloc_ints(n)
+
+
+def ref_abstract_solver_c(fnsup, fsprnds, fiaja, fblockrow, frhs, finiguess, fstoresol, ordtype=1, fcttyp=1,
+                          tau1=1e-2, tau2=1e-4, theta=0.1, sclmin=1e-8, pivmin=1e-8, memory=10.0, niter=1000,
+                          eps=1e-9):
+    """The reference's complex AbstractSolver (solvers.cpp:7103) on Python
+    callbacks with the arguments of CNSUP, CSPRNDS, CIAJA (arrays from
+    ref_alloc_ints), CBLOCKROWC, CRHSC, CINIGUESSC, CSTORESOLC (complex data
+    as interleaved doubles)."""
+    cbs = [CB_INTS(fnsup), CB_INTS(fsprnds), CB_IAJA(fiaja), CB_CROW(fblockrow), CB_CROW(frhs), CB_CROW(finiguess),
+           CB_CROW(fstoresol)]
+    msg = C.create_string_buffer(256)
+    if ref().ref_abstract_solver_c(None, ordtype, fcttyp, tau1, tau2, theta, sclmin, pivmin, memory, niter, eps,
+                                   *cbs, msg, 256):
+        raise RefFactorError(msg.value.decode(errors="replace"))
 
 
 def ref_find_separator(ia, ja):

@@ -285,12 +285,13 @@ void ref_csolve(int which, int nsup, const int* sprnd, const int* ia, const int*
 // CombineLU + CSMatrixCS::Ilu2 (ilu2cs.cpp) of a complex matrix, the factors
 // kept for ref_cfct_take. Returns 0, or 1 when the reference threw.
 int ref_cilu2(int nsup, const int* sprnd, const int* ia, const int* ja, const double* a, double tau1, double tau2,
-              double theta, double sclmin, double pivmin, double memory, char* msg, int msglen) {
+              double theta, double sclmin, double pivmin, double memory, int fcttyp, char* msg, int msglen) {
   CSMatrixCS am = make_cs(nsup, sprnd, ia, ja, a);
   CSMatrixCS al, au, dummy;
   am.CombineLU(al, au);
   CSlvParam p;
   set_fct_params(p, tau1, tau2, theta, sclmin, pivmin, memory);
+  p.fcttyp = fcttyp;
   std::ofstream fout("/dev/null");
   try {
     dummy.Ilu2(fout, p, al, au, g_cfct[0], g_cfct[1]);
@@ -304,6 +305,43 @@ int ref_cilu2(int nsup, const int* sprnd, const int* ia, const int* ja, const do
 
 void ref_cfct_take(int which, int* nzja, int* nza, int* oia, int* oja, double* oa) {
   take_cs(g_cfct[which], nzja, nza, oia, oja, oa);
+}
+
+// zgesvd_ as the complex ILU2 calls it (ilu2cs.cpp:132, 725): through
+// lapackd.cpp:944 -> lapackdm.f -> ZGESVD_SM, values interleaved re/im.
+void ref_zgesvd(int n, double* a, double* s, double* u, double* vt, double* work, int lwork, double* rwork,
+                int* info) {
+  zgesvd_("A", "A", &n, &n, reinterpret_cast<dcmplx*>(a), &n, s, reinterpret_cast<dcmplx*>(u), &n,
+          reinterpret_cast<dcmplx*>(vt), &n, reinterpret_cast<dcmplx*>(work), &lwork, rwork, info);
+}
+
+// Index arrays for a CIAJA callback: the reference releases them with delete[]
+// (solvers.cpp:7388-7389).
+int* ref_alloc_ints(int n) { return new int[n > 0 ? n : 1]; }
+
+// The reference's complex AbstractSolver (solvers.cpp:7103-7398) on the
+// caller's callbacks: assembly, CSVectorC::Solver (solvers.cpp:2588-2830:
+// ordering by ordtype, CombineLU + Ilu2, block GMRES, inverse ordering) and
+// CSTORESOLC. Returns 0, or 1 when the reference threw.
+int ref_abstract_solver_c(void* obj, int ordtype, int fcttyp, double tau1, double tau2, double theta, double sclmin,
+                          double pivmin, double memory, int niter, double eps, CNSUP fnsup, CSPRNDS fsprnds,
+                          CIAJA fiaja, CBLOCKROWC fblockrow, CRHSC frhs, CINIGUESSC finiguess,
+                          CSTORESOLC fstoresol, char* msg, int msglen) {
+  CSlvParam p;
+  set_fct_params(p, tau1, tau2, theta, sclmin, pivmin, memory);
+  p.ordtype = ordtype;
+  p.fcttyp = fcttyp;
+  p.niter = niter;
+  p.eps = eps;
+  std::ofstream fout("/dev/null");
+  try {
+    AbstractSolver(obj, fout, p, fnsup, fsprnds, fiaja, fblockrow, frhs, finiguess, fstoresol);
+  } catch (const char* e) {
+    std::strncpy(msg, e, msglen - 1);
+    msg[msglen - 1] = 0;
+    return 1;
+  }
+  return 0;
 }
 
 // dsyev_ / dgesvd_ as the factorizations call them (lapackd.cpp:865-905), for
