@@ -316,7 +316,10 @@ int run_complex(bcm_params* p, const CCallbacks& cb) {
     };
     double bnorm2 = 0.0;
     for (double v : bz) bnorm2 += v * v;
-    if (bnorm2 == 0.0) {  // no nonzero right-hand side: the guess is stored as it came (solvers.cpp:2725-2790)
+    // no nonzero right-hand side: the guess is stored as it came, the intent
+    // of the reference's column list (solvers.cpp:2725-2790), whose solver
+    // then ends the process in the closing ZGESVD of an empty system
+    if (bnorm2 == 0.0) {
       prm.nitperf = 0;
       prm.resfin = 0.0;
       store();

@@ -434,8 +434,11 @@ int bcm_abstract_par_solver(void* pgener, void* comm, bcm_params* p, bcm_cb_nsup
  * CombineLU + complex ILU2 (bcm_cfactor_ilu2), then block GMRES
  * (CSVectorC::BlockGMRES, gmres.cpp:1044-1466: one cycle of niter steps,
  * stopped when ||r|| < eps ||r0||) on the GPU, and the solution through
- * fstoresol. A zero right-hand side stores the initial guess unchanged, as
- * the reference's solver does (it solves only nonzero columns). The
+ * fstoresol. A zero right-hand side stores the initial guess unchanged and
+ * returns BCM_OK: the reference's solver skips zero columns
+ * (solvers.cpp:2725-2790), but its closing diagnostic ZGESVD of the empty
+ * system (gmres.cpp:1405) then fails its LWORK check and LAPACK's XERBLA
+ * stops the process. The
  * complex Krylov space over the real equivalent: the real block GMRES of
  * the columns [b, i b] from [x0, i x0] spans it step for step, so the
  * iterate after k steps is the complex block GMRES iterate. Writes nitperf,
