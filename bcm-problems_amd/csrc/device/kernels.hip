@@ -1594,6 +1594,7 @@ static void spmv_nr(const DevBsr& A, const double* x, double* y, const double* b
     double* dd = done == 0 ? dot_out : nullptr;
     if (left >= 6) { spmv_prec<B, 6>(A, x + o, y + o, bs, dd, wdot, rw, s, skip, gate); done += 6; }
     else if (left >= 3) { spmv_prec<B, 3>(A, x + o, y + o, bs, dd, wdot, rw, s, skip, gate); done += 3; }
+    else if (left == 2) { spmv_prec<B, 2>(A, x + o, y + o, bs, dd, wdot, rw, s, skip, gate); done += 2; }
     else { spmv_prec<B, 1>(A, x + o, y + o, bs, dd, wdot, rw, s, skip, RowGate()); done += 1; }
   }
 }
@@ -1686,6 +1687,7 @@ static void sweep_nr(const DevTri& T, const void* x, void* y, int vec_prec, int 
     char* ys = static_cast<char*>(y) + (size_t)done * T.n * es;
     if (left >= 6) { sweep_prec<B, 6>(T, xs, ys, vec_prec, s, skip, sync, gate); done += 6; }
     else if (left >= 3) { sweep_prec<B, 3>(T, xs, ys, vec_prec, s, skip, sync, gate); done += 3; }
+    else if (left == 2) { sweep_prec<B, 2>(T, xs, ys, vec_prec, s, skip, sync, gate); done += 2; }
     else { sweep_prec<B, 1>(T, xs, ys, vec_prec, s, skip, sync, gate); done += 1; }
   }
 }

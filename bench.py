@@ -39,7 +39,9 @@ cells, block 24, ILU2 + BiCGStab); `--workload precision` the FP64
 FGMRES(30) around FP64 / FP32 / BF16 factor sweeps on the same system;
 `--workload batch` one batched solve of `--group` cells of uneven sizes
 8^3..24^3; `--workload heat` the CPU-runnable 2D heat system (64 x 64
-cells, block 9, IC2 + PCG) against the reference's own Cg.
+cells, block 9, IC2 + PCG) against the reference's own Cg; `--workload
+complex` the complex solve phase (32^3 cells, complex blocks of 12, complex
+ILU2 + block GMRES) against the reference's own complex BlockGMRES.
 """
 
 import argparse
@@ -635,6 +637,103 @@ def run_dropin(args, B):
     }), flush=True)
 
 
+def run_complex(args, B):
+    """The complex solve phase (the complex AbstractSolver's CSVectorC::Solver,
+    solvers.cpp:2588-2830) on a 3D complex system: 32^3 cells, 7-point,
+    complex blocks of 12 (real part one seeded seepage matrix, imaginary part
+    0.4 x another of the same pattern), N = 393,216 complex unknowns, RCM +
+    complex ILU2(1e-2, 1e-4) on the host (bit-exact with the reference's),
+    block GMRES (gmres.cpp:1044) to ||r|| < 1e-10 ||r0|| on the GPU: the real
+    block GMRES over the real-equivalent columns [b, i b] (blocks of 24). The
+    CPU baseline is the reference's own complex BlockGMRES on one core with
+    the reference's own factors, which are also compared bit for bit with
+    this library's."""
+    import oracle as O
+
+    seed = 20261207
+    t0 = time.perf_counter()
+    m, _ = B.gen_seepage3d(32, 32, 32, 12, 0.3, seed)
+    m2, _ = B.gen_seepage3d(32, 32, 32, 12, 0.5, seed + 4)
+    order = m.rcm_order()
+    sp, ia, ja, a = m.reorder(order).arrays()
+    ac = a + 0.4j * m2.reorder(order).arrays()[3]
+    del m, m2, a
+    n = int(sp[-1])
+    rng = np.random.default_rng(seed)
+    bz = rng.standard_normal(n) + 1j * rng.standard_normal(n)
+    params = B.default_params()
+    f = B.Factor.ilu2_complex(sp, ia, ja, ac, params)
+    fct_s = time.perf_counter() - t0
+    mc = B.Matrix.from_complex(sp, ia, ja, ac)
+    log(f"complex system n={n} (real equivalent {2 * n}), complex ILU2 {params.timefct:.1f}s")
+    bb = np.empty((2 * n, 2), order="F")
+    bb[:, 0] = B.complex_to_real(sp, bz)
+    bb[:, 1] = B.complex_to_real(sp, 1j * bz)
+    bnorm = float(np.linalg.norm(bz))
+    ctx = B.Context()
+    db, dx = B.DeviceArray.from_host(bb), B.DeviceArray((2 * n, 2))
+    opts = B.default_opts(kind=B.BLOCKGMRES, eps=EPS, ittype=0, niter=400)
+    for _ in range(args.warmup):
+        dx.zero()
+        ctx.solve_block(mc, f, db, dx, 2, opts)
+    ctx.sync()
+    iters, relres, ok = [], [], True
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        dx.zero()
+        st, resi, _ = ctx.solve_block(mc, f, db, dx, 2, opts)
+        iters.append(st.iterations)
+        relres.append(float(resi[0]) / bnorm)
+        ok = ok and st.converged == 1
+    ctx.sync()
+    dt = time.perf_counter() - t0
+    x = B.real_to_complex(sp, dx.host()[:, 0])
+    ok = ok and max(relres) <= 2 * EPS
+    # one more solve with per-kernel HIP events (outside the timed region),
+    # polled every step so that no launch is queued past convergence
+    ctx.profile(True)
+    ctx.profile_read(reset=True)
+    dx.zero()
+    ctx.solve_block(mc, f, db, dx, 2, B.default_opts(kind=B.BLOCKGMRES, eps=EPS, ittype=0, niter=400, check_every=1))
+    ctx.sync()
+    prof = ctx.profile_read()
+    ctx.profile(False)
+    roof, kernels = roofline_from(prof)
+    cpu, same_factor, rel = None, None, None
+    if not args.no_cpu and O.ref_available():
+        L, U = O.ref_cilu2(sp, ia, ja, ac, memory=10.0)
+        fr = B.Factor.from_complex(sp, L, U)
+        nsup = len(sp) - 1
+        same_factor = all(np.array_equal(g, w) for which in (0, 1)
+                          for g, w in zip(f.arrays(which, nsup), fr.arrays(which, nsup)))
+        del fr, L, U
+        steps = int(np.median(iters)) + 1
+        probe = min(steps, 4)
+        _, sec = O.ref_cblock_gmres(sp, ia, ja, ac, bz, niter=probe, eps=0.0)
+        if sec / probe * steps <= 60.0:  # the whole solve fits the CPU budget: time it and compare
+            xr, sec = O.ref_cblock_gmres(sp, ia, ja, ac, bz, niter=opts.niter, eps=EPS)
+            rel = float(np.linalg.norm(x - xr) / np.linalg.norm(xr))
+            sample = f"one whole solve ({steps} block GMRES steps)"
+            value = 1.0 / sec
+        else:
+            sample = f"{probe} block GMRES steps, scaled to the {steps} steps of one GPU solve"
+            value = probe / (sec * steps)
+        cpu = dict(value=value, unit="solves/s", cores=1, kind="reference",
+                   sample=f"the reference's own complex BlockGMRES (gmres.cpp:1044, compiled from its sources) with "
+                          f"its own complex ILU2 factors on one core of {cpu_model()}: {sample}")
+    print(json.dumps({
+        "metric": METRIC, "value": args.steps / dt, "unit": "solves/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": dt / args.steps * 1e3, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f64", "data": "synthetic (seeded generator)",
+        "config": {"workload": f"complex solve phase: 3D nonsymmetric 32^3 cells, complex block 12, 7-point, "
+                               f"N={n} complex, RCM + complex ILU2(1e-2, 1e-4) + block GMRES over the real "
+                               f"equivalent [b, i b] to ||r|| < {EPS} ||r0||", "seed": seed},
+        "valid": bool(ok), "iterations": iters, "relres_true_max": max(relres), "factor_seconds_host": fct_s,
+        "factor_bitwise_equal_reference": same_factor, "rel_diff_vs_reference": rel,
+        "kernels": kernels, "roofline": roof, "cpu_baseline": cpu,
+    }), flush=True)
+
+
 def run_precision(args, B):
     """FP64 flexible GMRES(30) around FP64, FP32 and BF16 ILU2 factor sweeps on
     the 32^3 nonsymmetric system; every variant driven to FP64 true relative
@@ -899,7 +998,8 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3, help="timed evaluation groups (learned) or solves (drop-in)")
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", choices=["learned", "drop-in", "precision", "batch", "heat"], default="learned")
+    ap.add_argument("--workload", choices=["learned", "drop-in", "precision", "batch", "heat", "complex"],
+                    default="learned")
     ap.add_argument("--train-cells", type=int, default=1024)
     ap.add_argument("--fit-steps", type=int, default=16000)
     ap.add_argument("--fit-lr", type=float, default=5e-3)
@@ -934,6 +1034,8 @@ def main():
         run_batch(args, B)
     elif args.workload == "heat":
         run_heat(args, B)
+    elif args.workload == "complex":
+        run_complex(args, B)
     else:
         run_dropin(args, B)
     # release what the library keeps for the process before the runtime's teardown

@@ -109,6 +109,8 @@ def ref() -> C.CDLL:
         l.ref_cilu2.argtypes = mat + [C.c_double] * 6 + [C.c_int, C.c_char_p, C.c_int]
         l.ref_zgesvd.argtypes = [C.c_int, _DP, _DP, _DP, _DP, _DP, C.c_int, _DP, _IP]
         l.ref_alloc_ints.argtypes = [C.c_int]
+        l.ref_cblock_gmres.argtypes = mat + [_DP, _DP, C.c_int, C.c_double]
+        l.ref_cblock_gmres.restype = C.c_double
         l.ref_alloc_ints.restype = _IP
         l.ref_abstract_solver_c.argtypes = ([C.c_void_p, C.c_int, C.c_int] + [C.c_double] * 6 + [C.c_int, C.c_double]
                                             + [CB_INTS, CB_INTS, CB_IAJA, CB_CROW, CB_CROW, CB_CROW, CB_CROW]
@@ -387,6 +389,17 @@ def ref_zgesvd(a):
     ref().ref_zgesvd(n, A.ctypes.data_as(_DP), s.ctypes.data_as(_DP), U.ctypes.data_as(_DP),
                      VT.ctypes.data_as(_DP), work.ctypes.data_as(_DP), lw, rwork.ctypes.data_as(_DP), C.byref(info))
     return s, U, VT, info.value
+
+
+def ref_cblock_gmres(sprnd, ia, ja, a, b, x0=None, niter=100, eps=0.0):
+    """The reference's complex CSVectorC::BlockGMRES (gmres.cpp:1044) of one
+    right-hand side with the factors of the last ref_cilu2 call: (x, host
+    seconds of the call)."""
+    args, keep = _cargs(sprnd, ia, ja, a)
+    bv = np.ascontiguousarray(b, np.complex128)
+    x = np.zeros_like(bv) if x0 is None else np.array(x0, np.complex128)
+    sec = ref().ref_cblock_gmres(*args, bv.ctypes.data_as(_DP), x.ctypes.data_as(_DP), niter, eps)
+    return x, sec
 
 
 def ref_alloc_ints(n):

@@ -8,6 +8,7 @@
 // that would need MPI or METIS are never called. The IC2 / ILU2 entries run
 // the reference's Ich2 / Ilu2 over its own LAPACK 3.0 (lapacksm/*.f and
 // Bslib/lapackdm.f, compiled with flang into the same library).
+#include <chrono>
 #include <cstring>
 #include <fstream>
 
@@ -305,6 +306,30 @@ int ref_cilu2(int nsup, const int* sprnd, const int* ia, const int* ja, const do
 
 void ref_cfct_take(int which, int* nzja, int* nza, int* oia, int* oja, double* oa) {
   take_cs(g_cfct[which], nzja, nza, oia, oja, oa);
+}
+
+// The reference's complex BlockGMRES (gmres.cpp:1044-1466) of one
+// right-hand side with the factors of the last ref_cilu2 call: niter steps
+// at most, stopped at ||r|| < eps ||r0||; x holds x0 on entry and the
+// solution on return. Returns the seconds the call took (host clock).
+double ref_cblock_gmres(int nsup, const int* sprnd, const int* ia, const int* ja, const double* a, const double* b,
+                        double* x, int niter, double eps) {
+  CSMatrixCS am = make_cs(nsup, sprnd, ia, ja, a);
+  const int n = sprnd[nsup];
+  CSVectorC vb(n, 1, 1), vx(n, 1, 1), sol(n, 1, 1);
+  std::memcpy(vb.GetVect(), b, sizeof(double) * 2 * (size_t)n);
+  std::memcpy(vx.GetVect(), x, sizeof(double) * 2 * (size_t)n);
+  CSlvParam p;
+  p.msglev = 0;
+  p.niter = niter;
+  p.eps = eps;
+  p.ichk = niter + 1;
+  std::ofstream fout("/dev/null");
+  const auto t0 = std::chrono::steady_clock::now();
+  sol.BlockGMRES(fout, &am, &g_cfct[0], &g_cfct[1], vx, vb, p);
+  const double sec = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+  std::memcpy(x, sol.GetVect(), sizeof(double) * 2 * (size_t)n);
+  return sec;
 }
 
 // zgesvd_ as the complex ILU2 calls it (ilu2cs.cpp:132, 725): through

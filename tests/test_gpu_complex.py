@@ -32,7 +32,7 @@ def dev(B, x):
     return B.DeviceArray.from_host(np.asfortranarray(x))
 
 
-@pytest.mark.parametrize("nrhs", [1, 3])
+@pytest.mark.parametrize("nrhs", [1, 2, 3])
 def test_complex_operators_match_reference(B, O, ctx, csys, nrhs):
     s = csys
     sp = s["sp"]

@@ -26,7 +26,7 @@ def dev(B, x):
 
 
 @pytest.mark.parametrize("bla", [3, 6, 8, 12, 16, 24, 32])
-@pytest.mark.parametrize("nrhs", [1, 3, 7])
+@pytest.mark.parametrize("nrhs", [1, 2, 3, 5, 7])
 def test_spmv_and_transpose(B, O, ctx, bla, nrhs):
     s = seepage(B, O, 4, 3, 3, bla, seed=bla)
     A = s["A"]
@@ -79,7 +79,7 @@ def ragged_matrix(B, O, nsup, bla, maxblk, seed):
 
 
 @pytest.mark.parametrize("bla", [12, 16, 24])
-@pytest.mark.parametrize("nrhs", [3, 6])
+@pytest.mark.parametrize("nrhs", [2, 3, 6])
 def test_spmv_long_ragged_rows(B, O, ctx, bla, nrhs):
     """Rows of up to 22 blocks: several LDS staging chunks per row, rows
     shorter than the matrix prefetch, a diagonal-only row."""
@@ -95,7 +95,7 @@ def test_spmv_long_ragged_rows(B, O, ctx, bla, nrhs):
     assert rel(dy.host(), O.mvmat(A, x, nrhs)) < 1e-13
 
 
-@pytest.mark.parametrize("nrhs", [1, 6])
+@pytest.mark.parametrize("nrhs", [1, 2, 6])
 def test_sweeps_long_ragged_rows(B, O, ctx, nrhs):
     """ILU2 factors of a matrix with rows of up to 16 blocks: factor rows that
     span several dependency-staging rounds of the level sweep."""
@@ -114,7 +114,7 @@ def test_sweeps_long_ragged_rows(B, O, ctx, nrhs):
     assert rel(dz.host(), O.solveu(U, y, nrhs)) < 1e-12
 
 
-@pytest.mark.parametrize("nrhs", [1, 6])
+@pytest.mark.parametrize("nrhs", [1, 2, 6])
 def test_sweeps_ic2(B, O, ctx, nrhs):
     h = heat(B, O, 9, 8, 9, seed=5)
     U = h["U"]

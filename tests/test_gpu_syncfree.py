@@ -23,7 +23,7 @@ def dev(B, x):
     return B.DeviceArray.from_host(np.asfortranarray(x))
 
 
-@pytest.mark.parametrize("nrhs", [1, 6])
+@pytest.mark.parametrize("nrhs", [1, 2, 5, 6])
 def test_syncfree_ic2_sweeps(B, O, fctx, nrhs):
     h = heat(B, O, 9, 8, 9, seed=5)
     U = h["U"]
