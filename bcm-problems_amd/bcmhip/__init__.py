@@ -723,12 +723,16 @@ class Batch:
         u = np.asarray(union).reshape(self.n, -1, order="F")
         return [u[self.elem0[c]:self.elem0[c + 1], :] for c in range(self.ncell)]
 
-    def solve(self, ctx: "Context", b, x, opts: SolveOpts, nrhs: int, fallback: bool = False):
+    def solve(self, ctx: "Context", b, x, opts: SolveOpts, nrhs: int, fallback: bool = False,
+              galerkin: bool = False):
+        """bcm_batch_solve; galerkin: start from each cell's A-optimal
+        combination of its nrhs guesses."""
         npair = self.ncell * nrhs
         iters = np.zeros(npair, np.int32)
         relres = np.zeros(npair)
         st = BatchStats()
-        _check(lib().bcm_batch_solve(ctx.h, self.h, C.byref(opts), nrhs, ptr(b), ptr(x), int(fallback), C.byref(st),
+        flags = (1 if fallback else 0) | (2 if galerkin else 0)
+        _check(lib().bcm_batch_solve(ctx.h, self.h, C.byref(opts), nrhs, ptr(b), ptr(x), flags, C.byref(st),
                                      _ip(iters), _dp(relres)))
         return st, iters.reshape(self.ncell, nrhs), relres.reshape(self.ncell, nrhs)
 

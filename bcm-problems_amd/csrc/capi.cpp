@@ -920,7 +920,8 @@ int bcm_batch_solve(bcm_context c, bcm_batch b, const bcm_solve_opts* o, int nrh
     bcm::SolveOptions opt = to_opts(o);
     if (opt.kind != bcm::kPcg) opt.kind = bcm::kBlockPcg;
     bcm::BatchStats s;
-    bcm::batch_solve(b->b, c->c, opt, nrhs, rhs, x, fallback != 0, s, iters, relres);
+    need(fallback >= 0 && fallback <= 3, "batch_solve: start flags are bits 0 (fallback) and 1 (Galerkin)");
+    bcm::batch_solve(b->b, c->c, opt, nrhs, rhs, x, (fallback & 1) != 0, s, iters, relres, (fallback & 2) != 0);
     if (st) {
       st->pairs = s.pairs;
       st->converged = s.converged;

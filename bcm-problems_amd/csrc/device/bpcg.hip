@@ -169,6 +169,151 @@ __global__ __launch_bounds__(256) void k_bp_zero(int64_t n, const int* __restric
   }
 }
 
+// Block Galerkin start (no reference counterpart). Per cell, the NR guesses
+// X (rows of the cell x NR) and their products A X = B - R, known from the
+// initial residual R = B - A X, give G = X^T A X and F = X^T B; the guesses
+// are replaced by X C with G C = F, column q of X C being the A-norm closest
+// point of span(X) to the cell's solution of right-hand side q -- no farther
+// from it in the A-norm than its own guess, nor than x0 = 0 -- and R by
+// B - (B - R) C. This pass writes, per chunk, the partial sums of G (values
+// 0 .. NR^2-1, row-major) and F (NR^2 .. 2NR^2-1).
+template <int NR>
+__global__ __launch_bounds__(256) void k_bp_gram(int64_t n, const int* __restrict__ lo, const int* __restrict__ hi,
+                                                 const double* __restrict__ x, const double* __restrict__ b,
+                                                 const double* __restrict__ r, double* partials) {
+  constexpr int M = NR * NR;
+  double acc[2 * M];
+#pragma unroll
+  for (int j = 0; j < 2 * M; ++j) acc[j] = 0.0;
+  const int a = lo[blockIdx.x], e = hi[blockIdx.x];
+  for (int i = a + threadIdx.x; i < e; i += blockDim.x) {
+    double xv[NR], bv[NR], av[NR];
+#pragma unroll
+    for (int q = 0; q < NR; ++q) {
+      const size_t k = (size_t)q * n + i;
+      xv[q] = x[k];
+      bv[q] = b[k];
+      av[q] = bv[q] - r[k];
+    }
+#pragma unroll
+    for (int p = 0; p < NR; ++p)
+#pragma unroll
+      for (int q = 0; q < NR; ++q) {
+        acc[p * NR + q] += xv[p] * av[q];
+        acc[M + p * NR + q] += xv[p] * bv[q];
+      }
+  }
+  block_partials<2 * M>(acc, partials);
+}
+
+// One thread per cell: G and F from the cell's chunks, G symmetrized and
+// factored by Cholesky, C = G^-1 F. A pivot not above 1e-12 of G's largest
+// diagonal entry (guesses dependent to rounding, or zero) or a non-finite
+// coefficient keeps the cell's guesses as they are (C = I).
+template <int NR>
+__global__ void k_bp_galerkin(int ncell, const int* __restrict__ cell_chunk0, const double* __restrict__ partials,
+                              double* __restrict__ coef) {
+  constexpr int M = NR * NR;
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= ncell) return;
+  double g[NR][NR], f[NR][NR];
+#pragma unroll
+  for (int p = 0; p < NR; ++p)
+#pragma unroll
+    for (int q = 0; q < NR; ++q) {
+      double sg = 0.0, sf = 0.0;
+      for (int ch = cell_chunk0[c]; ch < cell_chunk0[c + 1]; ++ch) {
+        sg += partials[(size_t)ch * 2 * M + p * NR + q];
+        sf += partials[(size_t)ch * 2 * M + M + p * NR + q];
+      }
+      g[p][q] = sg;
+      f[p][q] = sf;
+    }
+  double dmax = 0.0;
+#pragma unroll
+  for (int p = 0; p < NR; ++p) dmax = fmax(dmax, g[p][p]);
+  bool ok = dmax > 0.0;
+  double l[NR][NR];
+#pragma unroll
+  for (int j = 0; j < NR; ++j) {
+    double d = g[j][j];
+#pragma unroll
+    for (int k = 0; k < j; ++k) d -= l[j][k] * l[j][k];
+    ok = ok && d > 1e-12 * dmax;
+    const double ljj = ok ? sqrt(d) : 1.0;
+    l[j][j] = ljj;
+#pragma unroll
+    for (int i = j + 1; i < NR; ++i) {
+      double t = 0.5 * (g[i][j] + g[j][i]);
+#pragma unroll
+      for (int k = 0; k < j; ++k) t -= l[i][k] * l[j][k];
+      l[i][j] = t / ljj;
+    }
+  }
+  double cq[NR][NR];
+#pragma unroll
+  for (int q = 0; q < NR; ++q) {
+    double y[NR];
+#pragma unroll
+    for (int i = 0; i < NR; ++i) {  // L y = f(:, q)
+      double t = f[i][q];
+#pragma unroll
+      for (int k = 0; k < i; ++k) t -= l[i][k] * y[k];
+      y[i] = t / l[i][i];
+    }
+#pragma unroll
+    for (int i = NR - 1; i >= 0; --i) {  // L^T c = y
+      double t = y[i];
+#pragma unroll
+      for (int k = i + 1; k < NR; ++k) t -= l[k][i] * cq[k][q];
+      cq[i][q] = t / l[i][i];
+    }
+  }
+#pragma unroll
+  for (int p = 0; p < NR; ++p)
+#pragma unroll
+    for (int q = 0; q < NR; ++q) ok = ok && isfinite(cq[p][q]);
+#pragma unroll
+  for (int p = 0; p < NR; ++p)
+#pragma unroll
+    for (int q = 0; q < NR; ++q) coef[(size_t)c * M + p * NR + q] = ok ? cq[p][q] : (p == q ? 1.0 : 0.0);
+}
+
+// x <- X C and r <- B - (B - R) C over the rows of one chunk.
+template <int NR>
+__global__ __launch_bounds__(256) void k_bp_combine(int64_t n, const int* __restrict__ ccell,
+                                                    const int* __restrict__ lo, const int* __restrict__ hi,
+                                                    const double* __restrict__ coef, const double* __restrict__ b,
+                                                    double* __restrict__ x, double* __restrict__ r) {
+  constexpr int M = NR * NR;
+  __shared__ double cs[M];
+  const int c = ccell[blockIdx.x], a = lo[blockIdx.x], e = hi[blockIdx.x];
+  for (int k = threadIdx.x; k < M; k += blockDim.x) cs[k] = coef[(size_t)c * M + k];
+  __syncthreads();
+  for (int i = a + threadIdx.x; i < e; i += blockDim.x) {
+    double xv[NR], av[NR], bv[NR];
+#pragma unroll
+    for (int q = 0; q < NR; ++q) {
+      const size_t k = (size_t)q * n + i;
+      xv[q] = x[k];
+      bv[q] = b[k];
+      av[q] = bv[q] - r[k];
+    }
+#pragma unroll
+    for (int q = 0; q < NR; ++q) {
+      double xn = 0.0, an = 0.0;
+#pragma unroll
+      for (int p = 0; p < NR; ++p) {
+        xn += xv[p] * cs[p * NR + q];
+        an += av[p] * cs[p * NR + q];
+      }
+      const size_t k = (size_t)q * n + i;
+      x[k] = xn;
+      r[k] = bv[q] - an;
+    }
+  }
+}
+
 // Convergence test and beta. Partials hold (r.z, r.r[, r_old.z]) per pair.
 __global__ __launch_bounds__(kPairThreads) void k_bp_beta(int npair, int nrhs, const int* cell_chunk0,
                                                           const double* partials, int K, BPtrs s, double eps2,
@@ -559,7 +704,7 @@ namespace {
 
 template <int NR>
 void solve_nr(Batch& bt, Context& ctx, const SolveOptions& opt, const double* b, double* x, bool fallback,
-              BatchStats& st, int* iters_out, double* relres_out) {
+              bool galerkin, BatchStats& st, int* iters_out, double* relres_out) {
   const Factor::Image& im = bt.img[opt.prec];
   if (!im.has) throw Error(kErrArg, "batch: factor sweeps of the requested precision were not built");
   const int npair = bt.ncell * NR;
@@ -604,6 +749,19 @@ void solve_nr(Batch& bt, Context& ctx, const SolveOptions& opt, const double* b,
     hipLaunchKernelGGL(k_copy_vec, dim3(grid_vec(nn)), dim3(256), 0, s, nn, (const double*)x, w.x);
     hipLaunchKernelGGL(k_zero_vec, dim3(grid_vec(nn)), dim3(256), 0, s, nn, w.p);
     residual_dots<NR>(bt, ctx, w);
+    if (galerkin) {  // each cell's guesses replaced by their A-optimal combination, then r.r and b.b again
+      double* gp = ctx.vec(110, (size_t)bt.nchunks * 2 * NR * NR * 8 + 64);
+      double* coef = ctx.vec(111, (size_t)bt.ncell * NR * NR * 8 + 64);
+      hipLaunchKernelGGL(k_bp_gram<NR>, dim3(bt.nchunks), dim3(256), 0, s, (int64_t)bt.n, bt.chunk_lo.as<int>(),
+                         bt.chunk_hi.as<int>(), (const double*)w.x, (const double*)w.b, (const double*)w.r, gp);
+      hipLaunchKernelGGL(k_bp_galerkin<NR>, dim3((bt.ncell + 63) / 64), dim3(64), 0, s, bt.ncell,
+                         bt.dcell_chunk0.as<int>(), (const double*)gp, coef);
+      hipLaunchKernelGGL(k_bp_combine<NR>, dim3(bt.nchunks), dim3(256), 0, s, (int64_t)bt.n,
+                         bt.chunk_cell.as<int>(), bt.chunk_lo.as<int>(), bt.chunk_hi.as<int>(), (const double*)coef,
+                         (const double*)w.b, w.x, w.r);
+      const double* xs[2] = {w.r, w.b};
+      dots<NR, 2>(bt, ctx, w, xs, xs, nullptr);
+    }
     hipLaunchKernelGGL(k_bp_start, dim3(1), dim3(kPairThreads), 0, s, npair, NR, bt.dcell_chunk0.as<int>(),
                        w.partials, w.s, eps2, opt.ittype, fallback ? 1 : 0);
     if (fallback)
@@ -703,7 +861,7 @@ void solve_nr(Batch& bt, Context& ctx, const SolveOptions& opt, const double* b,
 }  // namespace
 
 void batch_solve(Batch& bt, Context& ctx, const SolveOptions& opt, int nrhs, const double* b, double* x,
-                 bool fallback, BatchStats& st, int* iters, double* relres) {
+                 bool fallback, BatchStats& st, int* iters, double* relres, bool galerkin) {
   auto t0 = std::chrono::steady_clock::now();
   st = BatchStats();
   if (opt.prec < kF64 || opt.prec > kBF16) throw Error(kErrArg, "batch: unknown precision");
@@ -712,9 +870,9 @@ void batch_solve(Batch& bt, Context& ctx, const SolveOptions& opt, int nrhs, con
   if (nrhs != 1 && nrhs != 3 && nrhs != 6) throw Error(kErrArg, "batch: nrhs must be 1, 3 or 6");
   auto run = [&] {
     switch (nrhs) {
-      case 1: solve_nr<1>(bt, ctx, opt, b, x, fallback, st, iters, relres); break;
-      case 3: solve_nr<3>(bt, ctx, opt, b, x, fallback, st, iters, relres); break;
-      default: solve_nr<6>(bt, ctx, opt, b, x, fallback, st, iters, relres); break;
+      case 1: solve_nr<1>(bt, ctx, opt, b, x, fallback, galerkin, st, iters, relres); break;
+      case 3: solve_nr<3>(bt, ctx, opt, b, x, fallback, galerkin, st, iters, relres); break;
+      default: solve_nr<6>(bt, ctx, opt, b, x, fallback, galerkin, st, iters, relres); break;
     }
     BCM_HIP(hipStreamSynchronize(ctx.stream));
   };

@@ -231,8 +231,11 @@ void batch_build(Batch& bt, const std::vector<const Matrix*>& mats, const std::v
 // length bt.n; x holds x0. With opt_fallback, a pair whose ||b - A x0|| >=
 // ||b|| restarts from x0 = 0. iters/relres (host, ncell * nrhs, cell-major)
 // may be null.
+// fallback: pairs whose initial residual is not below ||b|| restart from 0;
+// galerkin: first replace each cell's nrhs guesses by their A-optimal
+// combination (X C with X^T A X C = X^T B).
 void batch_solve(Batch& bt, Context& ctx, const SolveOptions& opt, int nrhs, const double* b, double* x,
-                 bool fallback, BatchStats& st, int* iters, double* relres);
+                 bool fallback, BatchStats& st, int* iters, double* relres, bool galerkin = false);
 // Times the candidate schedules on this batch (a few iterations each, on
 // scratch copies of b) and keeps the fastest in bt.tune.
 BatchTune batch_tune(Batch& bt, Context& ctx, const SolveOptions& opt, int nrhs, const double* b,

@@ -273,6 +273,7 @@ def run_learned(args, B):
 
     # evaluation groups
     timed_s, timed_pairs, its_learned, its_zero, relres_max, ok = 0.0, 0, [], [], 0.0, True
+    its_learned_max, its_zero_max = [], []
     prof = None
     plain_cells = []
     guess_err = None
@@ -285,13 +286,14 @@ def run_learned(args, B):
         ctx.sync()
         t0 = time.perf_counter()
         guess.predict(ctx, params[idx], dx)
-        st, its, rr = bt.solve(ctx, db, dx, opts, 6, True)
+        st, its, rr = bt.solve(ctx, db, dx, opts, 6, True, args.galerkin)
         dt = time.perf_counter() - t0
         timed = gi >= args.warmup
         if timed:
             timed_s += dt
             timed_pairs += st.pairs
             its_learned.append(float(its.mean()))
+            its_learned_max.append(int(its.max()))
             relres_max = max(relres_max, float(rr.max()))
             ok = ok and st.converged == st.pairs and float(rr.max()) <= EPS
         if gi == args.warmup:
@@ -300,13 +302,14 @@ def run_learned(args, B):
             dz.zero()
             stz, itz, _ = bt.solve(ctx, db, dz, opts, 6)
             its_zero.append(float(itz.mean()))
+            its_zero_max.append(int(itz.max()))
             ctx.profile(True)
             ctx.profile_read(reset=True)
             bt.set_tune(**dict(tune, graph=0))
             guess.predict(ctx, params[idx], dz)
             ctx.sync()
             x0h = dz.host()
-            bt.solve(ctx, db, dz, opts, 6, True)
+            bt.solve(ctx, db, dz, opts, 6, True, args.galerkin)
             ctx.sync()
             prof = ctx.profile_read()
             ctx.profile(False)
@@ -411,6 +414,7 @@ def run_learned(args, B):
                            "roofline": plain_roof, "kernels": plain_kernels},
         "speedup_vs_plain": value / plain_rate,
         "iterations": {"learned_mean": float(np.mean(its_learned)), "zero_guess_mean": float(np.mean(its_zero)),
+                       "learned_group_max": its_learned_max, "zero_guess_group_max": its_zero_max,
                        "training_mean": float(np.mean(train_iters))},
         "guess_error": guess_err,
         "end_to_end": {
@@ -1013,6 +1017,8 @@ def main():
     ap.add_argument("--group", type=int, default=None,
                     help=f"cells per batched solve (learned: {GROUP}, batch: 64)")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--galerkin", action="store_true",
+                    help="start every evaluation cell from the A-optimal combination of its six learned guesses")
     ap.add_argument("--no-graph", action="store_true", help="leave hipGraph replay out of the batch schedules")
     ap.add_argument("--tune", default=None,
                     help="fixed batch schedule 'sweep_mode,waves_per_cu,level_order,graph,check_every' instead of "

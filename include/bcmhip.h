@@ -337,8 +337,13 @@ int bcm_batch_layout(bcm_batch b, int* ncell, long long* n, long long* elem0);
 /* Block PCG for all pairs; rhs and x are device arrays of n x nrhs (column q
  * holds every cell's q-th right-hand side at the cell's offsets); x holds x0.
  * o->precision selects the factor sweeps (F32/BF16: flexible CG with an FP64
- * outer recurrence). fallback != 0: a pair with ||b - A x0|| >= ||b||
- * restarts from x0 = 0. iters/relres: host arrays of ncell * nrhs entries
+ * outer recurrence). fallback, bit 0 (1): a pair with ||b - A x0|| >= ||b||
+ * restarts from x0 = 0; bit 1 (2, no reference counterpart): first the nrhs
+ * guesses X of every cell are replaced by their A-optimal combination X C,
+ * (X^T A X) C = X^T B -- each column no farther from its solution in the
+ * A-norm than its own guess or than zero -- at the cost of three vector
+ * passes (a cell whose guesses are dependent to rounding keeps them); other
+ * values BCM_E_ARG. iters/relres: host arrays of ncell * nrhs entries
  * (cell-major) or NULL. nrhs is 1, 3 or 6. */
 int bcm_batch_solve(bcm_context c, bcm_batch b, const bcm_solve_opts* o, int nrhs, const double* rhs, double* x,
                     int fallback, bcm_batch_stats* st, int* iters, double* relres);

@@ -55,14 +55,14 @@ def oracle_sol(O, cells):
     return out
 
 
-def run(B, ctx, bt, cells, nrhs=6, x0=None, fallback=False, **opts):
+def run(B, ctx, bt, cells, nrhs=6, x0=None, fallback=False, galerkin=False, **opts):
     b = bt.gather([c["b"][:, :nrhs] for c in cells])
     db = B.DeviceArray.from_host(b)
     dx = B.DeviceArray.from_host(x0) if x0 is not None else B.DeviceArray(b.shape)
     if x0 is None:
         dx.zero()
     o = B.default_opts(kind=B.BLOCKPCG, **opts)
-    st, its, rr = bt.solve(ctx, db, dx, o, nrhs, fallback)
+    st, its, rr = bt.solve(ctx, db, dx, o, nrhs, fallback, galerkin)
     return bt.scatter(dx.host()), st, its, rr
 
 
@@ -215,3 +215,52 @@ def test_batch_autotune_and_errors(B, ctx, batch, cells):
     assert st.converged == st.pairs
     with pytest.raises(B.BcmError):
         batch.solve(ctx, db, dx, B.default_opts(kind=B.BLOCKPCG), 2)
+
+
+def test_batch_galerkin_start(B, ctx, batch, cells, oracle_sol):
+    """Start flag 2: each cell's six guesses X become X C with
+    (X^T A X) C = X^T B. With niter = 0 the solve returns the start: it equals
+    the dense computation per cell to 1e-9, and every column is no farther
+    from the cell's solution in the A-norm than its own guess or than zero.
+    A cell whose guesses are linearly dependent keeps them. Then full solves
+    from the combined start converge to the oracle's solutions."""
+    rng = np.random.default_rng(41)
+    guesses = []
+    for c, (xo, _) in enumerate(oracle_sol):
+        g = xo * (1.0 + 0.2 * rng.standard_normal(xo.shape[1]))[None, :] + 0.02 * rng.standard_normal(xo.shape)
+        if c == 1:
+            g = -50.0 * g  # far worse than zero
+        if c == 3:
+            g[:, 4] = g[:, 2]  # dependent guesses: the cell keeps them
+        guesses.append(g)
+    x0 = batch.gather(guesses)
+    print("batch galerkin start, niter 0", flush=True)
+    xs, st, its, _ = run(B, ctx, batch, cells, x0=x0, galerkin=True, eps=1e-10, niter=0)
+    for c, (cl, g, x) in enumerate(zip(cells, guesses, xs)):
+        A = cl["mo"].dense()
+        b = cl["b"]
+        xo = oracle_sol[c][0]
+        if c == 3:
+            assert np.array_equal(x, g), "a cell with dependent guesses keeps them"
+            continue
+        G = g.T @ A @ g
+        want = g @ np.linalg.solve(0.5 * (G + G.T), g.T @ b)
+        assert rel(x, want) < 1e-9, c
+        for q in range(6):
+            def anorm(v):
+                return float(np.sqrt(v @ A @ v))
+            e_new, e_own, e_zero = anorm(x[:, q] - xo[:, q]), anorm(g[:, q] - xo[:, q]), anorm(xo[:, q])
+            assert e_new <= min(e_own, e_zero) * (1 + 1e-10), (c, q, e_new, e_own, e_zero)
+    print("batch galerkin start, full solves", flush=True)
+    xs, st, its, rr = run(B, ctx, batch, cells, x0=x0, fallback=True, galerkin=True, eps=1e-10)
+    assert st.converged == st.pairs and rr.max() <= 1e-10
+    for x, (xo, _) in zip(xs, oracle_sol):
+        assert rel(x, xo) < 1e-8
+    import ctypes
+
+    b = batch.gather([c["b"] for c in cells])
+    db, dx = B.DeviceArray.from_host(b), B.DeviceArray.from_host(b)
+    bst, o = B.BatchStats(), B.default_opts(kind=B.BLOCKPCG)
+    code = B.lib().bcm_batch_solve(ctx.h, batch.h, ctypes.byref(o), 6, B.ptr(db), B.ptr(dx), 4, ctypes.byref(bst),
+                                   None, None)
+    assert code == B.E_ARG  # start flags are bits 0 and 1 only

@@ -28,10 +28,11 @@ def _cpp_blocks():
 
 def test_shim_blocks_are_present():
     blocks = _cpp_blocks()
-    assert len(blocks) >= 3
+    assert len(blocks) >= 4
     assert "AbstractSymmSolverGpu" in blocks[0] and "bcm_abstract_par_solver" in blocks[0]
-    assert "GpuMvm" in blocks[1] and "GpuSolveLU" in blocks[1]
-    assert "bcm_solve_block" in blocks[2]
+    assert "CBLOCKROWC" in blocks[1] and "bcm_abstract_solver_c" in blocks[1]
+    assert "GpuMvm" in blocks[2] and "GpuSolveLU" in blocks[2]
+    assert "bcm_solve_block" in blocks[3]
 
 
 def test_shim_compiles_against_reference_headers(tmp_path):
@@ -40,9 +41,9 @@ def test_shim_compiles_against_reference_headers(tmp_path):
     fragment = (
         "void BlockGmresGpu(bcm_context c, bcm_matrix a, bcm_factor f, CSVector& b, CSVector& x,\n"
         "                   CSlvParam& param, int n, int nrhs, double* db, double* dx) {\n"
-        + blocks[2] + "}\n")
+        + blocks[3] + "}\n")
     src = tmp_path / "bslib_gpu.cpp"
-    src.write_text(blocks[0] + "\n" + blocks[1] + "\n" + fragment)
+    src.write_text(blocks[0] + "\n" + blocks[1] + "\n" + blocks[2] + "\n" + fragment)
     cmd = ["g++", "-std=c++11", "-fsyntax-only", "-Wall", "-Wno-unused-function", "-Wno-unused-variable",
            f"-I{BSLIB}", f"-I{ROOT / 'include'}", str(src)]
     r = subprocess.run(cmd, capture_output=True, text=True, timeout=120)
