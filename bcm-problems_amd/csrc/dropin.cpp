@@ -133,13 +133,17 @@ void gather_blocks(const Callbacks& cb, const bcm::BlockCsr& m, bcm_cb_rhs f, st
   for (int i = 0; i < m.nsup; ++i) f(cb.obj, i, &v[m.sprnd[i]]);
 }
 
+// Device vector of the drop-in; its allocation and free wait for any stream
+// capture in progress on another thread (the library's set-up lock).
 struct DevVec {
   double* p = nullptr;
   explicit DevVec(size_t n) {
+    bcm::SetupGuard g;
     if (hipMalloc(&p, std::max<size_t>(n, 1) * sizeof(double)) != hipSuccess)
       throw bcm::Error(bcm::kErrHip, "hipMalloc failed");
   }
   ~DevVec() {
+    bcm::SetupGuard g;
     if (p) (void)hipFree(p);
   }
 };
