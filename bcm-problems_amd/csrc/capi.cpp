@@ -229,6 +229,7 @@ int bcm_cmatrix_create(int nsup, const int* sprnd, const int* ia, const int* ja,
     try {
       h->m.h = bcm::real_equivalent(nsup, sprnd, ia, ja, a, false);
       h->m.symmetric = false;
+      h->m.cplx = true;
       h->m.h.validate(false);
     } catch (...) {
       delete h;

@@ -84,10 +84,12 @@ struct DevBuf {
 struct DevBsr {
   int nsup = 0, n = 0, b = 0, nzja = 0;
   int prec = kF64;
+  int cplx = 0;  // 1: the real equivalent of a complex matrix, a holding its complex (b/2)^2 blocks
   DevBuf ia, ja, a;
   DevBuf sp, bo;
   int64_t nval = 0;
   int64_t value_bytes() const {
+    if (cplx) return (int64_t)nzja * b * b / 2 * 8;
     return (b ? (int64_t)nzja * b * b : nval) * (int64_t)prec_bytes(prec);
   }
   int64_t index_bytes() const { return 4LL * (nzja + nsup + 1) + (b ? 0 : 8LL * nzja + 4LL * (nsup + 1)); }
@@ -95,6 +97,12 @@ struct DevBsr {
 
 // Builds a device block-CSR from host storage (uniform block size required).
 void dev_bsr_upload(const BlockCsr& h, int prec, DevBsr& d);
+// The real equivalent h of a complex matrix (every block [[Re C, -Im C],
+// [Im C, Re C]]) stored on the device as its complex blocks C, (re, im)
+// pairs column-major -- half the bytes -- for the complex-block SpMV.
+// Returns false (d untouched) for block sizes without that kernel or blocks
+// without that structure.
+bool dev_bsr_upload_complex(const BlockCsr& h, DevBsr& d);
 // Block-diagonal union of several matrices (same block size), part after part.
 void dev_bsr_upload_parts(const std::vector<const BlockCsr*>& parts, int prec, DevBsr& d);
 // The same for upper-stored symmetric parts, expanded to full storage on the

@@ -375,6 +375,141 @@ __global__ __launch_bounds__(256) void k_spmv_mr(int nsup, int n, const int* __r
 }
 
 // ---------------------------------------------------------------------------
+// Complex-block SpMV on real-equivalent vectors (DevBsr::cplx): the matrix
+// holds complex BC x BC blocks as (re, im) pairs, column-major -- the layout
+// of CSMatrixCS (mvmc.cpp:160) -- and each vector column holds, per block
+// row, its BC real parts and then its BC imaginary parts, so
+//   y_re = sum (Re C x_re - Im C x_im),  y_im = sum (Im C x_re + Re C x_im)
+// at half the matrix bytes of the real-equivalent blocks. y = A x or
+// bsub - A x for NR columns, optional dot(wdot, y) of the first. Lanes,
+// staging and fold are those of k_spmv_mr with complex entries: L = BC/VC
+// lanes cover a complex column with VC entries (2*VC doubles, one vector
+// load) each, G = 64/L lane groups walk the row's complex columns, the x
+// values of a chunk of block columns (2*BC real values each) sit in LDS.
+template <int BC>
+constexpr int spmv_cx_width() {
+  return BC % 2 == 0 ? 2 : 1;
+}
+
+template <int BC, int NR>
+__global__ __launch_bounds__(256) void k_spmv_cx(int nsup, int n, const int* __restrict__ ia,
+                                                 const int* __restrict__ ja, const double* __restrict__ a,
+                                                 const double* __restrict__ x, double* __restrict__ y,
+                                                 const double* __restrict__ bsub, const double* __restrict__ wdot,
+                                                 double* partials, int rows_per_wave, const int* skip) {
+  if (skip && *skip) return;
+  constexpr int B = 2 * BC;  // real rows of a block row
+  constexpr int VC = spmv_cx_width<BC>();
+  constexpr int L = BC / VC;
+  constexpr int G = 64 / L;
+  constexpr int NRP = NR % 2 ? NR + 1 : NR;
+  constexpr int KB = spmv_mr_chunk<B, NRP>();
+  constexpr int WL0 = KB * B * NRP;
+  constexpr int WL = WL0 > G * B ? WL0 : G * B;
+  constexpr int QF = WL / (G * B) < NR ? WL / (G * B) : NR;
+  static_assert(QF >= 1, "the fold needs room for one right-hand side");
+  constexpr int D = 2;
+  __shared__ __align__(16) double lds[4][WL];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int lr = lane % L, g = lane / L;
+  const bool act = g < G;
+  double* xs = lds[wave];
+  const int tile = xcd_tile(blockIdx.x, gridDim.x);
+  const int wave_global = __builtin_amdgcn_readfirstlane(tile * (blockDim.x >> 6) + wave);
+  double dotv = 0.0;
+  for (int q0 = 0; q0 < rows_per_wave; ++q0) {
+    const int row = wave_global * rows_per_wave + q0;
+    if (row >= nsup) break;
+    double are[NR][VC], aim[NR][VC];
+#pragma unroll
+    for (int q = 0; q < NR; ++q)
+#pragma unroll
+      for (int k = 0; k < VC; ++k) {
+        are[q][k] = 0.0;
+        aim[q][k] = 0.0;
+      }
+    const int jb = ia[row], nblk = ia[row + 1] - jb;
+    const double* ap = a + (size_t)jb * BC * BC * 2 + lr * VC * 2;
+    int t = g;  // this lane group's next complex column of the row's stream
+    for (int kb0 = 0; kb0 < nblk; kb0 += KB) {
+      const int nb = min(KB, nblk - kb0);
+      const int t0 = kb0 * BC, t1 = (kb0 + nb) * BC;
+      double av[D][2 * VC];
+      if (act) {
+#pragma unroll
+        for (int d = 0; d < D; ++d)
+          if (t + d * G < t1) load_vec_cached<2 * VC>(ap + (size_t)(t + d * G) * BC * 2, av[d]);
+      }
+      if constexpr (NR * B >= 64)
+        stage_cols<B, NR, NRP, KB>(lane, nb, ja + jb + kb0, x, n, xs);
+      else
+        stage_chunk<B, NR, NRP, KB, (KB * NR * B + 63) / 64>(lane, nb, ja + jb + kb0, x, n, xs);
+      if (act) {
+        for (int tc = t; tc < t1; tc += D * G) {
+#pragma unroll
+          for (int d = 0; d < D; ++d) {
+            const int tt = tc + d * G;
+            if (tt < t1) {
+              double cur[2 * VC];
+#pragma unroll
+              for (int k = 0; k < 2 * VC; ++k) cur[k] = av[d][k];
+              if (tt + D * G < t1) load_vec_cached<2 * VC>(ap + (size_t)(tt + D * G) * BC * 2, av[d]);
+              const int kc = tt - t0, kb = kc / BC, c = kc - kb * BC;
+              const double* xr = xs + (kb * B + c) * NRP;  // real part of x_c, then BC rows on the imaginary
+              const double* xi = xr + BC * NRP;
+              double vr[NRP], vi[NRP];
+#pragma unroll
+              for (int q = 0; q < NRP; ++q) {
+                vr[q] = xr[q];
+                vi[q] = xi[q];
+              }
+#pragma unroll
+              for (int q = 0; q < NR; ++q)
+#pragma unroll
+                for (int k = 0; k < VC; ++k) {
+                  are[q][k] += cur[2 * k] * vr[q] - cur[2 * k + 1] * vi[q];
+                  aim[q][k] += cur[2 * k + 1] * vr[q] + cur[2 * k] * vi[q];
+                }
+            }
+          }
+        }
+        if (t < t1) t += (t1 - t + G - 1) / G * G;
+      }
+    }
+#pragma unroll
+    for (int q0 = 0; q0 < NR; q0 += QF) {
+      wave_lds_sync();
+      if (act) {
+#pragma unroll
+        for (int q = q0; q < q0 + QF && q < NR; ++q)
+#pragma unroll
+          for (int k = 0; k < VC; ++k) {
+            xs[(g * QF + (q - q0)) * B + lr * VC + k] = are[q][k];
+            xs[(g * QF + (q - q0)) * B + BC + lr * VC + k] = aim[q][k];
+          }
+      }
+      wave_lds_sync();
+      if (lane < B) {
+        const size_t e = (size_t)row * B + lane;
+#pragma unroll
+        for (int q = q0; q < q0 + QF && q < NR; ++q) {
+          double sm = 0.0;
+#pragma unroll
+          for (int h = 0; h < G; ++h) sm += xs[(h * QF + (q - q0)) * B + lane];
+          const double v = bsub ? bsub[(size_t)q * n + e] - sm : sm;
+          y[(size_t)q * n + e] = v;
+          if (q == 0 && wdot) dotv += wdot[e] * v;
+        }
+      }
+    }
+  }
+  if (partials) {
+    const double vv[1] = {dotv};
+    block_partials<1>(vv, partials);
+  }
+}
+
+// ---------------------------------------------------------------------------
 // Level-scheduled triangular sweep: one launch per dependency level; every
 // row of a level depends only on rows of earlier levels, which earlier
 // launches on the same stream have completed, so no wave ever waits on
@@ -1078,6 +1213,7 @@ static void dev_bsr_upload_var(const BlockCsr& h, DevBsr& d) {
   d.nzja = h.nzja();
   d.prec = kF64;
   d.nval = h.nza();
+  d.cplx = 0;
   d.ia.upload(h.ia.data(), h.ia.size() * 4);
   d.ja.upload(h.ja.data(), std::max<size_t>(h.ja.size(), 1) * 4);
   d.sp.upload(h.sprnd.data(), h.sprnd.size() * 4);
@@ -1240,6 +1376,7 @@ void dev_bsr_upload_parts(const std::vector<const BlockCsr*>& parts, int prec, D
   d.b = b;
   d.nzja = (int)nzja;
   d.prec = prec;
+  d.cplx = 0;
   const size_t bb = (size_t)b * b, es = prec_bytes(prec);
   std::vector<int> ia(nsup + 1, 0), ja(std::max<int64_t>(nzja, 1), 0);
   d.a.alloc(std::max<size_t>(nzja * bb * es, es));
@@ -1268,6 +1405,50 @@ void dev_bsr_upload(const BlockCsr& h, int prec, DevBsr& d) {
     return;
   }
   dev_bsr_upload_parts({&h}, prec, d);
+}
+
+bool dev_bsr_upload_complex(const BlockCsr& h, DevBsr& d) {
+  const int b = uniform_block(h);
+  const int bc = b / 2;
+  if (b % 2 || !(bc == 3 || bc == 4 || bc == 6 || bc == 8 || bc == 12 || bc == 16)) return false;
+  const int64_t nzja = h.nzja();
+  if ((int64_t)h.nsup * b > INT32_MAX || nzja > INT32_MAX) return false;
+  const size_t bb = (size_t)b * b, cb = (size_t)bc * bc;
+  if (h.a.size() != (size_t)nzja * bb) return false;
+  // every block [[P, -Q], [Q, P]]: complex entry (r, c) = (P(r, c), Q(r, c))
+  std::vector<double> c((size_t)std::max<int64_t>(nzja, 1) * cb * 2, 0.0);
+  std::atomic<bool> ok{true};
+  const int nt = host_threads();
+  parallel_for(nt, nt, [&](int w) {
+    const int64_t j0 = nzja * w / nt, j1 = nzja * (w + 1) / nt;
+    for (int64_t j = j0; j < j1 && ok.load(std::memory_order_relaxed); ++j) {
+      const double* blk = h.a.data() + (size_t)j * bb;
+      double* dst = c.data() + (size_t)j * cb * 2;
+      for (int cc = 0; cc < bc; ++cc)
+        for (int r = 0; r < bc; ++r) {
+          const double p = blk[(size_t)cc * b + r], q = blk[(size_t)cc * b + r + bc];
+          if (blk[(size_t)(cc + bc) * b + r + bc] != p || blk[(size_t)(cc + bc) * b + r] != -q) {
+            ok.store(false, std::memory_order_relaxed);
+            return;
+          }
+          dst[((size_t)cc * bc + r) * 2] = p;
+          dst[((size_t)cc * bc + r) * 2 + 1] = q;
+        }
+    }
+  });
+  if (!ok.load()) return false;
+  std::vector<int> ia(h.ia.begin(), h.ia.end()), ja(h.ja.begin(), h.ja.end());
+  if (ja.empty()) ja.push_back(0);
+  d.nsup = h.nsup;
+  d.n = h.nsup * b;
+  d.b = b;
+  d.nzja = (int)nzja;
+  d.prec = kF64;
+  d.cplx = 1;
+  d.a.upload(c.data(), c.size() * 8);
+  d.ia.upload(ia.data(), ia.size() * 4);
+  d.ja.upload(ja.data(), ja.size() * 4);
+  return true;
 }
 
 // Full storage of a symmetric matrix from its upper blocks: full block k is
@@ -1599,8 +1780,63 @@ static void spmv_nr(const DevBsr& A, const double* x, double* y, const double* b
   }
 }
 
+template <int BC, int NR>
+static void spmv_cx_b(const DevBsr& A, const double* x, double* y, const double* bsub, double* dot_out,
+                      const double* wdot, RedWork& rw, hipStream_t s, const int* skip) {
+  constexpr int kWavesPerBlock = 4;
+  const int nwaves_max = (NR == 1 || dot_out) ? 2048 * kWavesPerBlock : (1 << 30);
+  const int rows_per_wave = (A.nsup + nwaves_max - 1) / nwaves_max;
+  const int nwaves = (A.nsup + rows_per_wave - 1) / rows_per_wave;
+  int grid = (nwaves + kWavesPerBlock - 1) / kWavesPerBlock;
+  if (grid > 8) grid = (grid + 7) / 8 * 8;
+  double* partials = nullptr;
+  if (dot_out) {
+    rw.ensure();
+    if (grid > kRedBlocks * kMaxDots) throw Error(kErrInternal, "spmv: too many blocks for the fused dot");
+    partials = rw.partials.as<double>();
+  }
+  hipLaunchKernelGGL((k_spmv_cx<BC, NR>), dim3(grid), dim3(64 * kWavesPerBlock), 0, s, A.nsup, A.n,
+                     A.ia.as<int>(), A.ja.as<int>(), A.a.as<double>(), x, y, bsub, wdot, partials, rows_per_wave,
+                     skip);
+  BCM_HIP(hipGetLastError());
+  if (dot_out) {
+    rw.pending_blocks = grid;
+    rw.pending_k = 1;
+    if (dot_out != kDeferred) launch_finalize(rw, dot_out, s, skip);
+  }
+}
+
+template <int BC>
+static void spmv_cx_nr(const DevBsr& A, const double* x, double* y, const double* bsub, int nrhs, double* dot_out,
+                       const double* wdot, RedWork& rw, hipStream_t s, const int* skip) {
+  int done = 0;
+  while (done < nrhs) {
+    const int left = nrhs - done;
+    const size_t o = (size_t)done * A.n;
+    const double* bs = bsub ? bsub + o : nullptr;
+    double* dd = done == 0 ? dot_out : nullptr;
+    if (left >= 6) { spmv_cx_b<BC, 6>(A, x + o, y + o, bs, dd, wdot, rw, s, skip); done += 6; }
+    else if (left >= 3) { spmv_cx_b<BC, 3>(A, x + o, y + o, bs, dd, wdot, rw, s, skip); done += 3; }
+    else if (left == 2) { spmv_cx_b<BC, 2>(A, x + o, y + o, bs, dd, wdot, rw, s, skip); done += 2; }
+    else { spmv_cx_b<BC, 1>(A, x + o, y + o, bs, dd, wdot, rw, s, skip); done += 1; }
+  }
+}
+
 void launch_spmv(const DevBsr& A, const double* x, double* y, const double* bsub, int nrhs, double* dot_out,
                  const double* wdot, RedWork& rw, hipStream_t s, const int* skip, RowGate gate) {
+  if (A.cplx) {
+    if (gate.active) throw Error(kErrArg, "row-gated products are not defined for complex matrices");
+    switch (A.b / 2) {
+      case 3: spmv_cx_nr<3>(A, x, y, bsub, nrhs, dot_out, wdot, rw, s, skip); break;
+      case 4: spmv_cx_nr<4>(A, x, y, bsub, nrhs, dot_out, wdot, rw, s, skip); break;
+      case 6: spmv_cx_nr<6>(A, x, y, bsub, nrhs, dot_out, wdot, rw, s, skip); break;
+      case 8: spmv_cx_nr<8>(A, x, y, bsub, nrhs, dot_out, wdot, rw, s, skip); break;
+      case 12: spmv_cx_nr<12>(A, x, y, bsub, nrhs, dot_out, wdot, rw, s, skip); break;
+      case 16: spmv_cx_nr<16>(A, x, y, bsub, nrhs, dot_out, wdot, rw, s, skip); break;
+      default: throw Error(kErrInternal, "complex SpMV: unsupported block size");
+    }
+    return;
+  }
   if (A.b == 0) {
     if (gate.active) throw Error(kErrArg, "row-gated products need a uniform tuned block size");
     launch_spmv_var(A, x, y, bsub, nrhs, s, skip);

@@ -344,7 +344,7 @@ void Matrix::to_device(bool with_transpose) {
     if (symmetric) {
       upload_symmetric(h, d);
       if (with_transpose) { upload_symmetric(h, dt); has_dt = true; }
-    } else {
+    } else if (!(cplx && dev_bsr_upload_complex(h, d))) {
       dev_bsr_upload(h, kF64, d);
     }
     has_d = true;

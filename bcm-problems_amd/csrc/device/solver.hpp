@@ -15,6 +15,7 @@ namespace bcm {
 struct Matrix {
   BlockCsr h;
   bool symmetric = false;
+  bool cplx = false;  // h is the real equivalent of a complex matrix: products read its complex blocks
   DevBsr d;       // full matrix on device
   DevBsr dt;      // transpose (two-sided Lanczos only)
   bool has_d = false, has_dt = false;

@@ -357,6 +357,7 @@ int run_complex(bcm_params* p, const CCallbacks& cb) {
     f.u = bcm::real_equivalent(nsup, sp_new.data(), u.ia.data(), u.ja.data(), u.a.data(), false);
     bcm::Matrix mat;
     mat.symmetric = false;
+    mat.cplx = true;
     mat.h = bcm::real_equivalent(nsup, sp_new.data(), co.ia.data(), co.ja.data(), co.a.data(), false);
     prm.timefct = f.stats.seconds;
     prm.density = f.stats.densu;
