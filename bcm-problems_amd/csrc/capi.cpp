@@ -246,6 +246,7 @@ int bcm_cfactor_create(int nsup, const int* sprnd, const int* lia, const int* lj
     auto* h = new bcm_factor_s;
     try {
       h->f.kind = 1;
+      h->f.cplx = true;
       h->f.l = bcm::real_equivalent(nsup, sprnd, lia, lja, la, true);
       h->f.u = bcm::real_equivalent(nsup, sprnd, uia, uja, ua, false);
       for (int i = 0; i < nsup; ++i) {
@@ -290,6 +291,7 @@ int bcm_cfactor_ilu2(int nsup, const int* sprnd, const int* ia, const int* ja, c
     auto* h = new bcm_factor_s;
     try {
       h->f.kind = 1;
+      h->f.cplx = true;
       h->f.l = bcm::real_equivalent(nsup, sprnd, l.ia.data(), l.ja.data(), l.a.data(), true);
       h->f.u = bcm::real_equivalent(nsup, sprnd, u.ia.data(), u.ja.data(), u.a.data(), false);
       h->f.stats = stats;

@@ -117,6 +117,8 @@ struct DevTri {
   bool forward = true;
   int prec = kF64;
   int levels = 0;
+  int cplx = 0;  // 1: real equivalent of a complex factor, off/diag holding its complex (b/2)^2 blocks
+  DevBuf lctl;   // complex factors: control words of the per-level launches of the complex sweep kernel
   std::vector<int> level_ptr;  // host: rows of level l are level_rows[level_ptr[l] .. level_ptr[l+1])
   DevBuf level_rows;
   DevBuf ia, ja;      // dependency lists per row (row index space of the sweep)
@@ -125,6 +127,7 @@ struct DevTri {
   DevBuf sp, doff, dgo;  // variable blocks: partition, value offsets of dependency and diagonal blocks
   int64_t nval = 0;
   int64_t value_bytes() const {
+    if (cplx) return ((int64_t)ndep + nsup) * b * b / 2 * 8;
     return (b ? ((int64_t)ndep + nsup) * b * b : nval) * (int64_t)prec_bytes(prec);
   }
   int64_t index_bytes() const { return 4LL * (ndep + nsup + 1) + (b ? 0 : 8LL * (ndep + nsup) + 4LL * (nsup + 1)); }
@@ -136,10 +139,14 @@ bool fast_block_size(int b);
 
 // Forward sweep through a factor stored by columns (reference SolveL):
 // y_k = D_k^T (x_k - sum_{i<k} F_ik^T y_i).
-void dev_tri_forward(const BlockCsr& f, int prec, DevTri& t);
+void dev_tri_forward(const BlockCsr& f, int prec, DevTri& t, bool cplx = false);
 // Backward sweep through a factor stored by rows (reference SolveU):
 // z_i = D_i (x_i - sum_{j>i} F_ij z_j).
-void dev_tri_backward(const BlockCsr& f, int prec, DevTri& t);
+void dev_tri_backward(const BlockCsr& f, int prec, DevTri& t, bool cplx = false);
+// cplx: f is the real equivalent of a complex factor (every block
+// [[P, -Q], [Q, P]]); FP64 images of block sizes 6, 8, 12, 16, 24, 32 then
+// store the complex blocks (P, Q) and sweep them with the complex kernel.
+
 // The same sweeps through the block-diagonal union of several factors.
 void dev_tri_forward_parts(const std::vector<const BlockCsr*>& f, int prec, DevTri& t);
 void dev_tri_backward_parts(const std::vector<const BlockCsr*>& f, int prec, DevTri& t);

@@ -848,6 +848,139 @@ __global__ __launch_bounds__(64) void k_sweep_free(int nsup, int n, int forward,
   }
 }
 
+// The sync-free sweep of a complex factor (DevTri::cplx): factor blocks are
+// complex BC x BC, (re, im) pairs column-major, vectors real-equivalent
+// ([Re z; Im z] per block row). The protocol -- row order, sentinel
+// completion, bounded waits, dependency staging -- is k_sweep_free's; lane
+// (r, g) multiplies complex row r of the columns g, g + G, ... of the
+// dependency stream (one 16-byte load each), then of the diagonal block:
+//   v = x_row - sum_k F_k y_k,  y_row = D v,  (P + iQ)(a + ib) = (Pa - Qb) + i(Qa + Pb).
+// With order given and every dependency complete before the launch (one
+// launch per level) no wait ever spins.
+__device__ __forceinline__ VecOf<double, 2>::T ldnt_c(const double* p) {
+  return __builtin_nontemporal_load(reinterpret_cast<const VecOf<double, 2>::T*>(p));
+}
+
+template <int BC, int NR>
+__global__ __launch_bounds__(64) void k_sweep_free_cx(int nsup, int n, int forward, const int* __restrict__ order,
+                                                      const int* __restrict__ ia, const int* __restrict__ ja,
+                                                      const double* __restrict__ off,
+                                                      const double* __restrict__ diag,
+                                                      const double* __restrict__ x, double* y, int* ctl,
+                                                      const int* skip) {
+  if (skip && *skip) return;
+  using C2 = VecOf<double, 2>::T;
+  constexpr int B = 2 * BC;                   // real rows of a block row
+  constexpr int G = 64 / BC;
+  constexpr int KD = 3;                       // dependency blocks per staged chunk
+  constexpr int CK = (KD * BC + G - 1) / G;   // complex columns of a chunk per lane
+  constexpr int PD = (BC + G - 1) / G;        // diagonal-block columns per lane
+  __shared__ double v[NR][B];
+  __shared__ double ys[KD * B * NR];
+  const int lane = threadIdx.x;
+  const int r = lane % BC, g = lane / BC;
+  const bool act = g < G;
+  const C2 zero2 = {0.0, 0.0};
+  for (int t = blockIdx.x; t < nsup; t += gridDim.x) {
+    const int row = order ? order[t] : (forward ? t : nsup - 1 - t);
+    const int jb = ia[row], ndep = ia[row + 1] - jb;
+    const double* blk0 = off + (size_t)jb * BC * BC * 2;
+    const double* d = diag + (size_t)row * BC * BC * 2;
+    C2 pd[PD];
+#pragma unroll
+    for (int i = 0; i < PD; ++i) pd[i] = (act && g + i * G < BC) ? ldnt_c(d + ((g + i * G) * BC + r) * 2) : zero2;
+    double xr[NR], xi[NR];
+#pragma unroll
+    for (int q = 0; q < NR; ++q) {
+      xr[q] = lane < BC ? x[(size_t)q * n + (size_t)row * B + lane] : 0.0;
+      xi[q] = lane < BC ? x[(size_t)q * n + (size_t)row * B + BC + lane] : 0.0;
+    }
+    double ar[NR], ai[NR];
+#pragma unroll
+    for (int q = 0; q < NR; ++q) {
+      ar[q] = 0.0;
+      ai[q] = 0.0;
+    }
+    for (int kb0 = 0; kb0 < ndep; kb0 += KD) {
+      const int nb = min(KD, ndep - kb0);
+      const int c0 = kb0 * BC, c1 = (kb0 + nb) * BC;
+      const int tf = c0 + ((g - c0 % G) % G + G) % G;
+      C2 av[CK];
+#pragma unroll
+      for (int i = 0; i < CK; ++i) {
+        const int tt = tf + i * G;
+        av[i] = (act && tt < c1) ? ldnt_c(blk0 + ((size_t)tt * BC + r) * 2) : zero2;
+      }
+      stage_deps<B, NR, KD>(kb0, nb, jb, ja, y, n, ys, ctl);
+      __syncthreads();
+      if (act) {
+#pragma unroll
+        for (int i = 0; i < CK; ++i) {
+          const int tt = tf + i * G;
+          if (tt < c1) {
+            const int k = tt / BC - kb0, c = tt % BC;
+#pragma unroll
+            for (int q = 0; q < NR; ++q) {
+              const double yr = ys[(k * NR + q) * B + c], yi = ys[(k * NR + q) * B + BC + c];
+              ar[q] += av[i].x * yr - av[i].y * yi;
+              ai[q] += av[i].y * yr + av[i].x * yi;
+            }
+          }
+        }
+      }
+      __syncthreads();
+    }
+#pragma unroll
+    for (int q = 0; q < NR; ++q) {
+      double sr = ar[q], si = ai[q];
+#pragma unroll
+      for (int h = 1; h < G; ++h) {
+        sr += __shfl(ar[q], lane + h * BC);
+        si += __shfl(ai[q], lane + h * BC);
+      }
+      if (lane < BC) {
+        v[q][lane] = xr[q] - sr;
+        v[q][BC + lane] = xi[q] - si;
+      }
+    }
+    __syncthreads();
+    double zr[NR], zi[NR];
+#pragma unroll
+    for (int q = 0; q < NR; ++q) {
+      zr[q] = 0.0;
+      zi[q] = 0.0;
+    }
+    if (act) {
+#pragma unroll
+      for (int i = 0; i < PD; ++i) {
+        const int c = g + i * G;
+        if (c < BC) {
+#pragma unroll
+          for (int q = 0; q < NR; ++q) {
+            const double vr = v[q][c], vi = v[q][BC + c];
+            zr[q] += pd[i].x * vr - pd[i].y * vi;
+            zi[q] += pd[i].y * vr + pd[i].x * vi;
+          }
+        }
+      }
+    }
+#pragma unroll
+    for (int q = 0; q < NR; ++q) {
+      double sr = zr[q], si = zi[q];
+#pragma unroll
+      for (int h = 1; h < G; ++h) {
+        sr += __shfl(zr[q], lane + h * BC);
+        si += __shfl(zi[q], lane + h * BC);
+      }
+      if (lane < BC) {
+        st_coherent(&y[(size_t)q * n + (size_t)row * B + lane], sr);
+        st_coherent(&y[(size_t)q * n + (size_t)row * B + BC + lane], si);
+      }
+    }
+    __syncthreads();
+  }
+}
+
 // ---------------------------------------------------------------------------
 // BLAS-1
 
@@ -1231,6 +1364,7 @@ static void build_tri_var(const BlockCsr& f, bool forward, int prec, DevTri& t) 
   t.nsup = ns;
   t.n = f.n();
   t.b = 0;
+  t.cplx = 0;
   t.forward = forward;
   t.prec = prec;
   t.ndep = f.nzja() - ns;
@@ -1566,9 +1700,40 @@ bool dev_bsr_upload_symm_parts(const std::vector<const BlockCsr*>& uppers, DevBs
 // Diagonal blocks are stored transposed for the forward sweep. Dependency
 // levels (a row's level is one more than the deepest row it depends on) are
 // merged over the parts for the level-scheduled launches.
-static void build_tri_parts(const std::vector<const BlockCsr*>& parts, bool forward, int prec, DevTri& t) {
+// Real-equivalent blocks [[P, -Q], [Q, P]] (b = 2 bc, column-major) to complex
+// bc x bc blocks of (P, Q) pairs, column-major; false if a block lacks the
+// structure.
+static bool complex_blocks(const std::vector<double>& src, int b, std::vector<double>& dst) {
+  const int bc = b / 2;
+  const size_t bb = (size_t)b * b, nblk = src.size() / bb;
+  dst.assign(nblk * bc * bc * 2, 0.0);
+  for (size_t k = 0; k < nblk; ++k) {
+    const double* blk = src.data() + k * bb;
+    double* o = dst.data() + k * bc * bc * 2;
+    for (int c = 0; c < bc; ++c)
+      for (int r = 0; r < bc; ++r) {
+        const double p = blk[(size_t)c * b + r], q = blk[(size_t)c * b + r + bc];
+        if (blk[(size_t)(c + bc) * b + r + bc] != p || blk[(size_t)(c + bc) * b + r] != -q) return false;
+        o[((size_t)c * bc + r) * 2] = p;
+        o[((size_t)c * bc + r) * 2 + 1] = q;
+      }
+  }
+  return true;
+}
+
+static bool complex_sweep_block(int b) {
+  return b == 6 || b == 8 || b == 12 || b == 16 || b == 24 || b == 32;
+}
+
+// cplx: the parts are real equivalents of complex factors; their blocks are
+// stored complex (half the bytes) for k_sweep_free_cx when every block has
+// the structure, else the image is built real.
+static void build_tri_parts(const std::vector<const BlockCsr*>& parts, bool forward, int prec, DevTri& t,
+                            bool cplx = false) {
   const int b = parts_block(parts, "device factors");
+  if (cplx && (prec != kF64 || !complex_sweep_block(b))) cplx = false;
   const size_t bb = (size_t)b * b, es = prec_bytes(prec);
+  const size_t vb = cplx ? bb / 2 : bb;  // stored doubles per block
   int64_t nsup = 0, ndep = 0;
   for (const BlockCsr* f : parts) { nsup += f->nsup; ndep += f->nzja() - f->nsup; }
   if (nsup * b > INT32_MAX || ndep > INT32_MAX) throw Error(kErrArg, "device factor exceeds 32-bit indexing");
@@ -1578,9 +1743,11 @@ static void build_tri_parts(const std::vector<const BlockCsr*>& parts, bool forw
   t.forward = forward;
   t.prec = prec;
   t.ndep = (int)ndep;
+  t.cplx = cplx ? 1 : 0;
   std::vector<int> ia_all(nsup + 1, 0), ja_all(std::max<int64_t>(ndep, 1), 0), lev_all(nsup, 0);
-  t.off.alloc(std::max<size_t>(ndep * bb * es, es));
-  t.diag.alloc(nsup * bb * es);
+  t.off.alloc(std::max<size_t>(ndep * vb * es, es));
+  t.diag.alloc(nsup * vb * es);
+  std::atomic<bool> cx_ok{true};
   const int np = (int)parts.size();
   std::vector<int64_t> rs(np + 1, 0), ds(np + 1, 0);  // first row and dependency block of each part
   for (int q = 0; q < np; ++q) {
@@ -1642,10 +1809,29 @@ static void build_tri_parts(const std::vector<const BlockCsr*>& parts, bool forw
     if ((int64_t)ja.size() != ds[q + 1] - d0) throw Error(kErrInternal, "factor image: dependency count");
     for (int i = 0; i < ns; ++i) ia_all[r0 + i + 1] = (int)(d0 + ia[i + 1]);
     for (size_t j = 0; j < ja.size(); ++j) ja_all[d0 + j] = (int)(r0 + ja[j]);
-    upload_values_at(off.data(), off.size(), prec, static_cast<char*>(t.off.p) + (size_t)d0 * bb * es);
-    upload_values_at(diag.data(), diag.size(), prec, static_cast<char*>(t.diag.p) + (size_t)r0 * bb * es);
+    if (cplx) {
+      std::vector<double> co, cd;
+      if (!complex_blocks(off, b, co) || !complex_blocks(diag, b, cd)) {
+        cx_ok.store(false);
+      } else {
+        off.swap(co);
+        diag.swap(cd);
+      }
+    }
+    if (!cplx || cx_ok.load()) {
+      upload_values_at(off.data(), off.size(), prec, static_cast<char*>(t.off.p) + (size_t)d0 * vb * es);
+      upload_values_at(diag.data(), diag.size(), prec, static_cast<char*>(t.diag.p) + (size_t)r0 * vb * es);
+    }
     part_mx[q] = mx;
   });
+  if (cplx && !cx_ok.load()) {  // a block without the complex structure: the real image
+    build_tri_parts(parts, forward, prec, t, false);
+    return;
+  }
+  if (cplx) {
+    const int zeros[16] = {};
+    t.lctl.upload(zeros, sizeof(zeros));
+  }
   const int mx = np ? *std::max_element(part_mx.begin(), part_mx.end()) : 0;
   t.levels = nsup ? mx + 1 : 0;
   t.level_ptr.assign(t.levels + 1, 0);
@@ -1664,13 +1850,13 @@ static void build_tri_parts(const std::vector<const BlockCsr*>& parts, bool forw
   t.ja.upload(ja_all.data(), ja_all.size() * 4);
 }
 
-void dev_tri_forward(const BlockCsr& f, int prec, DevTri& t) {
+void dev_tri_forward(const BlockCsr& f, int prec, DevTri& t, bool cplx) {
   if (!fast_block_size(uniform_block(f))) build_tri_var(f, true, prec, t);
-  else build_tri_parts({&f}, true, prec, t);
+  else build_tri_parts({&f}, true, prec, t, cplx);
 }
-void dev_tri_backward(const BlockCsr& f, int prec, DevTri& t) {
+void dev_tri_backward(const BlockCsr& f, int prec, DevTri& t, bool cplx) {
   if (!fast_block_size(uniform_block(f))) build_tri_var(f, false, prec, t);
-  else build_tri_parts({&f}, false, prec, t);
+  else build_tri_parts({&f}, false, prec, t, cplx);
 }
 void dev_tri_forward_parts(const std::vector<const BlockCsr*>& f, int prec, DevTri& t) {
   build_tri_parts(f, true, prec, t);
@@ -1928,9 +2114,79 @@ static void sweep_nr(const DevTri& T, const void* x, void* y, int vec_prec, int 
   }
 }
 
+template <int BC, int NR>
+static void sweep_cx_b(const DevTri& T, const double* x, double* y, hipStream_t s, const int* skip,
+                       SweepSync* sync) {
+  auto kern = k_sweep_free_cx<BC, NR>;
+  int n = T.n, fwd = T.forward ? 1 : 0;
+  const int* ia = T.ia.as<int>();
+  const int* ja = T.ja.as<int>();
+  const double* off = T.off.as<double>();
+  const double* dg = T.diag.as<double>();
+  if (sync) {  // as the real sync-free sweep: every wave resident, y filled with the sentinel first
+    static const int per_cu = [&] {
+      int v = 0;
+      BCM_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&v, kern, 64, 0));
+      return v;
+    }();
+    const int waves = std::max(1, std::min(T.nsup, std::min(per_cu, sync->waves_per_cu) * device_cu_count()));
+    const int64_t ny = (int64_t)T.n * NR;
+    hipLaunchKernelGGL((k_sweep_fill<double>), dim3(std::max<int64_t>(1, std::min<int64_t>((ny + 255) / 256, 2048))),
+                       dim3(256), 0, s, ny, y, skip);
+    int nsup = T.nsup;
+    int* ct = sync->ext_ctl ? sync->ext_ctl : sync->ctl.as<int>();
+    const int* ord = sync->level_order ? T.level_rows.as<int>() : nullptr;
+    void* args[] = {&nsup, &n, &fwd, &ord, &ia, &ja, &off, &dg, &x, &y, &ct, (void*)&skip};
+    if (sync->cooperative)
+      BCM_HIP(hipLaunchCooperativeKernel(reinterpret_cast<const void*>(kern), dim3(waves), dim3(64), args, 0, s));
+    else
+      BCM_HIP(hipLaunchKernel(reinterpret_cast<const void*>(kern), dim3(waves), dim3(64), args, 0, s));
+    return;
+  }
+  // level by level: a launch's rows depend only on rows of earlier launches, so its waits never spin
+  const int* rows = T.level_rows.as<int>();
+  int* ct = T.lctl.as<int>();
+  for (int l = 0; l < T.levels; ++l) {
+    const int r0 = T.level_ptr[l], nr = T.level_ptr[l + 1] - r0;
+    hipLaunchKernelGGL(kern, dim3(nr), dim3(64), 0, s, nr, n, fwd, rows + r0, ia, ja, off, dg, x, y, ct, skip);
+  }
+  BCM_HIP(hipGetLastError());
+}
+
+template <int BC>
+static void sweep_cx_nr(const DevTri& T, const double* x, double* y, int nrhs, hipStream_t s, const int* skip,
+                        SweepSync* sync) {
+  int done = 0;
+  while (done < nrhs) {
+    const int left = nrhs - done;
+    const size_t o = (size_t)done * T.n;
+    if (left >= 6) { sweep_cx_b<BC, 6>(T, x + o, y + o, s, skip, sync); done += 6; }
+    else if (left >= 3) { sweep_cx_b<BC, 3>(T, x + o, y + o, s, skip, sync); done += 3; }
+    else if (left == 2) { sweep_cx_b<BC, 2>(T, x + o, y + o, s, skip, sync); done += 2; }
+    else { sweep_cx_b<BC, 1>(T, x + o, y + o, s, skip, sync); done += 1; }
+  }
+}
+
 void launch_sweep(const DevTri& T, const void* x, void* y, int vec_prec, int nrhs, hipStream_t s,
                   const int* skip, SweepSync* sync, RowGate gate) {
   if (x == y) throw Error(kErrArg, "triangular sweep: input and output must not alias");
+  if (T.cplx) {
+    if (gate.active) throw Error(kErrArg, "row-gated sweeps are not defined for complex factors");
+    if (vec_prec != kF64) throw Error(kErrArg, "complex factors sweep FP64 vectors");
+    if (sync) sync->ensure(T.nsup, s);
+    const double* xd = static_cast<const double*>(x);
+    double* yd = static_cast<double*>(y);
+    switch (T.b / 2) {
+      case 3: sweep_cx_nr<3>(T, xd, yd, nrhs, s, skip, sync); break;
+      case 4: sweep_cx_nr<4>(T, xd, yd, nrhs, s, skip, sync); break;
+      case 6: sweep_cx_nr<6>(T, xd, yd, nrhs, s, skip, sync); break;
+      case 8: sweep_cx_nr<8>(T, xd, yd, nrhs, s, skip, sync); break;
+      case 12: sweep_cx_nr<12>(T, xd, yd, nrhs, s, skip, sync); break;
+      case 16: sweep_cx_nr<16>(T, xd, yd, nrhs, s, skip, sync); break;
+      default: throw Error(kErrInternal, "complex sweep: unsupported block size");
+    }
+    return;
+  }
   if (T.b == 0) {  // variable blocks: level by level whatever the schedule
     if (gate.active) throw Error(kErrArg, "row-gated sweeps need a uniform tuned block size");
     launch_sweep_var(T, x, y, vec_prec, nrhs, s, skip);

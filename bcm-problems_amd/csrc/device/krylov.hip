@@ -360,8 +360,8 @@ const Factor::Image& Factor::to_device(int prec, bool with_transpose) {
   if (prec < kF64 || prec > kBF16) throw Error(kErrArg, "unknown factor precision");
   Image& im = img[prec];
   if (!im.has) {
-    dev_tri_forward(lower(), prec, im.fwd);
-    dev_tri_backward(u, prec, im.bwd);
+    dev_tri_forward(lower(), prec, im.fwd, cplx);
+    dev_tri_backward(u, prec, im.bwd, cplx);
     im.has = true;
   }
   if (with_transpose && !im.has_t) {
@@ -369,8 +369,8 @@ const Factor::Image& Factor::to_device(int prec, bool with_transpose) {
       dev_tri_forward(u, prec, im.fwdt);
       dev_tri_backward(u, prec, im.bwdt);
     } else {
-      dev_tri_forward(u, prec, im.fwdt);
-      dev_tri_backward(l, prec, im.bwdt);
+      dev_tri_forward(u, prec, im.fwdt, cplx);
+      dev_tri_backward(l, prec, im.bwdt, cplx);
     }
     im.has_t = true;
   }

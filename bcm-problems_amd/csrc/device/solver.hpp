@@ -26,6 +26,7 @@ struct Matrix {
 // sweeps of M^-1 (fwd then bwd) and, on request, of M^-T (fwdt then bwdt).
 struct Factor {
   int kind = 0;            // 0 IC2: A ~ U^T U (U only); 1 ILU2: A ~ L U
+  bool cplx = false;       // l, u are real equivalents of complex factors: FP64 sweeps read their complex blocks
   BlockCsr l, u;           // L by columns, U by rows, inverted diagonal blocks
   FactorStats stats;
   std::vector<int> order;  // ordering the factor was computed in (order[old] = new), may be empty

@@ -352,6 +352,7 @@ int run_complex(bcm_params* p, const CCallbacks& cb) {
     bcm::combine_lu_complex(co, al, au);
     bcm::Factor f;
     f.kind = 1;
+    f.cplx = true;
     bcm::ilu2_complex(al, au, slv, l, u, &f.stats);
     f.l = bcm::real_equivalent(nsup, sp_new.data(), l.ia.data(), l.ja.data(), l.a.data(), true);
     f.u = bcm::real_equivalent(nsup, sp_new.data(), u.ia.data(), u.ja.data(), u.a.data(), false);

@@ -59,16 +59,20 @@ def test_complex_operators_match_reference(B, O, ctx, csys, nrhs):
 
 
 @pytest.mark.parametrize("bc", [3, 4, 6, 8, 16])
-def test_complex_block_spmv_every_block_size(B, O, ctx, bc):
-    """The complex-block SpMV (complex blocks of 3..16 read as (re, im)
-    pairs) at 1, 2, 6 and 7 columns (column chunks of 1, 2, 3 and 6)
-    against the reference's complex MvmA, and y = b - A x (the residual form
-    the solvers launch) through bcm_residual."""
+def test_complex_block_kernels_every_block_size(B, O, ctx, bc):
+    """The complex-block SpMV and sweeps (complex blocks of 3..16 read as
+    (re, im) pairs) at 1, 2, 6 and 7 columns (column chunks of 1, 2, 3 and
+    6): products against the reference's complex MvmA, y = b - A x (the
+    residual form the solvers launch) through bcm_residual, and both sweep
+    schedules against the reference's complex SolveL / SolveU on the
+    reference's own complex ILU2 factors."""
     sp, ia, ja, ac = complex_system(B, 4, 3, 3, bc, seed=40 + bc)
     mc = B.Matrix.from_complex(sp, ia, ja, ac)
+    L, U = O.ref_cilu2(sp, ia, ja, ac)
+    f = B.Factor.from_complex(sp, L, U)
     n = sp[-1]
     for nrhs in (1, 2, 6, 7):
-        print(f"complex spmv bc={bc} n={n} nrhs={nrhs}", flush=True)
+        print(f"complex kernels bc={bc} n={n} nrhs={nrhs}", flush=True)
         rng = np.random.default_rng(bc * 10 + nrhs)
         z = rng.standard_normal((n, nrhs)) + 1j * rng.standard_normal((n, nrhs))
         dx = dev(B, B.complex_to_real(sp, z))
@@ -83,6 +87,15 @@ def test_complex_block_spmv_every_block_size(B, O, ctx, bc):
         ctx.sync()
         r = B.real_to_complex(sp, dy.host())
         assert np.abs(r - (w - yr)).max() <= 1e-13 * np.abs(yr).max(), nrhs
+        for mode in (1, 0):
+            ctx.set_sweep_mode(mode)
+            for which, fn, part in ((0, ctx.solve_l, L), (1, ctx.solve_u, U)):
+                fn(f, which, dx, dy, nrhs)
+                ctx.sync()
+                y = B.real_to_complex(sp, dy.host())
+                yr = O.ref_csolve(which, sp, *part, z, nrhs)
+                assert np.abs(y - yr).max() <= 1e-13 * np.abs(yr).max(), (nrhs, mode, which)
+        ctx.set_sweep_mode(1)
 
 
 @pytest.mark.parametrize("kind", ["BICGSTAB", "FGMRES", "BLOCKGMRES"])
