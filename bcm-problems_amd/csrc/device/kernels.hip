@@ -755,6 +755,15 @@ __device__ __forceinline__ void stage_deps(int kb0, int nb, int jb, const int* _
     if (src[j] >= 0) ys[lane + 64 * j] = val[j];
 }
 
+// Rows per lane of the sync-free sweep: two (one 16-byte load per factor
+// column for FP64) for even blocks of 12 and more, so that more lanes work
+// and fewer registers hold the columns in flight; one for small blocks, whose
+// lane groups would otherwise grow the cross-group sums.
+template <int B>
+constexpr int sweep_free_width() {
+  return (B % 2 == 0 && B >= 12) ? 2 : 1;
+}
+
 template <int B, int NR, class TA, class TX>
 __global__ __launch_bounds__(64) void k_sweep_free(int nsup, int n, int forward, const int* __restrict__ order,
                                                    const int* __restrict__ ia,
@@ -763,30 +772,44 @@ __global__ __launch_bounds__(64) void k_sweep_free(int nsup, int n, int forward,
                                                    int* ctl, const int* skip) {
   if (skip && *skip) return;
   using Acc = typename AccOf<TX>::type;
-  constexpr int G = 64 / B;
+  constexpr int V = sweep_free_width<B>();  // consecutive rows per lane (one vector load per column)
+  constexpr int L = B / V;                  // lanes per column
+  constexpr int G = 64 / L;                 // column groups
   constexpr int KD = 3;                     // dependency blocks per staged chunk (one round trip for most rows)
   constexpr int CK = (KD * B + G - 1) / G;  // columns of a chunk per lane
   constexpr int PD = (B + G - 1) / G;       // diagonal-block columns per lane
   __shared__ Acc v[NR][B];
   __shared__ TX ys[KD * B * NR];
   const int lane = threadIdx.x;
-  const int r = lane % B, g = lane / B;
+  const int lr = lane % L, g = lane / L;
   const bool act = g < G;
   for (int t = blockIdx.x; t < nsup; t += gridDim.x) {
     const int row = order ? order[t] : (forward ? t : nsup - 1 - t);
     const int jb = ia[row], ndep = ia[row + 1] - jb;
-    const TA* blk0 = off + (size_t)jb * B * B;
-    const TA* d = diag + (size_t)row * B * B;
+    const TA* blk0 = off + (size_t)jb * B * B + lr * V;
+    const TA* d = diag + (size_t)row * B * B + lr * V;
     // the diagonal block and the right-hand side do not depend on the wait
-    Acc pd[PD];
+    Acc pd[PD][V];
 #pragma unroll
-    for (int i = 0; i < PD; ++i) pd[i] = (act && g + i * G < B) ? (Acc)ldnt(d + (g + i * G) * B + r) : Acc(0);
-    Acc xv[NR];
+    for (int i = 0; i < PD; ++i) {
+      if (act && g + i * G < B) {
+        load_vec<V>(d + (g + i * G) * B, pd[i]);
+      } else {
 #pragma unroll
-    for (int q = 0; q < NR; ++q) xv[q] = lane < B ? (Acc)x[(size_t)q * n + (size_t)row * B + lane] : Acc(0);
-    Acc acc[NR];
+        for (int k = 0; k < V; ++k) pd[i][k] = Acc(0);
+      }
+    }
+    Acc xv[NR][V];
 #pragma unroll
-    for (int q = 0; q < NR; ++q) acc[q] = Acc(0);
+    for (int q = 0; q < NR; ++q)
+#pragma unroll
+      for (int k = 0; k < V; ++k)
+        xv[q][k] = lane < L ? (Acc)x[(size_t)q * n + (size_t)row * B + lr * V + k] : Acc(0);
+    Acc acc[NR][V];
+#pragma unroll
+    for (int q = 0; q < NR; ++q)
+#pragma unroll
+      for (int k = 0; k < V; ++k) acc[q][k] = Acc(0);
     // Chunks of KD dependency blocks. Column tt of the row's block stream
     // (block tt / B, column tt % B) starts at blk0 + tt * B; the lane group g
     // multiplies the columns tt = g (mod G), in ascending order.
@@ -795,11 +818,16 @@ __global__ __launch_bounds__(64) void k_sweep_free(int nsup, int n, int forward,
       const int c0 = kb0 * B, c1 = (kb0 + nb) * B;
       const int tf = c0 + ((g - c0 % G) % G + G) % G;
       // the chunk's factor values do not depend on the wait: in flight first
-      Acc av[CK];
+      Acc av[CK][V];
 #pragma unroll
       for (int i = 0; i < CK; ++i) {
         const int tt = tf + i * G;
-        av[i] = (act && tt < c1) ? (Acc)ldnt(blk0 + (size_t)tt * B + r) : Acc(0);
+        if (act && tt < c1) {
+          load_vec<V>(blk0 + (size_t)tt * B, av[i]);
+        } else {
+#pragma unroll
+          for (int k = 0; k < V; ++k) av[i][k] = Acc(0);
+        }
       }
       stage_deps<B, NR, KD>(kb0, nb, jb, ja, y, n, ys, ctl);
       __syncthreads();
@@ -810,40 +838,54 @@ __global__ __launch_bounds__(64) void k_sweep_free(int nsup, int n, int forward,
           if (tt < c1) {
             const int k = tt / B - kb0, c = tt % B;
 #pragma unroll
-            for (int q = 0; q < NR; ++q) acc[q] += av[i] * (Acc)ys[(k * NR + q) * B + c];
+            for (int q = 0; q < NR; ++q) {
+              const Acc yv = (Acc)ys[(k * NR + q) * B + c];
+#pragma unroll
+              for (int e = 0; e < V; ++e) acc[q][e] += av[i][e] * yv;
+            }
           }
         }
       }
       __syncthreads();
     }
 #pragma unroll
-    for (int q = 0; q < NR; ++q) {
-      Acc s = acc[q];
+    for (int q = 0; q < NR; ++q)
 #pragma unroll
-      for (int h = 1; h < G; ++h) s += __shfl(acc[q], lane + h * B);
-      if (lane < B) v[q][lane] = xv[q] - s;
-    }
+      for (int k = 0; k < V; ++k) {
+        Acc s = acc[q][k];
+#pragma unroll
+        for (int h = 1; h < G; ++h) s += __shfl(acc[q][k], lane + h * L);
+        if (lane < L) v[q][lr * V + k] = xv[q][k] - s;
+      }
     __syncthreads();
-    Acc z[NR];
+    Acc z[NR][V];
 #pragma unroll
-    for (int q = 0; q < NR; ++q) z[q] = Acc(0);
+    for (int q = 0; q < NR; ++q)
+#pragma unroll
+      for (int k = 0; k < V; ++k) z[q][k] = Acc(0);
     if (act) {
 #pragma unroll
       for (int i = 0; i < PD; ++i) {
         const int c = g + i * G;
         if (c < B) {
 #pragma unroll
-          for (int q = 0; q < NR; ++q) z[q] += pd[i] * v[q][c];
+          for (int q = 0; q < NR; ++q) {
+            const Acc vc = v[q][c];
+#pragma unroll
+            for (int k = 0; k < V; ++k) z[q][k] += pd[i][k] * vc;
+          }
         }
       }
     }
 #pragma unroll
-    for (int q = 0; q < NR; ++q) {
-      Acc s = z[q];
+    for (int q = 0; q < NR; ++q)
 #pragma unroll
-      for (int h = 1; h < G; ++h) s += __shfl(z[q], lane + h * B);
-      if (lane < B) st_coherent(&y[(size_t)q * n + (size_t)row * B + lane], (TX)s);
-    }
+      for (int k = 0; k < V; ++k) {
+        Acc s = z[q][k];
+#pragma unroll
+        for (int h = 1; h < G; ++h) s += __shfl(z[q][k], lane + h * L);
+        if (lane < L) st_coherent(&y[(size_t)q * n + (size_t)row * B + lr * V + k], (TX)s);
+      }
     __syncthreads();
   }
 }
