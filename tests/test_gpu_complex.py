@@ -87,7 +87,7 @@ def test_complex_block_kernels_every_block_size(B, O, ctx, bc):
         ctx.sync()
         r = B.real_to_complex(sp, dy.host())
         assert np.abs(r - (w - yr)).max() <= 1e-13 * np.abs(yr).max(), nrhs
-        for mode in (1, 0):
+        for mode in (1, 2, 0):  # level-ordered sync-free, natural-order sync-free, one launch per level
             ctx.set_sweep_mode(mode)
             for which, fn, part in ((0, ctx.solve_l, L), (1, ctx.solve_u, U)):
                 fn(f, which, dx, dy, nrhs)
@@ -96,6 +96,12 @@ def test_complex_block_kernels_every_block_size(B, O, ctx, bc):
                 yr = O.ref_csolve(which, sp, *part, z, nrhs)
                 assert np.abs(y - yr).max() <= 1e-13 * np.abs(yr).max(), (nrhs, mode, which)
         ctx.set_sweep_mode(1)
+        # the allocation-free preconditioner shares the complex sweeps: bit for bit
+        dp, dq = B.DeviceArray((2 * n, nrhs)), B.DeviceArray((2 * n, nrhs))
+        ctx.precond(f, dx, dp, nrhs)
+        ctx.precond_noalloc(f, dx, dq, nrhs)
+        ctx.sync()
+        assert np.array_equal(dp.host(), dq.host()), nrhs
 
 
 @pytest.mark.parametrize("kind", ["BICGSTAB", "FGMRES", "BLOCKGMRES"])
