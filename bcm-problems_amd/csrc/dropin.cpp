@@ -162,6 +162,7 @@ void pick_solver(bool symmetric, bool par, const bcm_params& prm, bcm::SolveOpti
     } else {
       opt.kind = bcm::kBlockGmres;  // gmres.cpp:742-750: sttype 0 absolute, else relative to ||r0||
       opt.ittype = prm.sttype == 0 ? 1 : 0;
+      opt.check_every = 1;  // a block GMRES step costs far more than a host poll of the done flag
     }
     return;
   }
@@ -389,6 +390,7 @@ int run_complex(bcm_params* p, const CCallbacks& cb) {
     opt.eps = prm.eps;
     opt.ittype = 0;  // gmres.cpp:1308-1313: every column's ||r|| < eps ||r0||
     opt.max_restarts = 0;
+    opt.check_every = 1;  // a block GMRES step costs far more than a host poll of the done flag
     bcm::SolveStats st;
     double resi[2] = {0.0, 0.0};
     auto t0 = std::chrono::steady_clock::now();
